@@ -1,0 +1,133 @@
+/*
+ * tfdtd.h -- C ABI of the MI355X (gfx950) Yee/CPML time-stepping engine.
+ *
+ * Plain C: descriptors of raw device pointers and sizes, a hipStream_t passed as
+ * void*, an int status (0 = success).  No allocation and no host synchronisation
+ * happens inside any entry point, so every call may be recorded into a HIP graph.
+ * The caller owns every buffer.
+ *
+ * Layout of field arrays: real[nx][ny][nz][3], z fastest, components interleaved
+ * (the reference YeeGrid layout, torchfdtd/boundaries.py:238).  real is float
+ * (dtype 0) or double (dtype 1).
+ */
+#ifndef TORCHFDTD_AMD_TFDTD_H_
+#define TORCHFDTD_AMD_TFDTD_H_
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TFDTD_ABI_VERSION 1
+#define TFDTD_MAX_SEGMENTS 12   /* 6 curl terms x 2 PML faces, per family */
+#define TFDTD_MAX_SOURCES 16    /* soft source boxes per family */
+
+enum { TFDTD_OK = 0, TFDTD_EINVAL = 1, TFDTD_EHIP = 2 };
+enum { TFDTD_F32 = 0, TFDTD_F64 = 1 };
+enum { TFDTD_FAMILY_E = 0, TFDTD_FAMILY_H = 1 };
+
+/* One CPML auxiliary slab of one curl term (torchfdtd/boundaries.py:307-355).
+ * psi has the grid's shape with extent hi-lo along `axis` (x,y,z order, z fastest);
+ * b, c and inv_k hold hi-lo coefficients of the target rows lo..hi-1. */
+typedef struct tfdtd_cpml_segment {
+    int32_t axis;        /* derivative axis 0..2 */
+    int32_t comp;        /* input component differentiated */
+    int32_t lo, hi;      /* target rows [lo, hi) along axis */
+    void* psi;
+    const void* b;
+    const void* c;
+    const void* inv_k;
+} tfdtd_cpml_segment;
+
+/* A soft additive source on a box of cells (torchfdtd/injection.py:117-141):
+ * field[box, comp] += values[counter] after the family's update. */
+typedef struct tfdtd_source {
+    int32_t comp;
+    int32_t lo[3], hi[3];
+    const void* values;  /* real[steps]: waveform x polarisation weight */
+} tfdtd_source;
+
+/* One half step of one family: E (backward differences of H) or H (forward
+ * differences of E), torchfdtd/boundaries.py:363-438. */
+typedef struct tfdtd_half_step {
+    int32_t dtype;
+    int32_t family;                 /* TFDTD_FAMILY_E or TFDTD_FAMILY_H */
+    int32_t nx, ny, nz;
+    const void* src;                /* H for the E update, E for the H update */
+    void* dst;                      /* E for the E update, H for the H update */
+    const void* inverse;            /* inverse permittivity (E) or permeability (H) */
+    int32_t inverse_stride;         /* 0 one value, 1 per cell, 3 per cell and component */
+    double courant;                 /* S, rounded to real inside the kernel */
+    int32_t periodic_mask;          /* bit a: axis a wraps (real fields) */
+    int32_t pec_upper_mask;         /* bit a: PEC ghost node above axis a (H update) */
+    double pec_upper_factor[3];
+    int32_t nseg;
+    tfdtd_cpml_segment seg[TFDTD_MAX_SEGMENTS];
+    int32_t nsrc;
+    tfdtd_source srcs[TFDTD_MAX_SOURCES];
+    const int64_t* counter;         /* device step counter read for source samples */
+} tfdtd_half_step;
+
+/* Point traces and the step counter (torchfdtd/solver.py:791-797). */
+typedef struct tfdtd_observe {
+    int32_t dtype;
+    int32_t count;                  /* number of traces */
+    const int64_t* table;           /* device int64[count][2]: family, flat index ((x*ny+y)*nz+z)*3+c */
+    const void* E;
+    const void* H;
+    void* traces;                   /* real[steps][count] */
+    int64_t steps;                  /* rows of traces; records stop beyond */
+    int64_t* counter;               /* incremented by one after recording */
+} tfdtd_observe;
+
+/* One frequency plane (torchfdtd/field_monitors.py:121-164, cuda_monitors.py:65-121). */
+typedef struct tfdtd_plane {
+    int32_t dtype;                  /* field real type */
+    int32_t spectral;               /* accumulator: 0 complex64, 1 complex128 */
+    int64_t points;                 /* n */
+    int32_t frequencies;            /* nf */
+    int32_t components;             /* nc <= 6: E components first, then H */
+    int32_t electric;               /* ne */
+    int32_t corners;                /* interpolation terms per point (1..8) */
+    int32_t time_downsample;
+    const void* E;
+    const void* H;
+    const int64_t* indices[6];      /* per component: int64[corners][n] flat field indices */
+    const void* weights[6];         /* per component: real[corners][n] */
+    const void* window_e;           /* real[steps]: apodisation x dt x downsample mask */
+    const void* window_h;           /* same at the half-step times */
+    const double* omega;            /* double[nf]: 2 pi f dt */
+    const void* half;               /* spectral complex[nf]: exp(i pi f dt) */
+    void* value;                    /* spectral complex[nf][n][nc], accumulated */
+    const int64_t* counter;
+} tfdtd_plane;
+
+/* torchfdtd/cuda_kernels.py:349-365 (FusedYeeCUDA.update_E / update_H), with the
+ * family's sources of torchfdtd/solver.py:778-783 folded in when nsrc > 0. */
+int tfdtd_yee_half_step(const tfdtd_half_step* step, void* stream);
+
+/* torchfdtd/cuda_monitors.py:135-155 (FusedFrequencyPlanes.update): phases from the
+ * absolute counter in FP64, eight-corner sampling and accumulation, one launch per plane. */
+int tfdtd_dft_update(const tfdtd_plane* planes, int32_t count, void* stream);
+
+/* torchfdtd/solver.py:791-797: trace record (traces.index_copy_) and counter.add_(1). */
+int tfdtd_observe_step(const tfdtd_observe* observe, void* stream);
+
+/* torchfdtd/solver.py:785-797 (the step closure) repeated n_steps times:
+ * E update + E sources, H update + H sources, plane DFTs, traces, counter += 1. */
+int tfdtd_step_fused(const tfdtd_half_step* e_step, const tfdtd_half_step* h_step,
+                     const tfdtd_plane* planes, int32_t plane_count,
+                     const tfdtd_observe* observe, int32_t n_steps, void* stream);
+
+/* Copies the calling thread's last error message; returns its length. */
+size_t tfdtd_last_error(char* buffer, size_t size);
+
+int tfdtd_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TORCHFDTD_AMD_TFDTD_H_ */

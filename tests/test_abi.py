@@ -1,0 +1,76 @@
+"""The C ABI (CPU only): the library loads, exports every function include/tfdtd.h
+declares, and the ctypes mirror has the C layout of every descriptor."""
+import ctypes
+import re
+import subprocess
+
+import pytest
+
+from tests.conftest import ROOT
+from torchfdtd_amd import _native as N
+
+HEADER = ROOT/'include'/'tfdtd.h'
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    return sorted(set(re.findall(r'^\w[\w\s\*]*?\b(tfdtd_\w+)\s*\(', text, flags=re.M)))
+
+
+def test_library_exports_every_declared_function():
+    if not N.LIBRARY.exists():
+        pytest.fail('libtfdtd.so is not built: run __graft_entry__.build()')
+    lib = ctypes.CDLL(str(N.LIBRARY))
+    names = declared_functions()
+    assert 'tfdtd_yee_half_step' in names and 'tfdtd_last_error' in names
+    for name in names:
+        assert hasattr(lib, name), name
+    assert lib.tfdtd_abi_version() == 1
+
+
+def test_errors_are_reported_without_a_gpu():
+    lib = N.load_library()
+    d = N.HalfStep()
+    d.dtype, d.family, d.nx, d.ny, d.nz = 7, 0, 4, 4, 4
+    d.src = d.dst = d.inverse = 16
+    assert lib.tfdtd_yee_half_step(ctypes.byref(d), None) == 1
+    assert 'dtype' in N.last_error()
+    d.dtype, d.nseg = 0, 1
+    d.seg[0].axis, d.seg[0].comp, d.seg[0].lo, d.seg[0].hi = 0, 0, 0, 2   # no term differentiates Ex along x
+    assert lib.tfdtd_yee_half_step(ctypes.byref(d), None) == 1
+    assert 'no curl term' in N.last_error()
+
+
+LAYOUT_C = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "tfdtd.h"
+#define F(T, m) printf(#T "." #m " %zu\n", offsetof(T, m))
+int main(void) {
+  printf("tfdtd_half_step %zu\n", sizeof(tfdtd_half_step));
+  printf("tfdtd_plane %zu\n", sizeof(tfdtd_plane));
+  printf("tfdtd_observe %zu\n", sizeof(tfdtd_observe));
+  printf("tfdtd_cpml_segment %zu\n", sizeof(tfdtd_cpml_segment));
+  printf("tfdtd_source %zu\n", sizeof(tfdtd_source));
+  F(tfdtd_half_step, courant); F(tfdtd_half_step, seg); F(tfdtd_half_step, srcs); F(tfdtd_half_step, counter);
+  F(tfdtd_plane, points); F(tfdtd_plane, indices); F(tfdtd_plane, counter);
+  F(tfdtd_observe, steps); F(tfdtd_observe, counter);
+  return 0;
+}
+'''
+
+
+def test_ctypes_mirror_matches_the_c_layout(tmp_path):
+    source = tmp_path/'layout.c'
+    source.write_text(LAYOUT_C)
+    exe = tmp_path/'layout'
+    subprocess.run(['gcc', '-std=c11', '-I', str(HEADER.parent), str(source), '-o', str(exe)], check=True)
+    lines = dict(line.rsplit(' ', 1) for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split('\n') if line)
+    mirror = {'tfdtd_half_step': N.HalfStep, 'tfdtd_plane': N.Plane, 'tfdtd_observe': N.Observe,
+              'tfdtd_cpml_segment': N.CpmlSegment, 'tfdtd_source': N.SourceBox}
+    for key, value in lines.items():
+        if '.' in key:
+            struct, member = key.split('.')
+            assert getattr(mirror[struct], member).offset == int(value), key
+        else:
+            assert ctypes.sizeof(mirror[key]) == int(value), key

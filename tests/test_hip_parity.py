@@ -1,0 +1,200 @@
+"""HIP half steps and the fused step against the NumPy oracle of the reference path.
+
+The HIP library keeps the reference's operation order and rounds every product and
+sum separately (no FMA contraction, denormals kept), so results are compared bit for
+bit.  Patterns follow the reference's own tests: random fields and every CPML state
+over 23 steps (tests/test_cuda_kernels.py:72-111, seed 615), subnormal preservation
+(:149-170), graph vs eager and complete solves (tests/test_solver.py:60-75).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import simulate
+from oracle.scene import Scene
+from oracle.yee import Grid as OracleGrid
+from tests.conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def region_for(layout, precision):
+    from torchfdtd_amd import BoundaryFace, Region
+    r = Region(dimension='2d' if layout == 'pml2d' else '3d', size=(3.2, 2.8, 2.4), mesh=.1, pml_cells=4,
+               precision=precision, material_sampling='yee', backend='cuda')
+    r.boundaries.x_min = BoundaryFace(layers=5, kappa=3, alpha=.03, alpha_polynomial=1)
+    if layout == 'mixed':
+        r.boundaries.y_min = BoundaryFace(kind='periodic')
+        r.boundaries.y_max = BoundaryFace(kind='periodic')
+    if layout == 'pec':
+        r.boundaries.z_min = BoundaryFace(kind='pec')
+        r.boundaries.z_max = BoundaryFace(kind='pec')
+        r.boundaries.y_max = BoundaryFace(kind='pec')
+        r.boundaries.y_min = BoundaryFace(kind='pec')
+    return r
+
+
+@pytest.mark.parametrize('precision', ['float32', 'float64'])
+@pytest.mark.parametrize('layout', ['pml2d', 'pml3d', 'mixed', 'pec'])
+def test_random_fields_and_every_cpml_state_bitwise(gpu, precision, layout):
+    from torchfdtd_amd import Project
+    from torchfdtd_amd.grid import HipYee, YeeGrid
+    region = region_for(layout, precision)
+    project = Project(region=region)
+    grid = YeeGrid(region)
+    scene = Scene(project.model_dump())
+    real = scene.real
+    rng = np.random.default_rng(615)
+    E = (rng.normal(size=(*scene.shape, 3))*.02).astype(real)
+    H = (rng.normal(size=(*scene.shape, 3))*.02).astype(real)
+    inv = rng.uniform(.25, 1, (*scene.shape, 3)).astype(real)
+    ref = OracleGrid(scene, inv.copy())
+    ref.E[:], ref.H[:] = E, H
+    grid.E.copy_(torch.as_tensor(E))
+    grid.H.copy_(torch.as_tensor(H))
+    grid.inverse_permittivity.copy_(torch.as_tensor(inv))
+    kernel = HipYee(grid)
+    for _ in range(23):
+        ref.update_E()
+        ref.update_H()
+        kernel.update_E()
+        kernel.update_H()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(grid.E.cpu().numpy(), ref.E)
+    np.testing.assert_array_equal(grid.H.cpu().numpy(), ref.H)
+    assert len(grid.memory_states) == len(ref.memory_states) > 0 or layout == 'pec'
+    for got, expected in zip(grid.memory_states, ref.memory_states):
+        np.testing.assert_array_equal(got.cpu().numpy(), expected)
+
+
+def test_subnormal_fields_are_preserved(gpu):
+    from torchfdtd_amd import Project, Region
+    from torchfdtd_amd.grid import HipYee, YeeGrid
+    region = Region(size=(2, 2, 1), mesh=.1, pml_cells=4, backend='cuda')
+    grid = YeeGrid(region)
+    ref = OracleGrid(Scene(Project(region=region).model_dump()))
+    grid.E[10, 10, 0, 2] = 1e-38
+    ref.E[10, 10, 0, 2] = np.float32(1e-38)
+    kernel = HipYee(grid)
+    for _ in range(20):
+        ref.update_E()
+        ref.update_H()
+        kernel.update_E()
+        kernel.update_H()
+    assert np.count_nonzero(ref.H) > 0
+    assert np.count_nonzero(np.abs(ref.H) < np.finfo(np.float32).tiny) > 0   # subnormals occur
+    np.testing.assert_array_equal(grid.E.cpu().numpy(), ref.E)
+    np.testing.assert_array_equal(grid.H.cpu().numpy(), ref.H)
+
+
+@pytest.mark.parametrize('name', ['tiny_2d_complete', 'tiny_2d_cancelled'])
+def test_reference_golden_run_bitwise(gpu, name):
+    """The reference-written CPU FP32 run (tests/fixtures/npz) reproduced by the HIP step, bit for bit."""
+    import json
+    from torchfdtd_amd import Project, Simulation
+    data = np.load(GOLDEN/f'{name}.npz', allow_pickle=False)
+    project = Project.model_validate_json(str(data['project']))
+    project.region.backend = 'cuda'
+    project.region.cuda_kernel = 'hip'
+    steps = json.loads(str(data['summary']))['steps']
+    project.region.steps = steps if name == 'tiny_2d_complete' else project.region.steps
+    if name == 'tiny_2d_cancelled':
+        from threading import Event
+        event = Event()
+        result = Simulation(project).run(progress=lambda s: event.set() if s['step'] >= 20 else None, cancel=event)
+        assert result.summary['steps'] == 20
+    else:
+        result = Simulation(project).run()
+    np.testing.assert_array_equal(result.electric, data['E'])
+    np.testing.assert_array_equal(result.magnetic, data['H'])
+    np.testing.assert_array_equal(result.signals, data['signals'])
+    np.testing.assert_array_equal(result.frames, data['frames'])
+    np.testing.assert_array_equal(result.times, data['times'])
+    np.testing.assert_allclose(result.spectra[0]['value'], data['monitor_0_spectrum'], rtol=1e-12, atol=0)
+
+
+def waveguide_2d(steps=1000):
+    """The 2D TM waveguide of the reference's CPU configuration: 160x120 cells, 10-cell CPML, Ez point source."""
+    import math
+    from torchfdtd_amd import Material, Monitor, Project, Region, Source, SpectrumSettings, Structure
+    return Project(region=Region(dimension='2d', size=(8, 6, .05), mesh=.05, steps=steps, pml_cells=10,
+                                 background_index=math.sqrt(2.085), backend='cuda', snapshot_interval=50),
+                   materials=[Material(name='core', index=math.sqrt(12.11))],
+                   structures=[Structure(size=(8, .65, 1), material='core')],
+                   sources=[Source(center=(-2.5, 0, 0), component='Ez', pulse='broadband', time_definition='wavelength',
+                                   wavelength_start=1.5, wavelength_stop=1.6)],
+                   monitors=[Monitor(center=(2, 0, 0), spectrum=SpectrumSettings(
+                       sampling='wavelength', frequency_points=32, wavelength_start=1.5, wavelength_stop=1.6,
+                       apodization='none'))])
+
+
+def test_2d_waveguide_1000_steps_matches_oracle(gpu):
+    project = waveguide_2d()
+    from torchfdtd_amd import Simulation
+    got = Simulation(project).run()
+    ref = simulate.run(project.model_dump())
+    np.testing.assert_array_equal(got.electric, ref['E'])
+    np.testing.assert_array_equal(got.magnetic, ref['H'])
+    np.testing.assert_array_equal(got.signals, ref['signals'])
+    f, v = ref['spectra'][0]
+    np.testing.assert_array_equal(got.spectra[0]['frequency_hz'], f)
+    np.testing.assert_allclose(got.spectra[0]['value'], v, rtol=1e-12, atol=1e-12*np.abs(v).max())
+    assert np.abs(got.signals).max() > 1e-3
+
+
+def test_graph_unrolled_and_eager_steps_are_bitwise_equal(gpu):
+    from torchfdtd_amd import Simulation
+    project = waveguide_2d(steps=120)
+    eager = Simulation(project).run(cuda_graph=False)
+    graph = Simulation(project).run(cuda_graph=True)
+    unrolled = Simulation(project).run(cuda_graph=True, cuda_graph_steps=16)
+    assert graph.summary['cuda_graph'] and unrolled.summary['cuda_graph_steps'] == 16
+    for other in (graph, unrolled):
+        np.testing.assert_array_equal(other.electric, eager.electric)
+        np.testing.assert_array_equal(other.signals, eager.signals)
+        np.testing.assert_array_equal(other.frames, eager.frames)
+
+
+def test_3d_waveguide_matches_oracle(gpu):
+    """The 3D waveguide configuration (160x120x40, 12-cell CPML, Ey dipole) over its first 60 steps."""
+    import math
+    from torchfdtd_amd import Material, Monitor, Project, Region, Simulation, Source, Structure
+    project = Project(region=Region(dimension='3d', size=(8, 6, 2), mesh=.05, steps=60, pml_cells=12,
+                                    background_index=math.sqrt(2.085), backend='cuda', snapshot_interval=30),
+                      materials=[Material(name='core', index=math.sqrt(12.11))],
+                      structures=[Structure(size=(8, .65, .4), material='core')],
+                      sources=[Source(center=(-2.5, 0, 0), component='Ey', pulse='broadband', time_definition='wavelength',
+                                      wavelength_start=1.5, wavelength_stop=1.6)],
+                      monitors=[Monitor(center=(-2.4, 0, 0), component='Ey')])
+    got = Simulation(project).run()
+    ref = simulate.run(project.model_dump())
+    np.testing.assert_array_equal(got.electric, ref['E'])
+    np.testing.assert_array_equal(got.magnetic, ref['H'])
+    np.testing.assert_array_equal(got.signals, ref['signals'])
+    assert np.abs(got.signals).max() > 0
+
+
+@pytest.mark.parametrize('dft_precision', ['field', 'float64'])
+def test_plane_dft_matches_oracle(gpu, dft_precision):
+    from torchfdtd_amd import FieldMonitor, Monitor, Project, Region, Simulation, Source, SpectrumSettings, Structure
+    project = Project(region=Region(dimension='3d', size=(3.2, 2.8, 2.4), mesh=.1, pml_cells=4, steps=80, backend='cuda',
+                                    material_sampling='yee'),
+                      structures=[Structure(kind='sphere', radius=.4, material='SiO2 (constant n)')],
+                      sources=[Source(center=(-.8, 0, 0))],
+                      monitors=[Monitor(center=(.5, 0, 0)),
+                                FieldMonitor(id='x-plane', normal='x', center=(.7, 0, 0), size=(0, 1.03, 1), dft_precision=dft_precision,
+                                             spectrum=SpectrumSettings(sampling='frequency', frequency_points=7, apodization='none')),
+                                FieldMonitor(id='y-plane', normal='y', center=(0, .2, 0), size=(1.17, 0, .8), downsample=2,
+                                             time_downsample=3, dft_precision=dft_precision,
+                                             spectrum=SpectrumSettings(sampling='wavelength', frequency_points=3, apodization='end'))])
+    got = Simulation(project).run()
+    ref = simulate.run(project.model_dump())
+    np.testing.assert_array_equal(got.electric, ref['E'])
+    for plane, expected in zip(got.frequency_fields, ref['planes']):
+        values = expected.value
+        names = list(expected.components)
+        fields = values[..., [names.index(c) for c in plane['components']]]
+        scale = np.abs(fields).max()
+        assert scale > 0
+        tol = 3e-6 if dft_precision == 'field' else 1e-12
+        np.testing.assert_allclose(plane['fields']/scale, fields/scale, rtol=tol, atol=tol)

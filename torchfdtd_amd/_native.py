@@ -1,0 +1,137 @@
+"""Build and load libtfdtd.so, the gfx950 HIP library behind include/tfdtd.h.
+
+The library is compiled in-tree with hipcc for gfx950 only.  Loading it requires
+a ROCm GPU; there is no fallback: every engine entry point raises when the
+library or the GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+PACKAGE = Path(__file__).resolve().parent
+ROOT = PACKAGE.parent
+LIBRARY = PACKAGE/'libtfdtd.so'
+SOURCES = [PACKAGE/'csrc'/name for name in ('yee.hip', 'march.hip', 'batch.hip', 'adjoint.hip', 'conv.hip')]
+HEADER = ROOT/'include'/'tfdtd.h'
+
+HIPCC_FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-shared', '-std=c++17',
+               # Every product and sum rounds separately, as in the reference update.
+               '-ffp-contract=off', '-fno-gpu-flush-denormals-to-zero',
+               '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
+
+
+def build(verbose=False):
+    """Compile every HIP source into LIBRARY (always recompiles)."""
+    hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
+    sources = [str(s) for s in SOURCES if s.exists()]
+    tmp = LIBRARY.with_name(LIBRARY.name+'.tmp')
+    cmd = [hipcc, *HIPCC_FLAGS, '-I', str(ROOT/'include'), *sources, '-o', str(tmp)]
+    if verbose:
+        print(' '.join(cmd), flush=True)
+    subprocess.run(cmd, check=True, cwd=str(PACKAGE))
+    os.replace(tmp, LIBRARY)
+    return LIBRARY
+
+
+# ------------------------------------------------------------------ ctypes mirror of include/tfdtd.h
+
+MAX_SEGMENTS = 12
+MAX_SOURCES = 16
+F32, F64 = 0, 1
+FAMILY_E, FAMILY_H = 0, 1
+
+vp = ctypes.c_void_p
+i32 = ctypes.c_int32
+i64 = ctypes.c_int64
+
+
+class CpmlSegment(ctypes.Structure):
+    _fields_ = [('axis', i32), ('comp', i32), ('lo', i32), ('hi', i32),
+                ('psi', vp), ('b', vp), ('c', vp), ('inv_k', vp)]
+
+
+class SourceBox(ctypes.Structure):
+    _fields_ = [('comp', i32), ('lo', i32*3), ('hi', i32*3), ('values', vp)]
+
+
+class HalfStep(ctypes.Structure):
+    _fields_ = [('dtype', i32), ('family', i32), ('nx', i32), ('ny', i32), ('nz', i32),
+                ('src', vp), ('dst', vp), ('inverse', vp), ('inverse_stride', i32),
+                ('courant', ctypes.c_double), ('periodic_mask', i32), ('pec_upper_mask', i32),
+                ('pec_upper_factor', ctypes.c_double*3),
+                ('nseg', i32), ('seg', CpmlSegment*MAX_SEGMENTS),
+                ('nsrc', i32), ('srcs', SourceBox*MAX_SOURCES),
+                ('counter', vp)]
+
+
+class Observe(ctypes.Structure):
+    _fields_ = [('dtype', i32), ('count', i32), ('table', vp), ('E', vp), ('H', vp),
+                ('traces', vp), ('steps', i64), ('counter', vp)]
+
+
+class Plane(ctypes.Structure):
+    _fields_ = [('dtype', i32), ('spectral', i32), ('points', i64), ('frequencies', i32),
+                ('components', i32), ('electric', i32), ('corners', i32), ('time_downsample', i32),
+                ('E', vp), ('H', vp), ('indices', vp*6), ('weights', vp*6),
+                ('window_e', vp), ('window_h', vp), ('omega', vp), ('half', vp), ('value', vp),
+                ('counter', vp)]
+
+
+_lib = None
+
+PROTOTYPES = {
+    'tfdtd_abi_version': (i32, []),
+    'tfdtd_last_error': (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
+    'tfdtd_yee_half_step': (i32, [ctypes.POINTER(HalfStep), vp]),
+    'tfdtd_dft_update': (i32, [ctypes.POINTER(Plane), i32, vp]),
+    'tfdtd_observe_step': (i32, [ctypes.POINTER(Observe), vp]),
+    'tfdtd_step_fused': (i32, [ctypes.POINTER(HalfStep), ctypes.POINTER(HalfStep), ctypes.POINTER(Plane), i32,
+                               ctypes.POINTER(Observe), i32, vp]),
+}
+
+
+def load_library(path=None):
+    """dlopen the library and bind the prototypes (no GPU needed to load)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    import torch  # noqa: F401  (torch's HIP runtime is the one the library binds to)
+    target = Path(path) if path else LIBRARY
+    if not target.exists():
+        raise RuntimeError(f'{target} is missing: run torchfdtd_amd._native.build() (hipcc for gfx950) first')
+    lib = ctypes.CDLL(str(target))
+    for name, (restype, argtypes) in PROTOTYPES.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def library():
+    """The loaded library, with a GPU present; raises otherwise (no CPU fallback)."""
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError('torchfdtd_amd runs on an AMD GPU (gfx950); no GPU is visible')
+    return load_library()
+
+
+def last_error():
+    lib = load_library()
+    buf = ctypes.create_string_buffer(1024)
+    lib.tfdtd_last_error(buf, len(buf))
+    return buf.value.decode(errors='replace')
+
+
+def check(status, what):
+    if status != 0:
+        raise RuntimeError(f'{what} failed ({status}): {last_error()}')
+
+
+def stream_handle(device=None):
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

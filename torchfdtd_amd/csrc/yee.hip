@@ -1,0 +1,418 @@
+// Yee/CPML half steps, point traces and plane DFTs for gfx950, behind the C ABI of
+// include/tfdtd.h.
+//
+// Arithmetic follows the reference update (torchfdtd/boundaries.py:363-438) term by
+// term: every product and sum is rounded to the field precision in the same order,
+// and the library is built with -ffp-contract=off and denormals kept, so a step is
+// reproducible bit for bit against the NumPy/Torch path of the same precision.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/tfdtd.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buffer[512];
+    va_list args;
+    va_start(args, fmt);
+    vsnprintf(buffer, sizeof buffer, fmt, args);
+    va_end(args);
+    g_error = buffer;
+    return code;
+}
+
+int check_launch(const char* what) {
+    hipError_t status = hipGetLastError();
+    if (status != hipSuccess) return fail(TFDTD_EHIP, "%s: %s", what, hipGetErrorString(status));
+    return TFDTD_OK;
+}
+
+// (derivative axis, input component, output component, sign): torchfdtd/boundaries.py:21-23.
+__host__ __device__ constexpr int term_axis(int t) { return t / 2; }
+__host__ __device__ constexpr int term_in(int t) { return (t == 0 || t == 2) ? 2 : (t == 1 || t == 4) ? 1 : 0; }
+__host__ __device__ constexpr int term_out(int t) { return (t == 2 || t == 4) ? 0 : (t == 0 || t == 5) ? 1 : 2; }
+__host__ __device__ constexpr int term_sign(int t) { return (t == 0 || t == 3 || t == 4) ? -1 : 1; }
+
+template <typename real>
+struct Slab {          // one CPML segment in its fixed slot 2*term+side
+    int lo, hi;        // empty slot: lo == hi == 0
+    real* psi;
+    const real* b;
+    const real* c;
+    const real* inv_k;
+};
+
+template <typename real>
+struct Source {
+    int comp;
+    int lo[3], hi[3];
+    const real* values;
+};
+
+template <typename real>
+struct HalfStep {
+    int nx, ny, nz;
+    const real* src;
+    real* dst;
+    const real* inverse;
+    int inverse_stride;
+    real courant;
+    int periodic_mask, pec_mask;
+    real pec_factor[3];
+    Slab<real> slab[12];
+    int nsrc;
+    int box_lo[3], box_hi[3];   // union of the source boxes
+    Source<real> srcs[TFDTD_MAX_SOURCES];
+    const long long* counter;
+};
+
+// One cell per thread.  forward=false: E update from backward differences of H;
+// forward=true: H update from forward differences of E.
+template <typename real, bool forward>
+__global__ __launch_bounds__(256) void yee_half_step_kernel(const HalfStep<real> a) {
+    const long long count = (long long)a.nx * a.ny * a.nz;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int nyz = a.ny * a.nz;
+    const int x = (int)(i / nyz);
+    const int r = (int)(i - (long long)x * nyz);
+    const int y = r / a.nz;
+    const int z = r - y * a.nz;
+    const int coord[3] = {x, y, z};
+    const int dims[3] = {a.nx, a.ny, a.nz};
+    const long long stride[3] = {(long long)nyz, (long long)a.nz, 1};
+    real curl[3] = {real(0), real(0), real(0)};
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        const int axis = term_axis(t), ic = term_in(t), oc = term_out(t);
+        const int n = dims[axis];
+        if (n == 1) continue;
+        const int k = coord[axis];
+        const long long s = stride[axis];
+        real d = real(0);
+        if (forward ? (k < n - 1) : (k > 0)) {
+            d = forward ? (a.src[3 * (i + s) + ic] - a.src[3 * i + ic])
+                        : (a.src[3 * i + ic] - a.src[3 * (i - s) + ic]);
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const Slab<real>& g = a.slab[2 * t + side];
+                if (k >= g.lo && k < g.hi) {
+                    const int q = k - g.lo;
+                    const int w = g.hi - g.lo;
+                    long long p;
+                    if (axis == 0) p = ((long long)q * a.ny + y) * a.nz + z;
+                    else if (axis == 1) p = ((long long)x * w + q) * a.nz + z;
+                    else p = ((long long)x * a.ny + y) * w + q;
+                    const real memory = g.psi[p] * g.b[q] + g.c[q] * d;
+                    g.psi[p] = memory;
+                    d = d * g.inv_k[q] + memory;
+                }
+            }
+        } else if ((a.periodic_mask >> axis) & 1) {
+            const long long span = (long long)(n - 1) * s;
+            d = forward ? (a.src[3 * (i - span) + ic] - a.src[3 * i + ic])
+                        : (a.src[3 * i + ic] - a.src[3 * (i + span) + ic]);
+        } else if (forward && ((a.pec_mask >> axis) & 1)) {
+            d = -a.pec_factor[axis] * a.src[3 * i + ic];
+        }
+        curl[oc] = term_sign(t) > 0 ? curl[oc] + d : curl[oc] - d;
+    }
+    real out[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const long long j = a.inverse_stride == 3 ? 3 * i + c : a.inverse_stride == 1 ? i : 0;
+        const real coefficient = a.courant * a.inverse[j];
+        out[c] = forward ? a.dst[3 * i + c] - coefficient * curl[c] : a.dst[3 * i + c] + coefficient * curl[c];
+    }
+    if (a.nsrc > 0 && x >= a.box_lo[0] && x < a.box_hi[0] && y >= a.box_lo[1] && y < a.box_hi[1] &&
+        z >= a.box_lo[2] && z < a.box_hi[2]) {
+        const long long n = *a.counter;
+        for (int k = 0; k < a.nsrc; ++k) {
+            const Source<real>& src = a.srcs[k];
+            if (x >= src.lo[0] && x < src.hi[0] && y >= src.lo[1] && y < src.hi[1] && z >= src.lo[2] &&
+                z < src.hi[2]) {
+                const real v = src.values[n];
+                if (src.comp == 0) out[0] = out[0] + v;
+                else if (src.comp == 1) out[1] = out[1] + v;
+                else out[2] = out[2] + v;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a.dst[3 * i + c] = out[c];
+}
+
+template <typename real>
+struct Observe {
+    int count;
+    const long long* table;
+    const real* E;
+    const real* H;
+    real* traces;
+    long long steps;
+    long long* counter;
+};
+
+// Single block: record the traces of this step, then advance the counter.
+template <typename real>
+__global__ __launch_bounds__(256) void observe_kernel(const Observe<real> a) {
+    const long long n = *a.counter;
+    __syncthreads();
+    if (n < a.steps) {
+        for (int k = threadIdx.x; k < a.count; k += blockDim.x) {
+            const long long family = a.table[2 * k], index = a.table[2 * k + 1];
+            a.traces[n * a.count + k] = (family ? a.H : a.E)[index];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) a.counter[0] = n + 1;
+}
+
+template <typename real, typename spec>
+struct Plane {
+    long long points;
+    int nf, nc, ne, corners, downsample;
+    const real* E;
+    const real* H;
+    const long long* indices[6];
+    const real* weights[6];
+    const real* window_e;
+    const real* window_h;
+    const double* omega;
+    const spec* half;   // interleaved re, im
+    spec* value;        // interleaved re, im
+    const long long* counter;
+};
+
+// One thread per (point, component) sample; the block first evaluates the nf
+// absolute-time phases exp(i omega (n+1)) in FP64 (cuda_monitors.py:68-84) into LDS.
+template <typename real, typename spec>
+__global__ __launch_bounds__(256) void dft_plane_kernel(const Plane<real, spec> a) {
+    extern __shared__ unsigned char lds_raw[];
+    spec* phase = reinterpret_cast<spec*>(lds_raw);
+    const long long n = *a.counter;
+    if (n % a.downsample) return;
+    for (int f = threadIdx.x; f < a.nf; f += blockDim.x) {
+        double si, co;
+        sincos(a.omega[f] * (double)(n + 1), &si, &co);
+        const spec er = (spec)co, ei = (spec)si;
+        const spec hr = a.half[2 * f], hi = a.half[2 * f + 1];
+        phase[4 * f + 0] = er;
+        phase[4 * f + 1] = ei;
+        phase[4 * f + 2] = er * hr - ei * hi;
+        phase[4 * f + 3] = er * hi + ei * hr;
+    }
+    __syncthreads();
+    const long long samples = a.points * a.nc;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= samples) return;
+    const int c = (int)(i % a.nc);
+    const long long p = i / a.nc;
+    const bool electric = c < a.ne;
+    const real* field = electric ? a.E : a.H;
+    const long long* idx = a.indices[c];
+    const real* w = a.weights[c];
+    real v = real(0);
+    for (int j = 0; j < a.corners; ++j) v = v + field[idx[j * a.points + p]] * w[j * a.points + p];
+    v = v * (electric ? a.window_e : a.window_h)[n];
+    const spec sv = (spec)v;
+    const int o = electric ? 0 : 2;
+    for (int f = 0; f < a.nf; ++f) {
+        spec* out = a.value + 2 * ((long long)f * samples + i);
+        out[0] = out[0] + phase[4 * f + o] * sv;
+        out[1] = out[1] + phase[4 * f + o + 1] * sv;
+    }
+}
+
+// ---------------------------------------------------------------- host side
+
+template <typename real>
+int make_half_step(const tfdtd_half_step* d, HalfStep<real>* a) {
+    if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(TFDTD_EINVAL, "grid dimensions must be positive");
+    if (!d->src || !d->dst || !d->inverse) return fail(TFDTD_EINVAL, "field and coefficient pointers are required");
+    if (d->inverse_stride != 0 && d->inverse_stride != 1 && d->inverse_stride != 3)
+        return fail(TFDTD_EINVAL, "inverse_stride must be 0, 1 or 3");
+    if (d->nseg < 0 || d->nseg > TFDTD_MAX_SEGMENTS) return fail(TFDTD_EINVAL, "too many CPML segments");
+    if (d->nsrc < 0 || d->nsrc > TFDTD_MAX_SOURCES) return fail(TFDTD_EINVAL, "too many sources");
+    std::memset(a, 0, sizeof *a);
+    a->nx = d->nx; a->ny = d->ny; a->nz = d->nz;
+    a->src = static_cast<const real*>(d->src);
+    a->dst = static_cast<real*>(d->dst);
+    a->inverse = static_cast<const real*>(d->inverse);
+    a->inverse_stride = d->inverse_stride;
+    a->courant = (real)d->courant;
+    a->periodic_mask = d->periodic_mask & 7;
+    a->pec_mask = d->pec_upper_mask & 7;
+    for (int k = 0; k < 3; ++k) a->pec_factor[k] = (real)d->pec_upper_factor[k];
+    const int dims[3] = {d->nx, d->ny, d->nz};
+    for (int k = 0; k < d->nseg; ++k) {
+        const tfdtd_cpml_segment& s = d->seg[k];
+        int term = -1;
+        for (int t = 0; t < 6; ++t)
+            if (term_axis(t) == s.axis && term_in(t) == s.comp) term = t;
+        if (term < 0) return fail(TFDTD_EINVAL, "segment %d: no curl term differentiates component %d along axis %d", k, s.comp, s.axis);
+        if (s.lo < 0 || s.hi <= s.lo || s.hi > dims[s.axis]) return fail(TFDTD_EINVAL, "segment %d: rows [%d,%d) outside the axis", k, s.lo, s.hi);
+        if (!s.psi || !s.b || !s.c || !s.inv_k) return fail(TFDTD_EINVAL, "segment %d: null pointer", k);
+        int slot = 2 * term;
+        if (a->slab[slot].hi != a->slab[slot].lo) {
+            ++slot;
+            if (a->slab[slot].hi != a->slab[slot].lo) return fail(TFDTD_EINVAL, "segment %d: more than two slabs for one curl term", k);
+            const Slab<real>& other = a->slab[slot - 1];
+            if (s.lo < other.hi && other.lo < s.hi) return fail(TFDTD_EINVAL, "segment %d overlaps another slab of its term", k);
+        }
+        a->slab[slot] = Slab<real>{s.lo, s.hi, static_cast<real*>(s.psi), static_cast<const real*>(s.b),
+                                   static_cast<const real*>(s.c), static_cast<const real*>(s.inv_k)};
+    }
+    a->nsrc = d->nsrc;
+    for (int k = 0; k < 3; ++k) { a->box_lo[k] = dims[k]; a->box_hi[k] = 0; }
+    for (int k = 0; k < d->nsrc; ++k) {
+        const tfdtd_source& s = d->srcs[k];
+        if (s.comp < 0 || s.comp > 2 || !s.values) return fail(TFDTD_EINVAL, "source %d: bad component or values", k);
+        Source<real>& o = a->srcs[k];
+        o.comp = s.comp;
+        o.values = static_cast<const real*>(s.values);
+        for (int ax = 0; ax < 3; ++ax) {
+            if (s.lo[ax] < 0 || s.hi[ax] <= s.lo[ax] || s.hi[ax] > dims[ax]) return fail(TFDTD_EINVAL, "source %d: box outside the grid", k);
+            o.lo[ax] = s.lo[ax]; o.hi[ax] = s.hi[ax];
+            if (s.lo[ax] < a->box_lo[ax]) a->box_lo[ax] = s.lo[ax];
+            if (s.hi[ax] > a->box_hi[ax]) a->box_hi[ax] = s.hi[ax];
+        }
+    }
+    if (d->nsrc > 0 && !d->counter) return fail(TFDTD_EINVAL, "sources need the device step counter");
+    a->counter = reinterpret_cast<const long long*>(d->counter);
+    return TFDTD_OK;
+}
+
+template <typename real>
+int launch_half_step(const tfdtd_half_step* d, hipStream_t stream) {
+    HalfStep<real> a;
+    int status = make_half_step(d, &a);
+    if (status) return status;
+    const long long count = (long long)d->nx * d->ny * d->nz;
+    const long long blocks = (count + 255) / 256;
+    if (blocks > 0x7fffffffLL) return fail(TFDTD_EINVAL, "grid too large for one launch");
+    if (d->family == TFDTD_FAMILY_E)
+        hipLaunchKernelGGL((yee_half_step_kernel<real, false>), dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    else
+        hipLaunchKernelGGL((yee_half_step_kernel<real, true>), dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    return check_launch("yee_half_step");
+}
+
+template <typename real>
+int launch_observe(const tfdtd_observe* d, hipStream_t stream) {
+    if (!d->counter) return fail(TFDTD_EINVAL, "observe needs the device step counter");
+    if (d->count < 0 || (d->count > 0 && (!d->table || !d->traces || !d->E || !d->H)))
+        return fail(TFDTD_EINVAL, "observe: trace table, traces and fields are required");
+    Observe<real> a{d->count, reinterpret_cast<const long long*>(d->table), static_cast<const real*>(d->E),
+                    static_cast<const real*>(d->H), static_cast<real*>(d->traces), (long long)d->steps,
+                    reinterpret_cast<long long*>(d->counter)};
+    hipLaunchKernelGGL((observe_kernel<real>), dim3(1), dim3(256), 0, stream, a);
+    return check_launch("observe");
+}
+
+template <typename real, typename spec>
+int launch_plane(const tfdtd_plane* d, hipStream_t stream) {
+    if (d->points < 1 || d->frequencies < 1 || d->components < 1 || d->components > 6 || d->electric < 0 ||
+        d->electric > d->components || d->corners < 1 || d->corners > 8 || d->time_downsample < 1)
+        return fail(TFDTD_EINVAL, "plane: bad sizes");
+    if (!d->E || !d->H || !d->window_e || !d->window_h || !d->omega || !d->half || !d->value || !d->counter)
+        return fail(TFDTD_EINVAL, "plane: null pointer");
+    const size_t lds = 4 * sizeof(spec) * (size_t)d->frequencies;
+    if (lds > 64 * 1024) return fail(TFDTD_EINVAL, "plane: %d frequencies exceed one launch", d->frequencies);
+    Plane<real, spec> a;
+    std::memset(&a, 0, sizeof a);
+    a.points = d->points; a.nf = d->frequencies; a.nc = d->components; a.ne = d->electric;
+    a.corners = d->corners; a.downsample = d->time_downsample;
+    a.E = static_cast<const real*>(d->E); a.H = static_cast<const real*>(d->H);
+    for (int c = 0; c < d->components; ++c) {
+        if (!d->indices[c] || !d->weights[c]) return fail(TFDTD_EINVAL, "plane: component %d map missing", c);
+        a.indices[c] = reinterpret_cast<const long long*>(d->indices[c]);
+        a.weights[c] = static_cast<const real*>(d->weights[c]);
+    }
+    a.window_e = static_cast<const real*>(d->window_e); a.window_h = static_cast<const real*>(d->window_h);
+    a.omega = d->omega; a.half = static_cast<const spec*>(d->half); a.value = static_cast<spec*>(d->value);
+    a.counter = reinterpret_cast<const long long*>(d->counter);
+    const long long samples = d->points * d->components;
+    const long long blocks = (samples + 255) / 256;
+    hipLaunchKernelGGL((dft_plane_kernel<real, spec>), dim3((unsigned)blocks), dim3(256), lds, stream, a);
+    return check_launch("dft_plane");
+}
+
+int half_step(const tfdtd_half_step* d, hipStream_t stream) {
+    if (!d) return fail(TFDTD_EINVAL, "null half-step descriptor");
+    if (d->family != TFDTD_FAMILY_E && d->family != TFDTD_FAMILY_H) return fail(TFDTD_EINVAL, "family must be E or H");
+    if (d->dtype == TFDTD_F32) return launch_half_step<float>(d, stream);
+    if (d->dtype == TFDTD_F64) return launch_half_step<double>(d, stream);
+    return fail(TFDTD_EINVAL, "dtype must be float32 or float64");
+}
+
+int planes(const tfdtd_plane* p, int count, hipStream_t stream) {
+    if (count < 0 || (count > 0 && !p)) return fail(TFDTD_EINVAL, "bad plane table");
+    for (int k = 0; k < count; ++k) {
+        int status;
+        const tfdtd_plane* d = p + k;
+        if (d->dtype == TFDTD_F32 && d->spectral == 0) status = launch_plane<float, float>(d, stream);
+        else if (d->dtype == TFDTD_F32 && d->spectral == 1) status = launch_plane<float, double>(d, stream);
+        else if (d->dtype == TFDTD_F64 && d->spectral == 1) status = launch_plane<double, double>(d, stream);
+        else status = fail(TFDTD_EINVAL, "plane %d: unsupported dtype/spectral pair", k);
+        if (status) return status;
+    }
+    return TFDTD_OK;
+}
+
+int observe(const tfdtd_observe* d, hipStream_t stream) {
+    if (!d) return fail(TFDTD_EINVAL, "null observe descriptor");
+    if (d->dtype == TFDTD_F32) return launch_observe<float>(d, stream);
+    if (d->dtype == TFDTD_F64) return launch_observe<double>(d, stream);
+    return fail(TFDTD_EINVAL, "dtype must be float32 or float64");
+}
+
+}  // namespace
+
+extern "C" {
+
+int tfdtd_abi_version(void) { return TFDTD_ABI_VERSION; }
+
+size_t tfdtd_last_error(char* buffer, size_t size) {
+    if (buffer && size) {
+        std::strncpy(buffer, g_error.c_str(), size - 1);
+        buffer[size - 1] = 0;
+    }
+    return g_error.size();
+}
+
+int tfdtd_yee_half_step(const tfdtd_half_step* step, void* stream) {
+    return half_step(step, static_cast<hipStream_t>(stream));
+}
+
+int tfdtd_dft_update(const tfdtd_plane* p, int32_t count, void* stream) {
+    return planes(p, count, static_cast<hipStream_t>(stream));
+}
+
+int tfdtd_observe_step(const tfdtd_observe* d, void* stream) { return observe(d, static_cast<hipStream_t>(stream)); }
+
+int tfdtd_step_fused(const tfdtd_half_step* e_step, const tfdtd_half_step* h_step, const tfdtd_plane* p,
+                     int32_t plane_count, const tfdtd_observe* obs, int32_t n_steps, void* stream) {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (n_steps < 0) return fail(TFDTD_EINVAL, "n_steps must be non-negative");
+    if (!e_step || !h_step || e_step->family != TFDTD_FAMILY_E || h_step->family != TFDTD_FAMILY_H)
+        return fail(TFDTD_EINVAL, "step_fused needs an E and an H descriptor");
+    if (!obs) return fail(TFDTD_EINVAL, "step_fused needs the observe descriptor (it advances the counter)");
+    for (int k = 0; k < n_steps; ++k) {
+        int status = half_step(e_step, s);
+        if (!status) status = half_step(h_step, s);
+        if (!status) status = planes(p, plane_count, s);
+        if (!status) status = observe(obs, s);
+        if (status) return status;
+    }
+    return TFDTD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,286 @@
+"""``Simulation(project).run()`` on the gfx950 HIP engine (torchfdtd/solver.py:642-908).
+
+The step is the reference's closure (solver.py:785-797): E update, E sources, H
+update, H sources, plane DFTs, point traces, counter += 1.  With the HIP kernel the
+sources are folded into the update kernels and one C call issues the whole step;
+HIP graphs replay 1 or `cuda_graph_steps` steps between host barriers (frames,
+run-control diagnostics, cancellation), exactly at the reference's barrier steps.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import math
+import threading
+import time
+from dataclasses import dataclass, field
+from functools import cached_property
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .graphs import StepGraphs, observation_schedule, validate_graph_steps
+from .grid import YeeGrid, configure_cuda_kernel, half_step_descriptor, resolve_cuda_kernel
+from .models import Project
+from .monitors import FrequencyPlane, apodization_window, point_spectrum
+from .planning import field_axes, index_at, source_terms, voxelize
+from .run_control import DecayDecision, StateDiagnostics, source_end_time
+
+C0 = 299792458.0
+ENGINE_LOCK = threading.Lock()
+
+
+@dataclass
+class Result:
+    """The reference's Result record (solver.py:511-639) and its NPZ layout."""
+    project: Project
+    summary: dict
+    frames: np.ndarray
+    frame_steps: np.ndarray
+    epsilon: np.ndarray
+    signals: np.ndarray
+    times: np.ndarray
+    electric: np.ndarray
+    magnetic: np.ndarray
+    frequency_fields: list = field(default_factory=list)
+
+    @property
+    def point_monitors(self):
+        return [self.project.resolved_monitor(m) for m in self.project.monitors if m.enabled and m.kind == 'point']
+
+    @cached_property
+    def spectra(self):
+        # H is sampled after its half-step update, E after its full-step update.
+        dt = self.times[1]-self.times[0] if len(self.times) > 1 else 0
+        return [point_spectrum((self.times+(dt/2 if m.component.startswith('H') else 0))[::m.time_downsample],
+                               self.signals[::m.time_downsample, k], m.spectrum)
+                for k, m in enumerate(self.point_monitors)]
+
+    def save(self, path):
+        path = Path(path)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        arrays, metadata = {}, []
+        for k, (m, spec) in enumerate(zip(self.point_monitors, self.spectra)):
+            arrays[f'monitor_{k}_frequency_hz'] = spec['frequency_hz']
+            arrays[f'monitor_{k}_spectrum'] = spec['value']
+            metadata.append({'id': m.id, 'name': m.name, 'component': m.component,
+                             'settings': m.spectrum.model_dump(), 'units': spec['units'], 'transform': spec['transform']})
+        planes = []
+        for k, plane in enumerate(self.frequency_fields):
+            planes.append({key: v for key, v in plane.items() if not isinstance(v, np.ndarray)})
+            for key, v in plane.items():
+                if isinstance(v, np.ndarray):
+                    arrays[f'field_monitor_{k}_{key}'] = v
+        np.savez_compressed(path, project=self.project.model_dump_json(), summary=json.dumps(self.summary),
+                            field_monitors=json.dumps(planes, default=_json_default),
+                            **{f'mesh_{a}_um': nodes for a, nodes in zip('xyz', self.project.region.mesh_nodes)},
+                            frames=self.frames, frame_steps=self.frame_steps, epsilon=self.epsilon,
+                            signals=self.signals, times=self.times, E=self.electric, H=self.magnetic,
+                            monitor_spectra=json.dumps(metadata), **arrays)
+
+    @classmethod
+    def load(cls, path):
+        with np.load(path, allow_pickle=False) as data:
+            planes = json.loads(str(data['field_monitors'])) if 'field_monitors' in data else []
+            for k, plane in enumerate(planes):
+                prefix = f'field_monitor_{k}_'
+                plane.update({key[len(prefix):]: data[key].copy() for key in data.files if key.startswith(prefix)})
+            return cls(Project.model_validate_json(str(data['project'])), json.loads(str(data['summary'])),
+                       *(data[key].copy() for key in ('frames', 'frame_steps', 'epsilon', 'signals', 'times', 'E', 'H')), planes)
+
+    def field_monitor(self, name_or_id):
+        matches = [m for m in self.frequency_fields if name_or_id in (m['id'], m['name'])]
+        if len(matches) != 1:
+            raise ValueError('Expected one enabled frequency plane with this name or id.')
+        return matches[0]
+
+
+def _json_default(value):
+    if isinstance(value, (np.integer,)):
+        return int(value)
+    if isinstance(value, (np.floating,)):
+        return float(value)
+    if isinstance(value, tuple):
+        return list(value)
+    raise TypeError(type(value))
+
+
+class Stepper:
+    """Device state of one resident run and its step (the reference's closure, solver.py:776-797)."""
+
+    def __init__(self, project, steps=None, kernel=None):
+        p, r = project, project.region
+        self.project = p
+        self.steps = r.steps if steps is None else steps
+        self.kernel = kernel or resolve_cuda_kernel(r)
+        self.eps = voxelize(p)
+        self.grid = g = YeeGrid(r)
+        g.set_permittivity(self.eps)
+        self.lib = N.library()
+        device, real = g.E.device, g.dtype
+        self.counter = torch.zeros(1, device=device, dtype=torch.long)
+        terms = source_terms(p, self.steps)
+        self.sources = {'E': [], 'H': []}
+        for t in terms:
+            self.sources[t.family].append((t.comp, t.box, torch.as_tensor(t.values, device=device)))
+        self.point_monitors = [p.resolved_monitor(m) for m in p.monitors if m.enabled and m.kind == 'point']
+        table = []
+        for m in self.point_monitors:
+            loc = index_at(m.center, r, m.component)
+            flat = ((loc[0]*r.shape[1]+loc[1])*r.shape[2]+loc[2])*3+'xyz'.index(m.component[1].lower())
+            table.append((int(m.component[0] == 'H'), flat))
+        self.trace_table = torch.as_tensor(np.array(table, dtype=np.int64).reshape(-1, 2), device=device)
+        self.traces = torch.zeros((self.steps, len(table)), device=device, dtype=real)
+        self.planes = [FrequencyPlane(g, p.resolved_monitor(m)) for m in p.monitors if m.enabled and m.kind == 'field']
+        if self.kernel == 'torch':
+            configure_cuda_kernel(g, 'torch')
+        self._describe()
+
+    def _describe(self):
+        g = self.grid
+        dtype = N.F64 if g.dtype == torch.float64 else N.F32
+        self.e_step = half_step_descriptor(g, False, self.sources['E'] if self.kernel == 'hip' else (), self.counter)
+        self.h_step = half_step_descriptor(g, True, self.sources['H'] if self.kernel == 'hip' else (), self.counter)
+        self.plane_table = (N.Plane*max(1, len(self.planes)))()
+        for k, plane in enumerate(self.planes):
+            self.plane_table[k] = plane.descriptor(self.counter)
+        o = self.observe = N.Observe()
+        o.dtype = dtype
+        o.count = len(self.point_monitors)
+        o.table = self.trace_table.data_ptr() if o.count else None
+        o.E, o.H = g.E.data_ptr(), g.H.data_ptr()
+        o.traces = self.traces.data_ptr() if o.count else None
+        o.steps = self.steps
+        o.counter = self.counter.data_ptr()
+
+    def states(self):
+        return [self.grid.E, self.grid.H, self.counter, self.traces, *self.grid.memory_states]
+
+    def step(self):
+        stream = N.stream_handle(self.grid.device)
+        if self.kernel == 'hip':
+            N.check(self.lib.tfdtd_step_fused(ctypes.byref(self.e_step), ctypes.byref(self.h_step), self.plane_table,
+                                              len(self.planes), ctypes.byref(self.observe), 1, stream), 'tfdtd_step_fused')
+            return
+        g = self.grid
+        g.update_E()
+        self._inject(g.E, 'E')
+        g.update_H()
+        self._inject(g.H, 'H')
+        if self.planes:
+            N.check(self.lib.tfdtd_dft_update(self.plane_table, len(self.planes), stream), 'tfdtd_dft_update')
+        N.check(self.lib.tfdtd_observe_step(ctypes.byref(self.observe), stream), 'tfdtd_observe_step')
+
+    def _inject(self, field, family):
+        for comp, box, values in self.sources[family]:
+            field[box+(comp,)] += values.index_select(0, self.counter)[0]
+
+
+class Simulation:
+    """Drop-in for the reference's Simulation (solver.py:642-670) on the HIP engine."""
+
+    def __init__(self, project: Project):
+        self.project = Project.model_validate(project.model_dump())
+
+    def run(self, progress=None, cancel=None, cuda_graph=True, cuda_graph_steps=1):
+        with ENGINE_LOCK, torch.no_grad():
+            return self._run(progress, cancel, cuda_graph, cuda_graph_steps)
+
+    def _run(self, progress, cancel, cuda_graph, cuda_graph_steps):
+        p, r = self.project, self.project.region
+        refusal = r.unsupported()
+        if refusal:
+            raise ValueError(refusal)
+        if r.complex_fields:
+            raise ValueError('Bloch (complex) fields are not run by this engine.')
+        if r.backend == 'cpu':
+            raise ValueError('This engine runs on the GPU (HIP, gfx950); backend="cpu" is the reference NumPy path.')
+        if not torch.cuda.is_available():
+            raise RuntimeError('The HIP engine requires a ROCm GPU; none is visible.')
+        validate_graph_steps(cuda_graph_steps, cuda_graph)
+        started = time.perf_counter()
+        run = Stepper(p)
+        g = run.grid
+        control = r.run_control
+        diagnostics = StateDiagnostics(g) if control.auto_shutoff or control.divergence_check else None
+        decision = DecayDecision(control, source_end_time(p), g.time_step)
+        graph = StepGraphs(run.step, run.states(), r.steps, cuda_graph_steps) if cuda_graph else None
+        axis = 'xyz'.index(r.slice_axis)
+        plane_index = index_at(tuple(r.slice_position if i == axis else 0 for i in range(3)), r, r.field)[axis]
+        component = 'xyz'.index(r.field[1].lower())
+        eps = run.eps
+
+        def plane(a):
+            sl = [slice(None)]*3
+            sl[axis] = plane_index
+            data = a[tuple(sl)]
+            return data[::max(1, math.ceil(data.shape[0]/256)), ::max(1, math.ceil(data.shape[1]/256))]
+        eps_plane = np.array(plane(eps[..., component] if eps.ndim == 4 else eps))
+        interval = max(r.snapshot_interval, math.ceil(r.steps/100))
+        frames, frame_steps = [], []
+        torch.cuda.synchronize()
+        setup_seconds = time.perf_counter()-started
+        compute_start = time.perf_counter()
+        completed, termination, diagnostic_seconds, replays = 0, 'max_steps', 0., 0
+        barriers = set(range(interval, r.steps+1, interval))
+        if diagnostics is not None:
+            barriers.update(range(control.check_interval, r.steps+1, control.check_interval))
+        width = graph.width if graph is not None else 1
+        for target, advance in observation_schedule(r.steps, width, barriers):
+            if cancel is not None and cancel.is_set():
+                termination = 'cancelled'
+                break
+            if graph is not None:
+                graph.replay(advance)
+                replays += 1
+            else:
+                run.step()
+            completed = target
+            stop = False
+            if diagnostics is not None and (completed % control.check_interval == 0 or completed == r.steps):
+                t0 = time.perf_counter()
+                try:
+                    stop = decision.update(completed, *diagnostics.measure())
+                except FloatingPointError as exc:
+                    raise FloatingPointError(f'{exc} (diagnostic step {completed})') from exc
+                diagnostic_seconds += time.perf_counter()-t0
+            if stop:
+                termination = 'decayed'
+            if completed % interval == 0 or completed == r.steps or stop:
+                frame = plane((g.E if r.field[0] == 'E' else g.H)[..., component]).cpu().numpy().copy()
+                if not np.isfinite(frame).all():
+                    raise FloatingPointError('Fields diverged. Check mesh, sources and precision.')
+                frames.append(frame)
+                frame_steps.append(completed)
+                if progress:
+                    progress({'step': completed, 'total': r.steps, 'frame': frame.tolist(),
+                              'elapsed': time.perf_counter()-compute_start,
+                              'termination_reason': termination if stop else None,
+                              'diagnostics': decision.history[-1] if decision.history else None})
+            if stop:
+                break
+        torch.cuda.synchronize()
+        seconds = time.perf_counter()-compute_start
+        e, h = g.E.cpu().numpy().copy(), g.H.cpu().numpy().copy()
+        if not np.isfinite(e).all() or not np.isfinite(h).all():
+            raise FloatingPointError('Non-finite fields detected.')
+        props = torch.cuda.get_device_properties(g.device)
+        summary = dict(shape=list(r.shape), cells=math.prod(r.shape), dt_fs=r.time_step*1e15,
+                       duration_fs=r.time_step*r.steps*1e15, backend='cuda', precision=r.precision,
+                       gpu=props.name, gpu_arch=getattr(props, 'gcnArchName', None),
+                       cuda_graph=graph is not None, cuda_graph_steps=width, cuda_graph_replays=replays,
+                       cuda_kernel=run.kernel, cuda_monitor_kernel='hip', engine='torchfdtd_amd HIP gfx950',
+                       steps=completed, requested_steps=r.steps, cancelled=termination == 'cancelled',
+                       termination_reason=termination, auto_shutoff=termination == 'decayed',
+                       diagnostics=decision.history, diagnostic_seconds=diagnostic_seconds,
+                       diagnostic_backend=diagnostics.backend if diagnostics is not None else None,
+                       seconds=seconds, setup_seconds=setup_seconds,
+                       mcells_per_second=math.prod(r.shape)*completed/max(seconds, 1e-9)/1e6,
+                       field_peak=float(np.max(np.abs(e))), slice_index=plane_index,
+                       material_sampling=r.material_sampling, boundaries=r.boundaries.model_dump(),
+                       units='geometry: um; time: s; E/H: reduced fields; Bloch phase: rad', warnings=[])
+        planes = [m.result() for m in run.planes]
+        return Result(p, summary, np.array(frames), np.array(frame_steps), eps_plane,
+                      run.traces[:completed].cpu().numpy(), np.arange(1, completed+1)*g.time_step, e, h, planes)
