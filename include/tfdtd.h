@@ -67,6 +67,7 @@ typedef struct tfdtd_half_step {
     tfdtd_cpml_segment seg[TFDTD_MAX_SEGMENTS];
     int32_t nsrc;
     tfdtd_source srcs[TFDTD_MAX_SOURCES];
+    int64_t source_samples;         /* length of every values array; no drive at or beyond it */
     const int64_t* counter;         /* device step counter read for source samples */
 } tfdtd_half_step;
 
@@ -101,6 +102,7 @@ typedef struct tfdtd_plane {
     const double* omega;            /* double[nf]: 2 pi f dt */
     const void* half;               /* spectral complex[nf]: exp(i pi f dt) */
     void* value;                    /* spectral complex[nf][n][nc], accumulated */
+    int64_t samples;                /* length of the windows; no accumulation at or beyond it */
     const int64_t* counter;
 } tfdtd_plane;
 

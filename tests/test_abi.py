@@ -52,8 +52,8 @@ int main(void) {
   printf("tfdtd_observe %zu\n", sizeof(tfdtd_observe));
   printf("tfdtd_cpml_segment %zu\n", sizeof(tfdtd_cpml_segment));
   printf("tfdtd_source %zu\n", sizeof(tfdtd_source));
-  F(tfdtd_half_step, courant); F(tfdtd_half_step, seg); F(tfdtd_half_step, srcs); F(tfdtd_half_step, counter);
-  F(tfdtd_plane, points); F(tfdtd_plane, indices); F(tfdtd_plane, counter);
+  F(tfdtd_half_step, courant); F(tfdtd_half_step, seg); F(tfdtd_half_step, srcs); F(tfdtd_half_step, source_samples); F(tfdtd_half_step, counter);
+  F(tfdtd_plane, points); F(tfdtd_plane, indices); F(tfdtd_plane, samples); F(tfdtd_plane, counter);
   F(tfdtd_observe, steps); F(tfdtd_observe, counter);
   return 0;
 }

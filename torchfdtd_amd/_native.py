@@ -64,7 +64,7 @@ class HalfStep(ctypes.Structure):
                 ('pec_upper_factor', ctypes.c_double*3),
                 ('nseg', i32), ('seg', CpmlSegment*MAX_SEGMENTS),
                 ('nsrc', i32), ('srcs', SourceBox*MAX_SOURCES),
-                ('counter', vp)]
+                ('source_samples', i64), ('counter', vp)]
 
 
 class Observe(ctypes.Structure):
@@ -77,7 +77,7 @@ class Plane(ctypes.Structure):
                 ('components', i32), ('electric', i32), ('corners', i32), ('time_downsample', i32),
                 ('E', vp), ('H', vp), ('indices', vp*6), ('weights', vp*6),
                 ('window_e', vp), ('window_h', vp), ('omega', vp), ('half', vp), ('value', vp),
-                ('counter', vp)]
+                ('samples', i64), ('counter', vp)]
 
 
 _lib = None

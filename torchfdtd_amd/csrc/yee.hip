@@ -70,6 +70,7 @@ struct HalfStep {
     int nsrc;
     int box_lo[3], box_hi[3];   // union of the source boxes
     Source<real> srcs[TFDTD_MAX_SOURCES];
+    long long source_samples;
     const long long* counter;
 };
 
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(256) void yee_half_step_kernel(const HalfStep<real>
     if (a.nsrc > 0 && x >= a.box_lo[0] && x < a.box_hi[0] && y >= a.box_lo[1] && y < a.box_hi[1] &&
         z >= a.box_lo[2] && z < a.box_hi[2]) {
         const long long n = *a.counter;
-        for (int k = 0; k < a.nsrc; ++k) {
+        for (int k = 0; k < a.nsrc && n < a.source_samples; ++k) {
             const Source<real>& src = a.srcs[k];
             if (x >= src.lo[0] && x < src.hi[0] && y >= src.lo[1] && y < src.hi[1] && z >= src.lo[2] &&
                 z < src.hi[2]) {
@@ -188,6 +189,7 @@ struct Plane {
     const double* omega;
     const spec* half;   // interleaved re, im
     spec* value;        // interleaved re, im
+    long long samples;
     const long long* counter;
 };
 
@@ -198,7 +200,7 @@ __global__ __launch_bounds__(256) void dft_plane_kernel(const Plane<real, spec> 
     extern __shared__ unsigned char lds_raw[];
     spec* phase = reinterpret_cast<spec*>(lds_raw);
     const long long n = *a.counter;
-    if (n % a.downsample) return;
+    if (n >= a.samples || n % a.downsample) return;
     for (int f = threadIdx.x; f < a.nf; f += blockDim.x) {
         double si, co;
         sincos(a.omega[f] * (double)(n + 1), &si, &co);
@@ -285,7 +287,9 @@ int make_half_step(const tfdtd_half_step* d, HalfStep<real>* a) {
             if (s.hi[ax] > a->box_hi[ax]) a->box_hi[ax] = s.hi[ax];
         }
     }
-    if (d->nsrc > 0 && !d->counter) return fail(TFDTD_EINVAL, "sources need the device step counter");
+    if (d->nsrc > 0 && (!d->counter || d->source_samples < 0))
+        return fail(TFDTD_EINVAL, "sources need the device step counter and their sample count");
+    a->source_samples = d->source_samples;
     a->counter = reinterpret_cast<const long long*>(d->counter);
     return TFDTD_OK;
 }
@@ -339,6 +343,8 @@ int launch_plane(const tfdtd_plane* d, hipStream_t stream) {
     a.window_e = static_cast<const real*>(d->window_e); a.window_h = static_cast<const real*>(d->window_h);
     a.omega = d->omega; a.half = static_cast<const spec*>(d->half); a.value = static_cast<spec*>(d->value);
     a.counter = reinterpret_cast<const long long*>(d->counter);
+    if (d->samples < 0) return fail(TFDTD_EINVAL, "plane: negative window length");
+    a.samples = d->samples;
     const long long samples = d->points * d->components;
     const long long blocks = (samples + 255) / 256;
     hipLaunchKernelGGL((dft_plane_kernel<real, spec>), dim3((unsigned)blocks), dim3(256), lds, stream, a);

@@ -116,6 +116,10 @@ def half_step_descriptor(grid, forward, sources=(), counter=None):
         for a in range(3):
             s.lo[a], s.hi[a] = box[a].start, box[a].stop
         s.values = _ptr(values)
+        if k == 0:
+            d.source_samples = values.numel()
+        elif values.numel() != d.source_samples:
+            raise ValueError('every source waveform of one family must have the same length')
     d.nsrc = len(sources)
     d.counter = _ptr(counter) if counter is not None else None
     return d

@@ -202,6 +202,7 @@ class FrequencyPlane:
             d.weights[k] = weights.data_ptr()
         d.window_e, d.window_h = self.windows[0].data_ptr(), self.windows[1].data_ptr()
         d.omega, d.half, d.value = self.omega.data_ptr(), self.half_phase.data_ptr(), self.value.data_ptr()
+        d.samples = self.windows[0].numel()
         d.counter = counter.data_ptr()
         return d
 
