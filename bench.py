@@ -1,0 +1,160 @@
+"""Benchmark of the fused 3D Yee/CPML step on one MI355X: the 512^3 roofline cube.
+
+Workload: 512^3 cells FP32, 16-cell CPML on all faces, per-cell eps = 1 + 11 s
+(s = smoothed, thresholded seeded noise), Ez dipole at the centre.  One bench step is
+one Yee time step (E and H of every cell, CPML, source, counter) replayed from a HIP
+graph; the configuration runs 500 of them.  Inputs are resident in HBM before timing.
+
+Prints one JSON line: Gcell-updates/s over the timed steps, the roofline of the
+dominant kernel (algorithmic bytes per launch / its HIP-event-timed duration against
+the 8 TB/s HBM peak) and the NumPy oracle of the reference CPU path timed on a bounded
+slab of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(region, real_bytes=4):
+    """Per cell-update: E,H read+write (12 values), eps^-1 per cell, CPML psi read+write."""
+    from torchfdtd_amd.planning import psi_elements
+    cells = math.prod(region.shape)
+    k_eps = 3 if region.material_sampling == 'yee' else 1
+    psi = psi_elements(region)
+    return real_bytes*(12*cells+k_eps*cells+2*psi), psi/cells
+
+
+def cpu_baseline(seconds_budget, mask):
+    """Oracle (NumPy, one thread) on a 512x512x64 slab of the cube with the same CPML: cell-updates/s."""
+    from oracle.scene import Scene
+    from oracle.yee import Grid
+    nz = 64
+    h = .04
+    project = {'region': {'dimension': '3d', 'size': (512*h, 512*h, nz*h), 'mesh': h, 'pml_cells': 16, 'steps': 10}}
+    scene = Scene(project)
+    inv = np.where(mask[:, :, :nz].cpu().numpy(), np.float32(1)/np.float32(12), np.float32(1)).astype(np.float32)
+    g = Grid(scene, np.repeat(inv[..., None], 3, axis=-1))
+    rng = np.random.default_rng(1)
+    g.E[:] = rng.normal(size=g.E.shape).astype(np.float32)*1e-2
+    g.H[:] = rng.normal(size=g.H.shape).astype(np.float32)*1e-2
+    g.update_E()
+    g.update_H()
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        g.update_E()
+        g.update_H()
+        steps += 1
+        elapsed = time.perf_counter()-t0
+        if elapsed > seconds_budget or steps >= 20:
+            break
+    cells = math.prod(scene.shape)
+    return dict(value=cells*steps/elapsed/1e9, unit='Gcell-updates/s', cores=1, kind='port',
+                sample=f'NumPy oracle of the reference CPU path (FP32, one thread of {os.cpu_count()}), '
+                       f'512x512x{nz} slab of the 512^3 cube with its 16-cell CPML, {steps} timed steps after 1 warm-up, '
+                       f'{elapsed:.1f} s')
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=500)
+    ap.add_argument('--warmup', type=int, default=20)
+    ap.add_argument('--n', type=int, default=512)
+    ap.add_argument('--graph-width', type=int, default=50)
+    ap.add_argument('--cpu-seconds', type=float, default=15.)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py measures one GPU: the fused Yee step runs the whole grid on one MI355X '
+              f'(--gpus {args.gpus} is not run; see the sharding paragraph of the design notes).', file=sys.stderr)
+        return 2
+    import torch
+    from torchfdtd_amd import _native as N
+    from torchfdtd_amd.graphs import StepGraphs
+    from torchfdtd_amd.solver import Stepper
+    from torchfdtd_amd.workloads import cube, cube_inverse_permittivity, cube_noise_mask
+
+    project = cube(n=args.n, steps=args.steps+args.warmup+8)
+    region = project.region
+    print(f'bench: building {args.n}^3 cube, {region.pml_cells}-cell CPML', file=sys.stderr, flush=True)
+    mask = cube_noise_mask(args.n)
+    run = Stepper(project)
+    g = run.grid
+    g.inverse_permittivity.copy_(cube_inverse_permittivity(mask)[..., None])
+    torch.cuda.synchronize()
+    width = args.graph_width
+    graph = StepGraphs(run.step, run.states(), region.steps, width, warmup=1)
+    print('bench: graphs captured', file=sys.stderr, flush=True)
+
+    def advance(steps):
+        while steps >= width:
+            graph.replay(width)
+            steps -= width
+        for _ in range(steps):
+            graph.replay(1)
+
+    advance(args.warmup)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    wall = time.perf_counter()
+    start.record()
+    advance(args.steps)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter()-wall
+    seconds = start.elapsed_time(stop)/1e3
+    cells = math.prod(region.shape)
+    value = cells*args.steps/seconds/1e9
+    assert torch.isfinite(g.E).all() and torch.isfinite(g.H).all()
+
+    # Per-kernel durations with HIP events on the launch stream (the kernels are launched on torch's current stream).
+    lib = N.library()
+    stream = N.stream_handle()
+    kernel_ms = {}
+    for name, desc in (('yee_E', run.e_step), ('yee_H', run.h_step)):
+        reps = 20
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            N.check(lib.tfdtd_yee_half_step(ctypes.byref(desc), stream), name)
+        e1.record()
+        torch.cuda.synchronize()
+        kernel_ms[name] = e0.elapsed_time(e1)/reps
+    step_bytes, psi_per_cell = algorithmic_bytes(region)
+    per_step_ms = sum(kernel_ms.values())
+    achieved = step_bytes/(per_step_ms/1e3)/1e9
+    roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
+                    frac=round(achieved/HBM_PEAK_GBS, 4), traffic=None,
+                    kernel='yee_half_step E+H (one Yee step)', kernel_ms=kernel_ms,
+                    bytes_per_cell_update=round(step_bytes/cells, 2), psi_per_cell=round(psi_per_cell, 4))
+    baseline = None
+    if not args.no_cpu_baseline:
+        print('bench: CPU baseline (NumPy oracle)', file=sys.stderr, flush=True)
+        baseline = cpu_baseline(args.cpu_seconds, mask)
+    line = {'metric': 'Gcell-updates/s (fused 3D Yee+CPML step, 512^3 FP32, %HBM in roofline)',
+            'value': round(value, 3), 'unit': 'Gcell-updates/s', 'n_gpus': 1, 'steps': args.steps,
+            'warmup': args.warmup, 'ms_per_step': round(seconds*1e3/args.steps, 4), 'higher_is_better': True,
+            'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic (seeded smoothed-threshold eps noise)',
+            'config': {'workload': f'{args.n}^3 cube, 16-cell CPML all faces, eps=1+11s per cell, Ez dipole, '
+                                   f'{region.steps - 8} steps of which {args.steps} timed',
+                       'cells': cells, 'graph_steps': width, 'wall_s': round(wall, 3)},
+            'roofline': roofline, 'cpu_baseline': baseline}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == '__main__':
+    sys.exit(main())
