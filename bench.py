@@ -96,8 +96,9 @@ def main():
     g.inverse_permittivity.copy_(cube_inverse_permittivity(mask)[..., None])
     torch.cuda.synchronize()
     width = args.graph_width
-    graph = StepGraphs(run.step, run.states(), region.steps, width, warmup=1)
-    print('bench: graphs captured', file=sys.stderr, flush=True)
+    graph = StepGraphs(run.step, run.states(), region.steps, width, warmup=1,
+                       pair=run.step_pair if run.one_pass else None)
+    print(f'bench: graphs captured (one-pass kernel: {run.one_pass})', file=sys.stderr, flush=True)
 
     def advance(steps):
         while steps >= width:
@@ -124,21 +125,32 @@ def main():
     lib = N.library()
     stream = N.stream_handle()
     kernel_ms = {}
-    for name, desc in (('yee_E', run.e_step), ('yee_H', run.h_step)):
-        reps = 20
+    reps = 20
+    launches = [('yee_half_step_E', lambda: lib.tfdtd_yee_half_step(ctypes.byref(run.e_step), stream)),
+                ('yee_half_step_H', lambda: lib.tfdtd_yee_half_step(ctypes.byref(run.h_step), stream))]
+    if run.one_pass:
+        launches.append(('march', None))
+    for name, launch in launches:
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        for _ in range(reps):
-            N.check(lib.tfdtd_yee_half_step(ctypes.byref(desc), stream), name)
+        for k in range(reps):
+            if launch is None:
+                status = lib.tfdtd_yee_step_march(ctypes.byref(run.march[k % 2]), stream)
+            else:
+                status = launch()
+            N.check(status, name)
         e1.record()
         torch.cuda.synchronize()
         kernel_ms[name] = e0.elapsed_time(e1)/reps
     step_bytes, psi_per_cell = algorithmic_bytes(region)
-    per_step_ms = sum(kernel_ms.values())
+    if run.one_pass:
+        per_step_ms, kernel = kernel_ms['march'], 'march_kernel (one launch = one Yee step of every cell)'
+    else:
+        per_step_ms, kernel = kernel_ms['yee_half_step_E']+kernel_ms['yee_half_step_H'], 'yee_half_step E+H'
     achieved = step_bytes/(per_step_ms/1e3)/1e9
     roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
                     frac=round(achieved/HBM_PEAK_GBS, 4), traffic=None,
-                    kernel='yee_half_step E+H (one Yee step)', kernel_ms=kernel_ms,
+                    kernel=kernel, kernel_ms=kernel_ms,
                     bytes_per_cell_update=round(step_bytes/cells, 2), psi_per_cell=round(psi_per_cell, 4))
     baseline = None
     if not args.no_cpu_baseline:

@@ -106,6 +106,57 @@ typedef struct tfdtd_plane {
     const int64_t* counter;
 } tfdtd_plane;
 
+/* CPML slabs of one family for the one-pass step.  Every psi array of the family lies
+ * in one buffer and every coefficient array in another; the two curl terms that
+ * differentiate along one axis share their rows and coefficients on each face. */
+typedef struct tfdtd_march_face {
+    int32_t lo, hi;                 /* target rows [lo, hi) along the axis; lo == hi: no slab */
+    int64_t psi[2];                 /* element offsets of the two terms' psi arrays, curl order */
+    int64_t coef;                   /* element offset of b[hi-lo], then c[hi-lo], then inv_k[hi-lo] */
+} tfdtd_march_face;
+
+typedef struct tfdtd_march_family {
+    const void* psi_in;             /* psi at time n */
+    void* psi_out;                  /* psi at time n+1: a separate buffer for E, psi_in for H */
+    const void* coef;
+    tfdtd_march_face face[3][2];    /* [axis][low face, high face] */
+} tfdtd_march_family;
+
+/* One whole Yee step -- E, then H, of every cell -- in a single launch.  Time n is read
+ * from the *_in buffers and time n+1 written to the *_out buffers, which may not alias
+ * them; the caller alternates the two sets (and the E psi buffers) between steps.  The
+ * H psi buffer updates in place.  Periodic axes are not supported here (use the half
+ * steps), and 3 x cells must stay below 2^32. */
+typedef struct tfdtd_march {
+    int32_t dtype;
+    int32_t nx, ny, nz;             /* every axis >= 2 */
+    const void* e_in;
+    const void* h_in;
+    void* e_out;
+    void* h_out;
+    const void* inverse_permittivity;
+    int32_t eps_stride;             /* 1 per cell, 3 per cell and component */
+    const void* inverse_permeability; /* one value */
+    double courant;
+    int32_t pec_upper_mask;
+    double pec_upper_factor[3];
+    tfdtd_march_family e;
+    tfdtd_march_family h;
+    int32_t nsrc_e;                 /* at most 8 per family */
+    tfdtd_source srcs_e[TFDTD_MAX_SOURCES];
+    int32_t nsrc_h;
+    tfdtd_source srcs_h[TFDTD_MAX_SOURCES];
+    int64_t source_samples;
+    const int64_t* counter;
+    int32_t tile_y;                 /* rows per wavefront: 4, 8 or 16 */
+    int32_t chunk_x;                /* planes per workgroup; 0 = the whole x axis */
+} tfdtd_march;
+
+/* torchfdtd/reversible_cuda_fused.py:259-325 (FusedEHMarching) generalised to the
+ * CPML, PEC and source layouts of torchfdtd/cuda_kernels.py:150-259: E and H of one
+ * step in one pass over HBM. */
+int tfdtd_yee_step_march(const tfdtd_march* step, void* stream);
+
 /* torchfdtd/cuda_kernels.py:349-365 (FusedYeeCUDA.update_E / update_H), with the
  * family's sources of torchfdtd/solver.py:778-783 folded in when nsrc > 0. */
 int tfdtd_yee_half_step(const tfdtd_half_step* step, void* stream);

@@ -55,6 +55,12 @@ int main(void) {
   F(tfdtd_half_step, courant); F(tfdtd_half_step, seg); F(tfdtd_half_step, srcs); F(tfdtd_half_step, source_samples); F(tfdtd_half_step, counter);
   F(tfdtd_plane, points); F(tfdtd_plane, indices); F(tfdtd_plane, samples); F(tfdtd_plane, counter);
   F(tfdtd_observe, steps); F(tfdtd_observe, counter);
+  printf("tfdtd_march %zu\n", sizeof(tfdtd_march));
+  printf("tfdtd_march_face %zu\n", sizeof(tfdtd_march_face));
+  printf("tfdtd_march_family %zu\n", sizeof(tfdtd_march_family));
+  F(tfdtd_march, courant); F(tfdtd_march, e); F(tfdtd_march, h); F(tfdtd_march, nsrc_e); F(tfdtd_march, srcs_h);
+  F(tfdtd_march, source_samples); F(tfdtd_march, counter); F(tfdtd_march, chunk_x);
+  F(tfdtd_march_family, face); F(tfdtd_march_face, psi); F(tfdtd_march_face, coef);
   return 0;
 }
 '''
@@ -67,7 +73,8 @@ def test_ctypes_mirror_matches_the_c_layout(tmp_path):
     subprocess.run(['gcc', '-std=c11', '-I', str(HEADER.parent), str(source), '-o', str(exe)], check=True)
     lines = dict(line.rsplit(' ', 1) for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split('\n') if line)
     mirror = {'tfdtd_half_step': N.HalfStep, 'tfdtd_plane': N.Plane, 'tfdtd_observe': N.Observe,
-              'tfdtd_cpml_segment': N.CpmlSegment, 'tfdtd_source': N.SourceBox}
+              'tfdtd_cpml_segment': N.CpmlSegment, 'tfdtd_source': N.SourceBox, 'tfdtd_march': N.March,
+              'tfdtd_march_face': N.MarchFace, 'tfdtd_march_family': N.MarchFamily}
     for key, value in lines.items():
         if '.' in key:
             struct, member = key.split('.')

@@ -80,9 +80,29 @@ class Plane(ctypes.Structure):
                 ('samples', i64), ('counter', vp)]
 
 
+class MarchFace(ctypes.Structure):
+    _fields_ = [('lo', i32), ('hi', i32), ('psi', i64*2), ('coef', i64)]
+
+
+class MarchFamily(ctypes.Structure):
+    _fields_ = [('psi_in', vp), ('psi_out', vp), ('coef', vp), ('face', (MarchFace*2)*3)]
+
+
+class March(ctypes.Structure):
+    _fields_ = [('dtype', i32), ('nx', i32), ('ny', i32), ('nz', i32),
+                ('e_in', vp), ('h_in', vp), ('e_out', vp), ('h_out', vp),
+                ('inverse_permittivity', vp), ('eps_stride', i32), ('inverse_permeability', vp),
+                ('courant', ctypes.c_double), ('pec_upper_mask', i32), ('pec_upper_factor', ctypes.c_double*3),
+                ('e', MarchFamily), ('h', MarchFamily),
+                ('nsrc_e', i32), ('srcs_e', SourceBox*MAX_SOURCES),
+                ('nsrc_h', i32), ('srcs_h', SourceBox*MAX_SOURCES),
+                ('source_samples', i64), ('counter', vp), ('tile_y', i32), ('chunk_x', i32)]
+
+
 _lib = None
 
 PROTOTYPES = {
+    'tfdtd_yee_step_march': (i32, [ctypes.POINTER(March), vp]),
     'tfdtd_abi_version': (i32, []),
     'tfdtd_last_error': (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
     'tfdtd_yee_half_step': (i32, [ctypes.POINTER(HalfStep), vp]),

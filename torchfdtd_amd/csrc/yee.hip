@@ -13,32 +13,18 @@
 #include <string>
 
 #include "../../include/tfdtd.h"
+#include "common.h"
 
 namespace {
 
-thread_local std::string g_error;
+using ::tfdtd_fail;
 
-int fail(int code, const char* fmt, ...) {
-    char buffer[512];
-    va_list args;
-    va_start(args, fmt);
-    vsnprintf(buffer, sizeof buffer, fmt, args);
-    va_end(args);
-    g_error = buffer;
-    return code;
-}
+int fail(int code, const char* msg) { return tfdtd_fail(code, "%s", msg); }
 
-int check_launch(const char* what) {
-    hipError_t status = hipGetLastError();
-    if (status != hipSuccess) return fail(TFDTD_EHIP, "%s: %s", what, hipGetErrorString(status));
-    return TFDTD_OK;
-}
+template <typename... A>
+int fail(int code, const char* fmt, A... args) { return tfdtd_fail(code, fmt, args...); }
 
-// (derivative axis, input component, output component, sign): torchfdtd/boundaries.py:21-23.
-__host__ __device__ constexpr int term_axis(int t) { return t / 2; }
-__host__ __device__ constexpr int term_in(int t) { return (t == 0 || t == 2) ? 2 : (t == 1 || t == 4) ? 1 : 0; }
-__host__ __device__ constexpr int term_out(int t) { return (t == 2 || t == 4) ? 0 : (t == 0 || t == 5) ? 1 : 2; }
-__host__ __device__ constexpr int term_sign(int t) { return (t == 0 || t == 3 || t == 4) ? -1 : 1; }
+int check_launch(const char* what) { return tfdtd_check_launch(what); }
 
 template <typename real>
 struct Slab {          // one CPML segment in its fixed slot 2*term+side
@@ -254,21 +240,12 @@ int make_half_step(const tfdtd_half_step* d, HalfStep<real>* a) {
     a->pec_mask = d->pec_upper_mask & 7;
     for (int k = 0; k < 3; ++k) a->pec_factor[k] = (real)d->pec_upper_factor[k];
     const int dims[3] = {d->nx, d->ny, d->nz};
+    int slot_lo[12] = {0}, slot_hi[12] = {0};
     for (int k = 0; k < d->nseg; ++k) {
         const tfdtd_cpml_segment& s = d->seg[k];
-        int term = -1;
-        for (int t = 0; t < 6; ++t)
-            if (term_axis(t) == s.axis && term_in(t) == s.comp) term = t;
-        if (term < 0) return fail(TFDTD_EINVAL, "segment %d: no curl term differentiates component %d along axis %d", k, s.comp, s.axis);
-        if (s.lo < 0 || s.hi <= s.lo || s.hi > dims[s.axis]) return fail(TFDTD_EINVAL, "segment %d: rows [%d,%d) outside the axis", k, s.lo, s.hi);
+        const int slot = tfdtd_assign_slot(s.axis, s.comp, s.lo, s.hi, dims, k, slot_lo, slot_hi);
+        if (slot < 0) return TFDTD_EINVAL;
         if (!s.psi || !s.b || !s.c || !s.inv_k) return fail(TFDTD_EINVAL, "segment %d: null pointer", k);
-        int slot = 2 * term;
-        if (a->slab[slot].hi != a->slab[slot].lo) {
-            ++slot;
-            if (a->slab[slot].hi != a->slab[slot].lo) return fail(TFDTD_EINVAL, "segment %d: more than two slabs for one curl term", k);
-            const Slab<real>& other = a->slab[slot - 1];
-            if (s.lo < other.hi && other.lo < s.hi) return fail(TFDTD_EINVAL, "segment %d overlaps another slab of its term", k);
-        }
         a->slab[slot] = Slab<real>{s.lo, s.hi, static_cast<real*>(s.psi), static_cast<const real*>(s.b),
                                    static_cast<const real*>(s.c), static_cast<const real*>(s.inv_k)};
     }
@@ -387,11 +364,12 @@ extern "C" {
 int tfdtd_abi_version(void) { return TFDTD_ABI_VERSION; }
 
 size_t tfdtd_last_error(char* buffer, size_t size) {
+    const std::string& message = tfdtd_detail::last_error;
     if (buffer && size) {
-        std::strncpy(buffer, g_error.c_str(), size - 1);
+        std::strncpy(buffer, message.c_str(), size - 1);
         buffer[size - 1] = 0;
     }
-    return g_error.size();
+    return message.size();
 }
 
 int tfdtd_yee_half_step(const tfdtd_half_step* step, void* stream) {

@@ -30,9 +30,13 @@ def observation_schedule(steps, width, barriers=()):
 
 
 class StepGraphs:
-    """Capture `step` for 1 and `width` steps on a side stream; `states` are zeroed after warm-up and capture."""
+    """Capture `step` for 1 and `width` steps on a side stream; `states` are zeroed after warm-up and capture.
 
-    def __init__(self, step, states, steps, width=1, warmup=3):
+    With `pair` (two steps that return the state to its own buffers) the `width` graph
+    is built from pairs, plus one `step` when `width` is odd.
+    """
+
+    def __init__(self, step, states, steps, width=1, warmup=3, pair=None):
         self.width = min(validate_graph_steps(width), steps)
         states = list(states)
         stream = torch.cuda.Stream()
@@ -49,8 +53,14 @@ class StepGraphs:
             graph = torch.cuda.CUDAGraph()
             with torch.cuda.stream(stream):
                 graph.capture_begin()
-                for _ in range(count):
-                    step()
+                if pair is not None and count > 1:
+                    for _ in range(count // 2):
+                        pair()
+                    if count % 2:
+                        step()
+                else:
+                    for _ in range(count):
+                        step()
                 graph.capture_end()
             self.graphs[count] = graph
             for value in states:

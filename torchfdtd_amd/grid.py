@@ -50,16 +50,43 @@ class YeeGrid:
         self.pec_upper = pec_upper(region)
         self.cpml = {}
         self.memory_states = []
-        for key, slabs in cpml_slabs(region).items():
-            entries = []
-            for s in slabs:
-                psi = torch.zeros(s.psi_shape(shape), device=self.device, dtype=self.dtype)
-                self.memory_states.append(psi)
-                coefficient = lambda a: torch.as_tensor(a, device=self.device)
-                entries.append({'slab': s, 'psi': psi, 'b': coefficient(s.b), 'c': coefficient(s.c),
-                                'inv_k': coefficient(s.inv_k), 'side': s.side, 'lo': s.lo, 'hi': s.hi,
-                                'kappa': s.kappa, 'sigma': s.sigma, 'alpha': s.alpha})
-            self.cpml[key] = entries
+        # All psi arrays of a family share one buffer (each slab a contiguous view), and the
+        # two curl terms along an axis share one coefficient block per face: b, c, inv_k.
+        table = cpml_slabs(region)
+        self.psi_buffers, self.coef_buffers = {}, {}
+        for forward in (False, True):
+            keys = [(forward, axis, comp) for axis, comp, _, _ in CURL_TERMS if (forward, axis, comp) in table]
+            psi_total = sum(math.prod(s.psi_shape(shape)) for k in keys for s in table[k])
+            blocks, coef = {}, []
+            for key in keys:
+                for s in table[key]:
+                    face = (s.axis, s.side)
+                    if face in blocks:
+                        first = blocks[face][1]
+                        if (first.lo, first.hi) != (s.lo, s.hi) or not all(
+                                np.array_equal(getattr(first, n), getattr(s, n)) for n in ('b', 'c', 'inv_k')):
+                            raise ValueError('CPML slabs of one axis and face must share rows and coefficients')
+                    else:
+                        blocks[face] = (sum(len(c) for c in coef), s)
+                        coef.extend([s.b, s.c, s.inv_k])
+            psi_buffer = torch.zeros(max(psi_total, 1), device=self.device, dtype=self.dtype)
+            coef_host = np.concatenate(coef) if coef else np.zeros(1, real_dtype(region))
+            coef_buffer = torch.as_tensor(coef_host, device=self.device)
+            self.psi_buffers[forward], self.coef_buffers[forward] = psi_buffer, coef_buffer
+            offset = 0
+            for key in keys:
+                entries = []
+                for s in table[key]:
+                    size = math.prod(s.psi_shape(shape))
+                    psi = psi_buffer[offset:offset+size].view(s.psi_shape(shape))
+                    self.memory_states.append(psi)
+                    w = s.hi-s.lo
+                    c0 = blocks[(s.axis, s.side)][0]
+                    entries.append({'slab': s, 'psi': psi, 'psi_offset': offset, 'coef_offset': c0,
+                                    'b': coef_buffer[c0:c0+w], 'c': coef_buffer[c0+w:c0+2*w], 'inv_k': coef_buffer[c0+2*w:c0+3*w],
+                                    'side': s.side, 'lo': s.lo, 'hi': s.hi, 'kappa': s.kappa, 'sigma': s.sigma, 'alpha': s.alpha})
+                    offset += size
+                self.cpml[key] = entries
 
     def set_permittivity(self, eps):
         """inverse_permittivity = 1/eps, with the reciprocal taken in the field precision (solver.py:742-745)."""
@@ -122,6 +149,75 @@ def half_step_descriptor(grid, forward, sources=(), counter=None):
             raise ValueError('every source waveform of one family must have the same length')
     d.nsrc = len(sources)
     d.counter = _ptr(counter) if counter is not None else None
+    return d
+
+
+MARCH_SOURCES = 8
+
+
+def march_supported(grid, sources=None):
+    """Whether the one-pass kernel runs this grid: 3D, no periodic axis, 32-bit offsets, <= 8 sources per family."""
+    r = grid.region
+    if r.dimension != '3d' or min(r.shape) < 2 or grid.wrap:
+        return False
+    if 3*math.prod(r.shape) >= 2**32 or max(b.numel() for b in grid.psi_buffers.values()) >= 2**32:
+        return False
+    if sources is not None and any(len(v) > MARCH_SOURCES for v in sources.values()):
+        return False
+    return True
+
+
+def march_family(grid, forward, psi_in, psi_out):
+    """tfdtd_march_family: offsets of every slab in the family's psi and coefficient buffers."""
+    f = N.MarchFamily()
+    f.psi_in, f.psi_out, f.coef = _ptr(psi_in), _ptr(psi_out), _ptr(grid.coef_buffers[forward])
+    for axis in range(3):
+        terms = [(forward, a, comp) for a, comp, _, _ in CURL_TERMS if a == axis]
+        for side in (0, 1):
+            entries = [next((e for e in grid.cpml.get(t, []) if e['side'] == side), None) for t in terms]
+            if entries[0] is None:
+                continue
+            face = f.face[axis][side]
+            face.lo, face.hi = entries[0]['lo'], entries[0]['hi']
+            face.psi[0], face.psi[1] = entries[0]['psi_offset'], entries[1]['psi_offset']
+            face.coef = entries[0]['coef_offset']
+    return f
+
+
+def march_descriptor(grid, e_in, h_in, e_out, h_out, psi_e_in, psi_e_out, sources, counter, tile_y=4, chunk_x=64):
+    """tfdtd_march: one whole step reading (e_in, h_in, psi_e_in) and writing (e_out, h_out, psi_e_out)."""
+    d = N.March()
+    d.dtype = N.F64 if grid.dtype == torch.float64 else N.F32
+    d.nx, d.ny, d.nz = grid.Nx, grid.Ny, grid.Nz
+    d.e_in, d.h_in, d.e_out, d.h_out = (_ptr(t) for t in (e_in, h_in, e_out, h_out))
+    d.inverse_permittivity = _ptr(grid.inverse_permittivity)
+    d.eps_stride = grid.inverse_permittivity.shape[-1]
+    d.inverse_permeability = _ptr(grid.inverse_permeability)
+    d.courant = grid.courant_number
+    d.pec_upper_mask = sum(1 << a for a in grid.pec_upper)
+    for a, f in grid.pec_upper.items():
+        d.pec_upper_factor[a] = f
+    d.e = march_family(grid, False, psi_e_in, psi_e_out)
+    d.h = march_family(grid, True, grid.psi_buffers[True], grid.psi_buffers[True])
+    samples = 0
+    for family, count_name, table_name in (('E', 'nsrc_e', 'srcs_e'), ('H', 'nsrc_h', 'srcs_h')):
+        entries = sources.get(family, ())
+        if len(entries) > MARCH_SOURCES:
+            raise ValueError(f'the one-pass kernel folds at most {MARCH_SOURCES} sources per family')
+        table = getattr(d, table_name)
+        for k, (comp, box, values) in enumerate(entries):
+            s = table[k]
+            s.comp = comp
+            for a in range(3):
+                s.lo[a], s.hi[a] = box[a].start, box[a].stop
+            s.values = _ptr(values)
+            if samples and values.numel() != samples:
+                raise ValueError('every source waveform must have the same length')
+            samples = values.numel()
+        setattr(d, count_name, len(entries))
+    d.source_samples = samples
+    d.counter = _ptr(counter)
+    d.tile_y, d.chunk_x = tile_y, chunk_x
     return d
 
 

@@ -185,7 +185,8 @@ class FrequencyPlane:
         self.windows = [torch.as_tensor(apodization_window(times+shift, monitor.spectrum)*r.time_step*mask, device=device, dtype=real)
                         for shift in (0, r.time_step/2)]
 
-    def descriptor(self, counter):
+    def descriptor(self, counter, E=None, H=None):
+        """tfdtd_plane sampling `E`/`H` (the grid's fields by default)."""
         d = N.Plane()
         g = self.grid
         d.dtype = N.F64 if g.dtype == torch.float64 else N.F32
@@ -196,7 +197,7 @@ class FrequencyPlane:
         d.electric = sum(c.startswith('E') for c in self.components)
         d.corners = self.maps[0][0].shape[0]
         d.time_downsample = self.monitor.time_downsample
-        d.E, d.H = g.E.data_ptr(), g.H.data_ptr()
+        d.E, d.H = (g.E if E is None else E).data_ptr(), (g.H if H is None else H).data_ptr()
         for k, (indices, weights) in enumerate(self.maps):
             d.indices[k] = indices.data_ptr()
             d.weights[k] = weights.data_ptr()

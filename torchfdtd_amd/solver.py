@@ -22,7 +22,8 @@ import torch
 
 from . import _native as N
 from .graphs import StepGraphs, observation_schedule, validate_graph_steps
-from .grid import YeeGrid, configure_cuda_kernel, half_step_descriptor, resolve_cuda_kernel
+from .grid import (YeeGrid, configure_cuda_kernel, half_step_descriptor, march_descriptor, march_supported,
+                   resolve_cuda_kernel)
 from .models import Project
 from .monitors import FrequencyPlane, apodization_window, point_spectrum
 from .planning import field_axes, index_at, source_terms, voxelize
@@ -110,7 +111,7 @@ def _json_default(value):
 class Stepper:
     """Device state of one resident run and its step (the reference's closure, solver.py:776-797)."""
 
-    def __init__(self, project, steps=None, kernel=None):
+    def __init__(self, project, steps=None, kernel=None, one_pass=None, tile_y=4, chunk_x=64):
         p, r = project, project.region
         self.project = p
         self.steps = r.steps if steps is None else steps
@@ -136,27 +137,75 @@ class Stepper:
         self.planes = [FrequencyPlane(g, p.resolved_monitor(m)) for m in p.monitors if m.enabled and m.kind == 'field']
         if self.kernel == 'torch':
             configure_cuda_kernel(g, 'torch')
+        supported = self.kernel == 'hip' and march_supported(g, self.sources)
+        self.one_pass = supported if one_pass is None else bool(one_pass)
+        if self.one_pass and not supported:
+            raise ValueError('the one-pass kernel needs a 3D grid without periodic axes and at most 8 sources per family')
+        self.tile_y, self.chunk_x = tile_y, chunk_x
         self._describe()
+
+    def _observe(self, E, H):
+        g = self.grid
+        o = N.Observe()
+        o.dtype = N.F64 if g.dtype == torch.float64 else N.F32
+        o.count = len(self.point_monitors)
+        o.table = self.trace_table.data_ptr() if o.count else None
+        o.E, o.H = E.data_ptr(), H.data_ptr()
+        o.traces = self.traces.data_ptr() if o.count else None
+        o.steps = self.steps
+        o.counter = self.counter.data_ptr()
+        return o
+
+    def _planes(self, E, H):
+        table = (N.Plane*max(1, len(self.planes)))()
+        for k, plane in enumerate(self.planes):
+            table[k] = plane.descriptor(self.counter, E, H)
+        return table
 
     def _describe(self):
         g = self.grid
         dtype = N.F64 if g.dtype == torch.float64 else N.F32
         self.e_step = half_step_descriptor(g, False, self.sources['E'] if self.kernel == 'hip' else (), self.counter)
         self.h_step = half_step_descriptor(g, True, self.sources['H'] if self.kernel == 'hip' else (), self.counter)
-        self.plane_table = (N.Plane*max(1, len(self.planes)))()
-        for k, plane in enumerate(self.planes):
-            self.plane_table[k] = plane.descriptor(self.counter)
-        o = self.observe = N.Observe()
-        o.dtype = dtype
-        o.count = len(self.point_monitors)
-        o.table = self.trace_table.data_ptr() if o.count else None
-        o.E, o.H = g.E.data_ptr(), g.H.data_ptr()
-        o.traces = self.traces.data_ptr() if o.count else None
-        o.steps = self.steps
-        o.counter = self.counter.data_ptr()
+        self.plane_table = self._planes(g.E, g.H)
+        self.observe = self._observe(g.E, g.H)
+        if self.one_pass:
+            # Second buffer set of the one-pass step: time n is read from one set, n+1 written to the other.
+            self.E_b, self.H_b = torch.empty_like(g.E), torch.empty_like(g.H)
+            self.psi_e_b = torch.zeros_like(g.psi_buffers[False])
+            psi_a = g.psi_buffers[False]
+            self.march = [march_descriptor(g, g.E, g.H, self.E_b, self.H_b, psi_a, self.psi_e_b, self.sources, self.counter,
+                                           self.tile_y, self.chunk_x),
+                          march_descriptor(g, self.E_b, self.H_b, g.E, g.H, self.psi_e_b, psi_a, self.sources, self.counter,
+                                           self.tile_y, self.chunk_x)]
+            self.plane_table_b = self._planes(self.E_b, self.H_b)
+            self.observe_b = self._observe(self.E_b, self.H_b)
 
     def states(self):
-        return [self.grid.E, self.grid.H, self.counter, self.traces, *self.grid.memory_states]
+        extra = [self.E_b, self.H_b, self.psi_e_b] if self.one_pass else []
+        return [self.grid.E, self.grid.H, self.counter, self.traces, *self.grid.memory_states, *extra]
+
+    def step_pair(self):
+        """Two steps with the one-pass kernel: grid buffers -> second set -> grid buffers."""
+        stream = N.stream_handle(self.grid.device)
+        lib = self.lib
+        for march, planes, observe in ((self.march[0], self.plane_table_b, self.observe_b),
+                                       (self.march[1], self.plane_table, self.observe)):
+            N.check(lib.tfdtd_yee_step_march(ctypes.byref(march), stream), 'tfdtd_yee_step_march')
+            if self.planes:
+                N.check(lib.tfdtd_dft_update(planes, len(self.planes), stream), 'tfdtd_dft_update')
+            N.check(lib.tfdtd_observe_step(ctypes.byref(observe), stream), 'tfdtd_observe_step')
+
+    def advance(self, count):
+        """`count` steps, eagerly: pairs through the one-pass kernel where available, else single steps."""
+        if self.one_pass:
+            for _ in range(count // 2):
+                self.step_pair()
+            if count % 2:
+                self.step()
+        else:
+            for _ in range(count):
+                self.step()
 
     def step(self):
         stream = N.stream_handle(self.grid.device)
@@ -206,7 +255,8 @@ class Simulation:
         control = r.run_control
         diagnostics = StateDiagnostics(g) if control.auto_shutoff or control.divergence_check else None
         decision = DecayDecision(control, source_end_time(p), g.time_step)
-        graph = StepGraphs(run.step, run.states(), r.steps, cuda_graph_steps) if cuda_graph else None
+        graph = (StepGraphs(run.step, run.states(), r.steps, cuda_graph_steps, pair=run.step_pair if run.one_pass else None)
+                 if cuda_graph else None)
         axis = 'xyz'.index(r.slice_axis)
         plane_index = index_at(tuple(r.slice_position if i == axis else 0 for i in range(3)), r, r.field)[axis]
         component = 'xyz'.index(r.field[1].lower())
