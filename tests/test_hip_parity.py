@@ -68,13 +68,15 @@ def test_random_fields_and_every_cpml_state_bitwise(gpu, precision, layout):
 
 
 @pytest.mark.parametrize('precision', ['float32', 'float64'])
-@pytest.mark.parametrize('layout,tile_y,chunk_x', [('pml3d', 4, 7), ('pml3d', 8, 64), ('pec', 16, 5), ('pml3d', 4, 0)])
-def test_one_pass_step_matches_oracle_bitwise(gpu, precision, layout, tile_y, chunk_x):
-    """The one-pass E+H kernel (ping-pong buffers, halo recompute, x chunks) equals the reference update."""
+@pytest.mark.parametrize('layout,tile_y,chunk_x,nz', [('pml3d', 8, 7, 131), ('pml3d', 16, 64, 200), ('pec', 8, 5, 131),
+                                                      ('pml3d', 16, 0, 131), ('pml3d', 8, 9, 200), ('pec', 16, 6, 200)])
+def test_one_pass_step_matches_oracle_bitwise(gpu, precision, layout, tile_y, chunk_x, nz):
+    """The one-pass E+H kernel (ping-pong buffers, LDS tiles with halo recompute, x chunks, CPML-free
+    interior and CPML boundary launches) equals the reference update bit for bit."""
     from torchfdtd_amd import Project
     from torchfdtd_amd.solver import Stepper
     region = region_for(layout, precision)
-    region.size = (3.2, 7.4, 13.1)      # 32 x 74 x 131 cells: partial y tiles and three z windows
+    region.size = (3.2, 7.4, nz/10)     # 32 x 74 x nz cells: partial y and z tiles
     project = Project(region=region)
     run = Stepper(project, steps=40, one_pass=True, tile_y=tile_y, chunk_x=chunk_x)
     g = run.grid

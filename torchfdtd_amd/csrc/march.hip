@@ -75,7 +75,10 @@ struct March {
     MSource<real> sh[kMarchSources];
     long long source_samples;
     const long long* counter;
-    int chunk, zwin, ygroups;
+    int chunk, ytiles, ztiles;
+    int box_lo[2], box_hi[2];   // x chunks and y tiles [lo, hi) free of x and y CPML rows
+    int zbox_lo, zbox_hi;       // z tiles [lo, hi) free of z CPML rows
+    int nchunks;
 };
 
 struct Hit {           // slab membership of a coordinate: face -1 = none
@@ -102,6 +105,21 @@ __device__ __forceinline__ real cpml(real d, const Family<real>& F, const Hit& h
     const real m = F.psi_in[o] * b + c * d;
     if (write) F.psi_out[o] = m;
     return d * ik + m;
+}
+
+// Per-lane z-slab membership packed in one int: -1 none, else face << 20 | row.
+template <typename real>
+__device__ __forceinline__ int zcode(const Family<real>& F, int z) {
+    const Hit h = find_hit(F, 2, z);
+    return h.face < 0 ? -1 : (h.face << 20) | h.q;
+}
+
+template <typename real>
+__device__ __forceinline__ Hit zexpand(const Family<real>& F, int code) {
+    if (code < 0) return Hit{-1, 0, 1, 0u, 0u, 0u};
+    const int face = code >> 20, q = code & 0xfffff;
+    const Face& f = face ? F.face[2][1] : F.face[2][0];
+    return Hit{face, q, f.hi - f.lo, f.psi[0], f.psi[1], f.coef};
 }
 
 template <typename real>
@@ -172,7 +190,7 @@ __device__ __forceinline__ void store3(real* __restrict__ base, unsigned i, V3<r
 
 // E_new at (x, y, z) from backward differences of H (terms in CURL_TERMS order).
 // h: H(x,y,z); hxm: H(x-1,y,z); hym: H(x,y-1,z); hx_zm, hy_zm: Hx, Hy at (x,y,z-1).
-template <typename real>
+template <typename real, int CPML = 7>
 __device__ __forceinline__ V3<real> e_cell(const March<real>& a, int x, int y, int z, bool write, const Hit& hx, const Hit& hy,
                                            const Hit& hz, const ZCoef<real>& zc, V3<real> h, V3<real> hxm, V3<real> hym,
                                            real hx_zm, real hy_zm, V3<real> e, V3<real> inv) {
@@ -185,7 +203,7 @@ __device__ __forceinline__ V3<real> e_cell(const March<real>& a, int x, int y, i
         real d = real(0);
         if (x > 0) {
             d = h.z - hxm.z;
-            if (hx.face >= 0) d = cpml(d, a.ef, hx, 0, px, write);
+            if ((CPML & 1) && hx.face >= 0) d = cpml(d, a.ef, hx, 0, px, write);
         }
         cy = cy - d;
     }
@@ -193,7 +211,7 @@ __device__ __forceinline__ V3<real> e_cell(const March<real>& a, int x, int y, i
         real d = real(0);
         if (x > 0) {
             d = h.y - hxm.y;
-            if (hx.face >= 0) d = cpml(d, a.ef, hx, 1, px, write);
+            if ((CPML & 1) && hx.face >= 0) d = cpml(d, a.ef, hx, 1, px, write);
         }
         cz = cz + d;
     }
@@ -201,7 +219,7 @@ __device__ __forceinline__ V3<real> e_cell(const March<real>& a, int x, int y, i
         real d = real(0);
         if (y > 0) {
             d = h.z - hym.z;
-            if (hy.face >= 0) d = cpml(d, a.ef, hy, 0, py, write);
+            if ((CPML & 2) && hy.face >= 0) d = cpml(d, a.ef, hy, 0, py, write);
         }
         cx = cx + d;
     }
@@ -209,7 +227,7 @@ __device__ __forceinline__ V3<real> e_cell(const March<real>& a, int x, int y, i
         real d = real(0);
         if (y > 0) {
             d = h.x - hym.x;
-            if (hy.face >= 0) d = cpml(d, a.ef, hy, 1, py, write);
+            if ((CPML & 2) && hy.face >= 0) d = cpml(d, a.ef, hy, 1, py, write);
         }
         cz = cz - d;
     }
@@ -217,7 +235,7 @@ __device__ __forceinline__ V3<real> e_cell(const March<real>& a, int x, int y, i
         real d = real(0);
         if (z > 0) {
             d = h.y - hy_zm;
-            if (hz.face >= 0) d = cpml_lane(d, a.ef, hz, zc, 0, pz, write);
+            if ((CPML & 4) && hz.face >= 0) d = cpml_lane(d, a.ef, hz, zc, 0, pz, write);
         }
         cx = cx - d;
     }
@@ -225,7 +243,7 @@ __device__ __forceinline__ V3<real> e_cell(const March<real>& a, int x, int y, i
         real d = real(0);
         if (z > 0) {
             d = h.x - hx_zm;
-            if (hz.face >= 0) d = cpml_lane(d, a.ef, hz, zc, 1, pz, write);
+            if ((CPML & 4) && hz.face >= 0) d = cpml_lane(d, a.ef, hz, zc, 1, pz, write);
         }
         cy = cy + d;
     }
@@ -234,7 +252,7 @@ __device__ __forceinline__ V3<real> e_cell(const March<real>& a, int x, int y, i
 
 // H_new at (x, y, z) from forward differences of E_new (terms in CURL_TERMS order).
 // e: E(x,y,z); exn: E(x+1,y,z); eyn: E(x,y+1,z); ex_zp, ey_zp: Ex, Ey at (x,y,z+1).
-template <typename real>
+template <typename real, int CPML = 7>
 __device__ __forceinline__ V3<real> h_cell(const March<real>& a, int x, int y, int z, bool write, real coefficient, const Hit& hx,
                                            const Hit& hy, const Hit& hz, const ZCoef<real>& zc, V3<real> e, V3<real> exn,
                                            V3<real> eyn, real ex_zp, real ey_zp, V3<real> h) {
@@ -247,7 +265,7 @@ __device__ __forceinline__ V3<real> h_cell(const March<real>& a, int x, int y, i
         real d = real(0);
         if (x < nx - 1) {
             d = exn.z - e.z;
-            if (hx.face >= 0) d = cpml(d, a.hf, hx, 0, px, write);
+            if ((CPML & 1) && hx.face >= 0) d = cpml(d, a.hf, hx, 0, px, write);
         } else if (a.pec_mask & 1) {
             d = -a.pec_factor[0] * e.z;
         }
@@ -257,7 +275,7 @@ __device__ __forceinline__ V3<real> h_cell(const March<real>& a, int x, int y, i
         real d = real(0);
         if (x < nx - 1) {
             d = exn.y - e.y;
-            if (hx.face >= 0) d = cpml(d, a.hf, hx, 1, px, write);
+            if ((CPML & 1) && hx.face >= 0) d = cpml(d, a.hf, hx, 1, px, write);
         } else if (a.pec_mask & 1) {
             d = -a.pec_factor[0] * e.y;
         }
@@ -267,7 +285,7 @@ __device__ __forceinline__ V3<real> h_cell(const March<real>& a, int x, int y, i
         real d = real(0);
         if (y < ny - 1) {
             d = eyn.z - e.z;
-            if (hy.face >= 0) d = cpml(d, a.hf, hy, 0, py, write);
+            if ((CPML & 2) && hy.face >= 0) d = cpml(d, a.hf, hy, 0, py, write);
         } else if (a.pec_mask & 2) {
             d = -a.pec_factor[1] * e.z;
         }
@@ -277,7 +295,7 @@ __device__ __forceinline__ V3<real> h_cell(const March<real>& a, int x, int y, i
         real d = real(0);
         if (y < ny - 1) {
             d = eyn.x - e.x;
-            if (hy.face >= 0) d = cpml(d, a.hf, hy, 1, py, write);
+            if ((CPML & 2) && hy.face >= 0) d = cpml(d, a.hf, hy, 1, py, write);
         } else if (a.pec_mask & 2) {
             d = -a.pec_factor[1] * e.x;
         }
@@ -287,7 +305,7 @@ __device__ __forceinline__ V3<real> h_cell(const March<real>& a, int x, int y, i
         real d = real(0);
         if (z < nz - 1) {
             d = ey_zp - e.y;
-            if (hz.face >= 0) d = cpml_lane(d, a.hf, hz, zc, 0, pz, write);
+            if ((CPML & 4) && hz.face >= 0) d = cpml_lane(d, a.hf, hz, zc, 0, pz, write);
         } else if (a.pec_mask & 4) {
             d = -a.pec_factor[2] * e.y;
         }
@@ -297,7 +315,7 @@ __device__ __forceinline__ V3<real> h_cell(const March<real>& a, int x, int y, i
         real d = real(0);
         if (z < nz - 1) {
             d = ex_zp - e.x;
-            if (hz.face >= 0) d = cpml_lane(d, a.hf, hz, zc, 1, pz, write);
+            if ((CPML & 4) && hz.face >= 0) d = cpml_lane(d, a.hf, hz, zc, 1, pz, write);
         } else if (a.pec_mask & 4) {
             d = -a.pec_factor[2] * e.x;
         }
@@ -306,28 +324,71 @@ __device__ __forceinline__ V3<real> h_cell(const March<real>& a, int x, int y, i
     return V3<real>{h.x - coefficient * cx, h.y - coefficient * cy, h.z - coefficient * cz};
 }
 
-template <typename real, int TY, bool EPS3>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TY <= 4 ? 3 : TY <= 8 ? 2 : 1, 8))) void march_kernel(const March<real> a) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    // XCD-aware order: workgroup w runs on XCD w % 8; give each XCD a contiguous range of tiles.
-    const int G = gridDim.x, w = blockIdx.x, full = (G / 8) * 8;
-    int t = w < full ? (w % 8) * (G / 8) + (w / 8) : w;
-    const int group = t % a.ygroups;
-    t /= a.ygroups;
-    const int zw = t % a.zwin;
-    const int chunk = t / a.zwin;
+// One workgroup (4 wavefronts, 256 lanes) owns a TY x 64 tile of (y, z) cells, TY = 4*RW, and
+// marches over a chunk of x planes.  Wavefront w owns rows y0 + w*RW .. y0 + w*RW + RW-1; lane l
+// owns column z0 + l.  Per plane x:
+//   A. the workgroup stages H(x) on rows y0-1 .. y0+TY and columns z0-1 .. z0+64 in LDS (sH), and
+//      every lane loads E(x) and 1/eps of its cells into registers;
+//   B. barrier;
+//   C. E(x) of the owned cells, and of the halo row y0+TY (wavefront 0) and halo column z0+64
+//      (wavefront 1, TY lanes) that H(x-1) needs, from sH and the carried H(x-1);
+//   D. H(x-1) of the owned cells from E(x-1) in LDS (sE), E(x) in registers and the carried H(x-1);
+//   E. barrier;  F. E(x) into sE, H(x) becomes the carried H(x-1).
+// Cells outside the grid read clamped in-grid addresses and are never computed or stored.
+// CPML: bit 0 x, bit 1 y, bit 2 z slabs compiled in.  The lean variant (z only) runs the tiles
+// whose cells avoid every x and y slab; the full variant runs the rest.
+template <typename real, int RW, bool EPS3, int CPML>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ? 2 : CPML == 4 ? (RW <= 2 ? 4 : 3) : (RW <= 2 ? 6 : 4), 8))) void tile_kernel(const March<real> a) {
+    constexpr int TY = 4 * RW, TZ = 64;
+    constexpr int HR = TY + 2, HC = TZ + 2;    // sH: rows y0-1 .. y0+TY, columns z0-1 .. z0+TZ
+    constexpr int EC = TZ + 1;                 // sE: rows y0 .. y0+TY, columns z0 .. z0+TZ
+    constexpr int NC = RW + 1;                 // cell slots per lane: RW owned + one halo task
+    __shared__ real sH[HR * HC * 3];
+    __shared__ real sE[(TY + 1) * EC * 3];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    int ty, tz, chunk;
+    if (CPML != 7) {
+        // Lean tiles: the box of x chunks [lo0, hi0) and y tiles [lo1, hi1), every z tile.  Each XCD
+        // (workgroup w runs on XCD w % 8) takes a contiguous range, so neighbouring tiles share L2.
+        const int G = gridDim.x, w = blockIdx.x, full = (G / 8) * 8;
+        int t = w < full ? (w % 8) * (G / 8) + (w / 8) : w;
+        // CPML == 0: z tiles [zlo, zhi) free of z slabs; CPML == 4: the other z tiles.
+        const int my = a.box_hi[1] - a.box_lo[1];
+        const int zlo = a.zbox_lo, zhi = a.zbox_hi;
+        const int mz = CPML == 0 ? zhi - zlo : a.ztiles - (zhi - zlo);
+        ty = a.box_lo[1] + t % my;
+        t /= my;
+        const int k = t % mz;
+        tz = CPML == 0 ? zlo + k : (k < zlo ? k : zhi + (k - zlo));
+        chunk = a.box_lo[0] + t / mz;
+    } else {
+        // Full tiles: the complement of the box, enumerated as four blocks, round robin over XCDs.
+        int b = blockIdx.x;
+        const int per_chunk = a.ytiles * a.ztiles, inner = a.box_hi[0] - a.box_lo[0];
+        const int nA = a.box_lo[0] * per_chunk, nB = (a.nchunks - a.box_hi[0]) * per_chunk;
+        const int nC = inner * a.box_lo[1] * a.ztiles;
+        if (b < nA + nB) {
+            chunk = b < nA ? b / per_chunk : a.box_hi[0] + (b - nA) / per_chunk;
+            const int r = b < nA ? b % per_chunk : (b - nA) % per_chunk;
+            ty = r % a.ytiles;
+            tz = r / a.ytiles;
+        } else if ((b -= nA + nB) < nC) {
+            ty = b % a.box_lo[1];
+            b /= a.box_lo[1];
+            tz = b % a.ztiles;
+            chunk = a.box_lo[0] + b / a.ztiles;
+        } else {
+            b -= nC;
+            const int high = a.ytiles - a.box_hi[1];
+            ty = a.box_hi[1] + b % high;
+            b /= high;
+            tz = b % a.ztiles;
+            chunk = a.box_lo[0] + b / a.ztiles;
+        }
+    }
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int y0 = (group * 4 + wave) * TY;
-    if (y0 >= ny) return;
-    const int zr = zw * 62 - 1 + lane;
-    const bool zin = zr >= 0 && zr < nz;
-    const bool zown = lane >= 1 && lane <= 62 && zin;
-    // Lanes and rows outside the grid read (and compute on) clamped in-grid cells; their
-    // results are never stored and never reach an owned cell, so every load of a plane is
-    // unconditional and can be issued at once.
-    const int z = zin ? zr : 0;
-    const int xa = chunk * a.chunk;
-    const int xb = min(nx, xa + a.chunk);
+    const int y0 = ty * TY, z0 = tz * TZ;
+    const int xa = chunk * a.chunk, xb = min(nx, xa + a.chunk);
     const long long n = *a.counter;
     const real* __restrict__ e_in = a.e_in;
     const real* __restrict__ h_in = a.h_in;
@@ -336,85 +397,120 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TY <= 4 ? 3
     const real* __restrict__ inv_eps = a.inv_eps;
     const real hcoef = a.courant * a.inv_mu[0];
     const unsigned plane = (unsigned)ny * nz;
-    const Hit ez_hit = zin ? find_hit(a.ef, 2, z) : Hit{-1, 0, 1, 0u, 0u, 0u};
-    const Hit hz_hit = zin ? find_hit(a.hf, 2, z) : Hit{-1, 0, 1, 0u, 0u, 0u};
-    const ZCoef<real> ez_coef = lane_coefficients(a.ef, ez_hit);
-    const ZCoef<real> hz_coef = lane_coefficients(a.hf, hz_hit);
-    int row[TY + 2];   // clamped rows y0-1 .. y0+TY
-#pragma unroll
-    for (int j = 0; j < TY + 2; ++j) row[j] = min(max(y0 - 1 + j, 0), ny - 1);
 
-    V3<real> Ec[TY + 1];   // E_new(x-1), rows y0..y0+TY
-    V3<real> Hc[TY + 1];   // H_old(x-1), rows y0..y0+TY
+    // Slots: k < RW owned cells; k == RW the halo task of wavefronts 0 and 1.
+    int cy[NC], cz[NC];
+    bool active[NC];
 #pragma unroll
-    for (int j = 0; j <= TY; ++j) {
-        Ec[j] = zero3<real>();
-        Hc[j] = zero3<real>();
+    for (int k = 0; k < RW; ++k) {
+        cy[k] = y0 + wave * RW + k;
+        cz[k] = z0 + lane;
     }
-    if (xa > 0) {
-        const unsigned base = (unsigned)(xa - 1) * plane + (unsigned)z;
+    cy[RW] = wave == 0 ? y0 + TY : y0 + lane;
+    cz[RW] = wave == 0 ? z0 + lane : z0 + TZ;
 #pragma unroll
-        for (int j = 0; j <= TY; ++j) Hc[j] = load3(h_in, 3u * (base + (unsigned)row[j + 1] * nz));
+    for (int k = 0; k < NC; ++k) active[k] = cy[k] < ny && cz[k] < nz;
+    active[RW] = active[RW] && (wave == 0 || (wave == 1 && lane < TY));
+    const bool zown = z0 + lane < nz;
+    const int ez_own = zown ? zcode(a.ef, z0 + lane) : -1;
+    const int hz_own = zown ? zcode(a.hf, z0 + lane) : -1;
+    const ZCoef<real> ez_coef = lane_coefficients(a.ef, zexpand(a.ef, ez_own));
+    const ZCoef<real> hz_coef = lane_coefficients(a.hf, zexpand(a.hf, hz_own));
+    const int ez_task = active[RW] ? zcode(a.ef, cz[RW]) : -1;
+    const ZCoef<real> ez_task_coef = lane_coefficients(a.ef, zexpand(a.ef, ez_task));
+
+    // Element tid of an sH row: component tid % 3 of column z0 - 1 + tid / 3 (clamped into the grid).
+    const unsigned row_offset = 3u * (unsigned)min(max(z0 - 1 + tid / 3, 0), nz - 1) + (unsigned)(tid % 3);
+    V3<real> Hc[NC];   // H(x-1) of each slot
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+        Hc[k] = zero3<real>();
+        if (xa > 0) {
+            const unsigned c = (unsigned)(xa - 1) * plane + (unsigned)min(cy[k], ny - 1) * nz + (unsigned)min(cz[k], nz - 1);
+            Hc[k] = load3(h_in, 3u * c);
+        }
     }
     for (int x = xa; x <= xb; ++x) {
         const bool has_e = x < nx;
         const bool has_h = x > xa;
-        const int xl = has_e ? x : nx - 1;
-        const unsigned base = (unsigned)xl * plane + (unsigned)z;
-        const unsigned hbase = (unsigned)(x - 1) * plane + (unsigned)z;   // plane x-1 (used only when has_h)
-        const Hit ex_hit = find_hit(a.ef, 0, x);
-        const Hit hx_hit = find_hit(a.hf, 0, x - 1);
-        V3<real> hp[TY + 2], eo[TY + 1], inv[TY + 1];
+        V3<real> eo[NC], inv[NC];
+        if (has_e) {
+            const unsigned base = (unsigned)x * plane;
+            // Row r of sH is 3*HC contiguous reals in memory (clamped at the grid edges): lane t < 3*HC
+            // copies element t of every row.
+            if (tid < 3 * HC) {
 #pragma unroll
-        for (int j = 0; j < TY + 2; ++j) hp[j] = load3(h_in, 3u * (base + (unsigned)row[j] * nz));
+                for (int r = 0; r < HR; ++r) {
+                    const int y = min(max(y0 - 1 + r, 0), ny - 1);
+                    sH[r * 3 * HC + tid] = h_in[3u * (base + (unsigned)y * nz) + row_offset];
+                }
+            }
 #pragma unroll
-        for (int j = 0; j <= TY; ++j) {
-            const unsigned cell = base + (unsigned)row[j + 1] * nz;
-            eo[j] = load3(e_in, 3u * cell);
-            if (EPS3) {
-                inv[j] = load3(inv_eps, 3u * cell);
-            } else {
-                const real v = inv_eps[cell];
-                inv[j] = V3<real>{v, v, v};
+            for (int k = 0; k < NC; ++k) {
+                const unsigned cell = base + (unsigned)min(cy[k], ny - 1) * nz + (unsigned)min(cz[k], nz - 1);
+                eo[k] = load3(e_in, 3u * cell);
+                if (EPS3) {
+                    inv[k] = load3(inv_eps, 3u * cell);
+                } else {
+                    const real v = inv_eps[cell];
+                    inv[k] = V3<real>{v, v, v};
+                }
             }
         }
-        V3<real> prev_e = zero3<real>();       // E_new(x, y-1): the x+1 neighbour of the H row y-1
-        // Row j: E_new(x, y0+j); then H_new(x-1, y0+j-1), which needs E_new(x-1) of rows j-1 and j
-        // (carried) and E_new(x, y0+j-1) (previous row); then rotate row j-1 into the carried planes.
+        __syncthreads();
+        V3<real> en[NC], hn[NC];
+        const Hit ex_hit = find_hit(a.ef, 0, x);
+        const Hit hx_hit = find_hit(a.hf, 0, x - 1);
+        if (has_e) {
 #pragma unroll
-        for (int j = 0; j <= TY + 1; ++j) {
-            V3<real> en = zero3<real>();
-            if (j <= TY && has_e) {
-                const int y = y0 + j;
-                const V3<real> h = hp[j + 1];
-                const real hx_zm = shfl_up1(h.x);
-                const real hy_zm = shfl_up1(h.y);
-                const bool owned = zown && j < TY && y < ny && x < xb;
-                const Hit ey_hit = find_hit(a.ef, 1, y);
-                en = e_cell(a, x, row[j + 1], z, owned, ex_hit, ey_hit, ez_hit, ez_coef, h, Hc[j], hp[j], hx_zm, hy_zm,
-                            eo[j], inv[j]);
-                add_sources(a.se, a.nsrc_e, a.ebox_lo, a.ebox_hi, n, a.source_samples, x, y, zr, en.x, en.y, en.z);
-                if (owned) store3(e_out, 3u * (base + (unsigned)y * nz), en);
-            }
-            if (j >= 1) {
-                const int r = j - 1;
-                if (has_h && r < TY) {
-                    const int y = y0 + r;
-                    const real ex_zp = shfl_down1(Ec[r].x);
-                    const real ey_zp = shfl_down1(Ec[r].y);
-                    const bool owned = zown && y < ny;
-                    const Hit hy_hit = find_hit(a.hf, 1, y);
-                    V3<real> out = h_cell(a, x - 1, row[r + 1], z, owned, hcoef, hx_hit, hy_hit, hz_hit, hz_coef, Ec[r], prev_e,
-                                          Ec[r + 1], ex_zp, ey_zp, Hc[r]);
-                    add_sources(a.sh, a.nsrc_h, a.hbox_lo, a.hbox_hi, n, a.source_samples, x - 1, y, zr, out.x, out.y, out.z);
-                    if (owned) store3(h_out, 3u * (hbase + (unsigned)y * nz), out);
-                }
-                if (has_e) {
-                    Ec[r] = prev_e;
-                    Hc[r] = hp[r + 1];
+            for (int k = 0; k < NC; ++k) {
+                en[k] = zero3<real>();
+                const int r = cy[k] - y0 + 1, c = cz[k] - z0 + 1;
+                const real* p = sH + 3 * (r * HC + c);
+                hn[k] = V3<real>{p[0], p[1], p[2]};
+                if (active[k]) {
+                    const real* pym = p - 3 * HC;
+                    const real* pzm = p - 3;
+                    const bool owned = k < RW && x < xb;
+                    const Hit ey_hit = find_hit(a.ef, 1, cy[k]);
+                    en[k] = e_cell<real, CPML>(a, x, cy[k], cz[k], owned, ex_hit, ey_hit, zexpand(a.ef, k < RW ? ez_own : ez_task),
+                                   k < RW ? ez_coef : ez_task_coef, hn[k], Hc[k], V3<real>{pym[0], pym[1], pym[2]},
+                                   pzm[0], pzm[1], eo[k], inv[k]);
+                    add_sources(a.se, a.nsrc_e, a.ebox_lo, a.ebox_hi, n, a.source_samples, x, cy[k], cz[k],
+                                en[k].x, en[k].y, en[k].z);
+                    if (owned) store3(e_out, 3u * ((unsigned)x * plane + (unsigned)cy[k] * nz + (unsigned)cz[k]), en[k]);
                 }
             }
-            prev_e = en;
+        }
+        if (has_h) {
+            const int xh = x - 1;
+#pragma unroll
+            for (int k = 0; k < RW; ++k) {
+                if (active[k]) {
+                    const int r = cy[k] - y0, c = cz[k] - z0;
+                    const real* p = sE + 3 * (r * EC + c);
+                    const V3<real> e{p[0], p[1], p[2]};
+                    const real* pyn = p + 3 * EC;
+                    const real* pzn = p + 3;
+                    const Hit hy_hit = find_hit(a.hf, 1, cy[k]);
+                    V3<real> out = h_cell<real, CPML>(a, xh, cy[k], cz[k], true, hcoef, hx_hit, hy_hit, zexpand(a.hf, hz_own), hz_coef, e,
+                                          has_e ? en[k] : e, V3<real>{pyn[0], pyn[1], pyn[2]}, pzn[0], pzn[1], Hc[k]);
+                    add_sources(a.sh, a.nsrc_h, a.hbox_lo, a.hbox_hi, n, a.source_samples, xh, cy[k], cz[k],
+                                out.x, out.y, out.z);
+                    store3(h_out, 3u * ((unsigned)xh * plane + (unsigned)cy[k] * nz + (unsigned)cz[k]), out);
+                }
+            }
+        }
+        __syncthreads();
+        if (has_e) {
+#pragma unroll
+            for (int k = 0; k < NC; ++k) {
+                if (active[k]) {
+                    real* p = sE + 3 * ((cy[k] - y0) * EC + (cz[k] - z0));
+                    p[0] = en[k].x; p[1] = en[k].y; p[2] = en[k].z;
+                }
+                Hc[k] = hn[k];
+            }
         }
     }
 }
@@ -495,33 +591,80 @@ int make_march(const tfdtd_march* d, March<real>* a) {
     return TFDTD_OK;
 }
 
-template <typename real, int TY>
-int launch_march(const tfdtd_march* d, hipStream_t stream) {
+// Tiles [lo, hi) along one axis whose computed E and H cells avoid every CPML row of that axis.
+// E cells of tile k cover [k*size, k*size + size + e_extra), H cells [k*size, k*size + size).
+inline void free_range(int tiles, int size, int e_extra, int n, const Face* ef, const Face* hf, int* lo, int* hi) {
+    auto clear = [&](int k) {
+        const int a0 = k * size, e1 = std::min(n, a0 + size + e_extra), h1 = std::min(n, a0 + size);
+        for (int f = 0; f < 2; ++f) {
+            if (ef[f].hi > ef[f].lo && ef[f].lo < e1 && a0 < ef[f].hi) return false;
+            if (hf[f].hi > hf[f].lo && hf[f].lo < h1 && a0 < hf[f].hi) return false;
+        }
+        return true;
+    };
+    int best_lo = 0, best_hi = 0;
+    for (int k = 0; k < tiles;) {
+        if (!clear(k)) { ++k; continue; }
+        int j = k;
+        while (j < tiles && clear(j)) ++j;
+        if (j - k > best_hi - best_lo) { best_lo = k; best_hi = j; }
+        k = j;
+    }
+    *lo = best_lo;
+    *hi = best_hi;
+}
+
+template <typename real, int RW>
+int launch_tile(const tfdtd_march* d, hipStream_t stream) {
     March<real> a;
     int status = make_march(d, &a);
     if (status) return status;
+    constexpr int TY = 4 * RW;
     const int chunk = d->chunk_x > 0 ? d->chunk_x : d->nx;
     a.chunk = chunk;
-    a.zwin = (d->nz + 61) / 62;
-    const int ytiles = (d->ny + TY - 1) / TY;
-    a.ygroups = (ytiles + 3) / 4;
-    const long long chunks = (d->nx + chunk - 1) / chunk;
-    const long long blocks = chunks * a.zwin * a.ygroups;
+    a.ytiles = (d->ny + TY - 1) / TY;
+    a.ztiles = (d->nz + 63) / 64;
+    const int chunks = (d->nx + chunk - 1) / chunk;
+    const long long blocks = (long long)chunks * a.ytiles * a.ztiles;
     if (blocks > 0x7fffffffLL) return tfdtd_fail(TFDTD_EINVAL, "march: too many workgroups");
-    if (d->eps_stride == 3)
-        hipLaunchKernelGGL((march_kernel<real, TY, true>), dim3((unsigned)blocks), dim3(256), 0, stream, a);
-    else
-        hipLaunchKernelGGL((march_kernel<real, TY, false>), dim3((unsigned)blocks), dim3(256), 0, stream, a);
-    return tfdtd_check_launch("march");
+    // E planes of chunk c are xa .. xb (xb recomputed), so E extends one row/plane past each tile.
+    free_range(chunks, chunk, 1, d->nx, a.ef.face[0], a.hf.face[0], &a.box_lo[0], &a.box_hi[0]);
+    free_range(a.ytiles, TY, 1, d->ny, a.ef.face[1], a.hf.face[1], &a.box_lo[1], &a.box_hi[1]);
+    a.nchunks = chunks;
+    free_range(a.ztiles, 64, 1, d->nz, a.ef.face[2], a.hf.face[2], &a.zbox_lo, &a.zbox_hi);
+    const long long xy = (long long)(a.box_hi[0] - a.box_lo[0]) * (a.box_hi[1] - a.box_lo[1]);
+    const long long pure = xy * (a.zbox_hi - a.zbox_lo), inner = xy * a.ztiles;
+    auto lean = [&](auto kernel, long long count, const char* what) {
+        if (count <= 0 || status) return;
+        hipLaunchKernelGGL(kernel, dim3((unsigned)count), dim3(256), 0, stream, a);
+        status = tfdtd_check_launch(what);
+    };
+    if (d->eps_stride == 3) {
+        lean(tile_kernel<real, RW, true, 0>, pure, "march interior");
+        lean(tile_kernel<real, RW, true, 4>, inner - pure, "march z faces");
+    } else {
+        lean(tile_kernel<real, RW, false, 0>, pure, "march interior");
+        lean(tile_kernel<real, RW, false, 4>, inner - pure, "march z faces");
+    }
+    if (status) return status;
+    if (inner < blocks) {
+        if (inner == 0) { a.box_lo[0] = a.box_hi[0] = 0; a.box_lo[1] = a.box_hi[1] = 0; }
+        const unsigned rest = (unsigned)(blocks - inner);
+        if (d->eps_stride == 3)
+            hipLaunchKernelGGL((tile_kernel<real, RW, true, 7>), dim3(rest), dim3(256), 0, stream, a);
+        else
+            hipLaunchKernelGGL((tile_kernel<real, RW, false, 7>), dim3(rest), dim3(256), 0, stream, a);
+        status = tfdtd_check_launch("march boundary");
+    }
+    return status;
 }
 
 template <typename real>
 int dispatch_march(const tfdtd_march* d, hipStream_t stream) {
     switch (d->tile_y) {
-        case 4: return launch_march<real, 4>(d, stream);
-        case 8: return launch_march<real, 8>(d, stream);
-        case 16: return launch_march<real, 16>(d, stream);
-        default: return tfdtd_fail(TFDTD_EINVAL, "march: tile_y must be 4, 8 or 16");
+        case 8: return launch_tile<real, 2>(d, stream);
+        case 16: return launch_tile<real, 4>(d, stream);
+        default: return tfdtd_fail(TFDTD_EINVAL, "march: tile_y must be 8 or 16");
     }
 }
 
