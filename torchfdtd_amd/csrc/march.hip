@@ -78,6 +78,7 @@ struct March {
     int chunk, ytiles, ztiles;
     int box_lo[2], box_hi[2];   // x chunks and y tiles [lo, hi) free of x and y CPML rows
     int zbox_lo, zbox_hi;       // z tiles [lo, hi) free of z CPML rows
+    int zshift;                 // z tile tz starts at tz*63 - zshift
     int nchunks;
 };
 
@@ -324,25 +325,27 @@ __device__ __forceinline__ V3<real> h_cell(const March<real>& a, int x, int y, i
     return V3<real>{h.x - coefficient * cx, h.y - coefficient * cy, h.z - coefficient * cz};
 }
 
-// One workgroup (4 wavefronts, 256 lanes) owns a TY x 64 tile of (y, z) cells, TY = 4*RW, and
-// marches over a chunk of x planes.  Wavefront w owns rows y0 + w*RW .. y0 + w*RW + RW-1; lane l
-// owns column z0 + l.  Per plane x:
-//   A. the workgroup stages H(x) on rows y0-1 .. y0+TY and columns z0-1 .. z0+64 in LDS (sH), and
+// One workgroup (4 wavefronts, 256 lanes) marches a tile of TY x 63 (y, z) cells, TY = 4*RW, over
+// a chunk of x planes.  Wavefront w owns rows y0 + w*RW .. y0 + w*RW + RW-1 (a wave-uniform y); lane l
+// covers column z0 + l, lanes 0..62 own their cells and lane 63 recomputes E of the next tile's first
+// column.  Per plane x:
+//   A. the workgroup stages H(x) on rows y0-1 .. y0+TY and columns z0-1 .. z0+63 in LDS (sH), and
 //      every lane loads E(x) and 1/eps of its cells into registers;
 //   B. barrier;
-//   C. E(x) of the owned cells, and of the halo row y0+TY (wavefront 0) and halo column z0+64
-//      (wavefront 1, TY lanes) that H(x-1) needs, from sH and the carried H(x-1);
+//   C. E(x) of the tile's rows (and, in wavefront 0, of the halo row y0+TY) from sH and the carried
+//      H(x-1);
 //   D. H(x-1) of the owned cells from E(x-1) in LDS (sE), E(x) in registers and the carried H(x-1);
 //   E. barrier;  F. E(x) into sE, H(x) becomes the carried H(x-1).
-// Cells outside the grid read clamped in-grid addresses and are never computed or stored.
-// CPML: bit 0 x, bit 1 y, bit 2 z slabs compiled in.  The lean variant (z only) runs the tiles
-// whose cells avoid every x and y slab; the full variant runs the rest.
+// Cells outside the grid read clamped in-grid addresses and are never computed or stored.  z tiles
+// start at tz*63 - zshift, chosen so a tile edge meets the end of the low z slab.
+// CPML: bit 0 x, bit 1 y, bit 2 z slabs compiled in.  Variant 0 runs the tiles free of every slab,
+// variant 4 the tiles that meet only z slabs, variant 7 the rest.
 template <typename real, int RW, bool EPS3, int CPML>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ? 2 : CPML == 4 ? (RW <= 2 ? 4 : 3) : (RW <= 2 ? 6 : 4), 8))) void tile_kernel(const March<real> a) {
-    constexpr int TY = 4 * RW, TZ = 64;
-    constexpr int HR = TY + 2, HC = TZ + 2;    // sH: rows y0-1 .. y0+TY, columns z0-1 .. z0+TZ
-    constexpr int EC = TZ + 1;                 // sE: rows y0 .. y0+TY, columns z0 .. z0+TZ
-    constexpr int NC = RW + 1;                 // cell slots per lane: RW owned + one halo task
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ? (RW <= 2 ? 3 : 2) : CPML == 4 ? (RW <= 2 ? 4 : 3) : (RW <= 2 ? 6 : 4), 8))) void tile_kernel(const March<real> a) {
+    constexpr int TY = 4 * RW, TZ = 63;
+    constexpr int HR = TY + 2, HC = TZ + 2;    // sH: rows y0-1 .. y0+TY, columns z0-1 .. z0+63
+    constexpr int EC = TZ + 1;                 // sE: rows y0 .. y0+TY, columns z0 .. z0+63
+    constexpr int NC = RW + 1;                 // row slots per lane: RW owned + the halo row (wavefront 0)
     __shared__ real sH[HR * HC * 3];
     __shared__ real sE[(TY + 1) * EC * 3];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -387,7 +390,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ?
         }
     }
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int y0 = ty * TY, z0 = tz * TZ;
+    const int y0 = ty * TY, z0 = tz * TZ - a.zshift;
     const int xa = chunk * a.chunk, xb = min(nx, xa + a.chunk);
     const long long n = *a.counter;
     const real* __restrict__ e_in = a.e_in;
@@ -398,37 +401,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ?
     const real hcoef = a.courant * a.inv_mu[0];
     const unsigned plane = (unsigned)ny * nz;
 
-    // Slots: k < RW owned cells; k == RW the halo task of wavefronts 0 and 1.
-    int cy[NC], cz[NC];
-    bool active[NC];
-#pragma unroll
-    for (int k = 0; k < RW; ++k) {
-        cy[k] = y0 + wave * RW + k;
-        cz[k] = z0 + lane;
-    }
-    cy[RW] = wave == 0 ? y0 + TY : y0 + lane;
-    cz[RW] = wave == 0 ? z0 + lane : z0 + TZ;
-#pragma unroll
-    for (int k = 0; k < NC; ++k) active[k] = cy[k] < ny && cz[k] < nz;
-    active[RW] = active[RW] && (wave == 0 || (wave == 1 && lane < TY));
-    const bool zown = z0 + lane < nz;
-    const int ez_own = zown ? zcode(a.ef, z0 + lane) : -1;
-    const int hz_own = zown ? zcode(a.hf, z0 + lane) : -1;
-    const ZCoef<real> ez_coef = lane_coefficients(a.ef, zexpand(a.ef, ez_own));
-    const ZCoef<real> hz_coef = lane_coefficients(a.hf, zexpand(a.hf, hz_own));
-    const int ez_task = active[RW] ? zcode(a.ef, cz[RW]) : -1;
-    const ZCoef<real> ez_task_coef = lane_coefficients(a.ef, zexpand(a.ef, ez_task));
-
+    const int z = z0 + lane;
+    const bool zin = z >= 0 && z < nz;
+    const bool zown = zin && lane < TZ;
+    const unsigned zc = (unsigned)min(max(z, 0), nz - 1);
+    // Row slots: k < RW the wavefront's rows; k == RW the halo row y0+TY, computed by wavefront 0 only.
+    auto row = [&](int k) { return k < RW ? y0 + wave * RW + k : y0 + TY; };
+    auto computed = [&](int k) { return zin && row(k) < ny && (k < RW || wave == 0); };
+    const int ez_code = zin ? zcode(a.ef, z) : -1;
+    const int hz_code = zin ? zcode(a.hf, z) : -1;
+    const ZCoef<real> ez_coef = lane_coefficients(a.ef, zexpand(a.ef, ez_code));
+    const ZCoef<real> hz_coef = lane_coefficients(a.hf, zexpand(a.hf, hz_code));
     // Element tid of an sH row: component tid % 3 of column z0 - 1 + tid / 3 (clamped into the grid).
     const unsigned row_offset = 3u * (unsigned)min(max(z0 - 1 + tid / 3, 0), nz - 1) + (unsigned)(tid % 3);
+
     V3<real> Hc[NC];   // H(x-1) of each slot
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
         Hc[k] = zero3<real>();
-        if (xa > 0) {
-            const unsigned c = (unsigned)(xa - 1) * plane + (unsigned)min(cy[k], ny - 1) * nz + (unsigned)min(cz[k], nz - 1);
-            Hc[k] = load3(h_in, 3u * c);
-        }
+        if (xa > 0) Hc[k] = load3(h_in, 3u * ((unsigned)(xa - 1) * plane + (unsigned)min(row(k), ny - 1) * nz + zc));
     }
     for (int x = xa; x <= xb; ++x) {
         const bool has_e = x < nx;
@@ -447,7 +438,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ?
             }
 #pragma unroll
             for (int k = 0; k < NC; ++k) {
-                const unsigned cell = base + (unsigned)min(cy[k], ny - 1) * nz + (unsigned)min(cz[k], nz - 1);
+                const unsigned cell = base + (unsigned)min(row(k), ny - 1) * nz + zc;
                 eo[k] = load3(e_in, 3u * cell);
                 if (EPS3) {
                     inv[k] = load3(inv_eps, 3u * cell);
@@ -465,20 +456,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ?
 #pragma unroll
             for (int k = 0; k < NC; ++k) {
                 en[k] = zero3<real>();
-                const int r = cy[k] - y0 + 1, c = cz[k] - z0 + 1;
-                const real* p = sH + 3 * (r * HC + c);
+                const int y = row(k);
+                const int r = y - y0 + 1;
+                const real* p = sH + 3 * (r * HC + lane + 1);
                 hn[k] = V3<real>{p[0], p[1], p[2]};
-                if (active[k]) {
+                if (computed(k)) {
                     const real* pym = p - 3 * HC;
                     const real* pzm = p - 3;
-                    const bool owned = k < RW && x < xb;
-                    const Hit ey_hit = find_hit(a.ef, 1, cy[k]);
-                    en[k] = e_cell<real, CPML>(a, x, cy[k], cz[k], owned, ex_hit, ey_hit, zexpand(a.ef, k < RW ? ez_own : ez_task),
-                                   k < RW ? ez_coef : ez_task_coef, hn[k], Hc[k], V3<real>{pym[0], pym[1], pym[2]},
-                                   pzm[0], pzm[1], eo[k], inv[k]);
-                    add_sources(a.se, a.nsrc_e, a.ebox_lo, a.ebox_hi, n, a.source_samples, x, cy[k], cz[k],
-                                en[k].x, en[k].y, en[k].z);
-                    if (owned) store3(e_out, 3u * ((unsigned)x * plane + (unsigned)cy[k] * nz + (unsigned)cz[k]), en[k]);
+                    const bool owned = zown && k < RW && x < xb;
+                    const Hit ey_hit = find_hit(a.ef, 1, y);
+                    en[k] = e_cell<real, CPML>(a, x, y, z, owned, ex_hit, ey_hit, zexpand(a.ef, ez_code), ez_coef, hn[k], Hc[k],
+                                               V3<real>{pym[0], pym[1], pym[2]}, pzm[0], pzm[1], eo[k], inv[k]);
+                    add_sources(a.se, a.nsrc_e, a.ebox_lo, a.ebox_hi, n, a.source_samples, x, y, z, en[k].x, en[k].y, en[k].z);
+                    if (owned) store3(e_out, 3u * ((unsigned)x * plane + (unsigned)y * nz + (unsigned)z), en[k]);
                 }
             }
         }
@@ -486,18 +476,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ?
             const int xh = x - 1;
 #pragma unroll
             for (int k = 0; k < RW; ++k) {
-                if (active[k]) {
-                    const int r = cy[k] - y0, c = cz[k] - z0;
-                    const real* p = sE + 3 * (r * EC + c);
+                const int y = row(k);
+                if (zown && y < ny) {
+                    const real* p = sE + 3 * ((y - y0) * EC + lane);
                     const V3<real> e{p[0], p[1], p[2]};
                     const real* pyn = p + 3 * EC;
                     const real* pzn = p + 3;
-                    const Hit hy_hit = find_hit(a.hf, 1, cy[k]);
-                    V3<real> out = h_cell<real, CPML>(a, xh, cy[k], cz[k], true, hcoef, hx_hit, hy_hit, zexpand(a.hf, hz_own), hz_coef, e,
-                                          has_e ? en[k] : e, V3<real>{pyn[0], pyn[1], pyn[2]}, pzn[0], pzn[1], Hc[k]);
-                    add_sources(a.sh, a.nsrc_h, a.hbox_lo, a.hbox_hi, n, a.source_samples, xh, cy[k], cz[k],
-                                out.x, out.y, out.z);
-                    store3(h_out, 3u * ((unsigned)xh * plane + (unsigned)cy[k] * nz + (unsigned)cz[k]), out);
+                    const Hit hy_hit = find_hit(a.hf, 1, y);
+                    V3<real> out = h_cell<real, CPML>(a, xh, y, z, true, hcoef, hx_hit, hy_hit, zexpand(a.hf, hz_code), hz_coef, e,
+                                                      has_e ? en[k] : e, V3<real>{pyn[0], pyn[1], pyn[2]}, pzn[0], pzn[1], Hc[k]);
+                    add_sources(a.sh, a.nsrc_h, a.hbox_lo, a.hbox_hi, n, a.source_samples, xh, y, z, out.x, out.y, out.z);
+                    store3(h_out, 3u * ((unsigned)xh * plane + (unsigned)y * nz + (unsigned)z), out);
                 }
             }
         }
@@ -505,8 +494,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ?
         if (has_e) {
 #pragma unroll
             for (int k = 0; k < NC; ++k) {
-                if (active[k]) {
-                    real* p = sE + 3 * ((cy[k] - y0) * EC + (cz[k] - z0));
+                if (computed(k)) {
+                    real* p = sE + 3 * ((row(k) - y0) * EC + lane);
                     p[0] = en[k].x; p[1] = en[k].y; p[2] = en[k].z;
                 }
                 Hc[k] = hn[k];
@@ -593,9 +582,10 @@ int make_march(const tfdtd_march* d, March<real>* a) {
 
 // Tiles [lo, hi) along one axis whose computed E and H cells avoid every CPML row of that axis.
 // E cells of tile k cover [k*size, k*size + size + e_extra), H cells [k*size, k*size + size).
-inline void free_range(int tiles, int size, int e_extra, int n, const Face* ef, const Face* hf, int* lo, int* hi) {
+inline void free_range(int tiles, int size, int e_extra, int n, const Face* ef, const Face* hf, int* lo, int* hi, int shift = 0) {
     auto clear = [&](int k) {
-        const int a0 = k * size, e1 = std::min(n, a0 + size + e_extra), h1 = std::min(n, a0 + size);
+        const int a0 = std::max(0, k * size - shift), e1 = std::min(n, k * size - shift + size + e_extra);
+        const int h1 = std::min(n, k * size - shift + size);
         for (int f = 0; f < 2; ++f) {
             if (ef[f].hi > ef[f].lo && ef[f].lo < e1 && a0 < ef[f].hi) return false;
             if (hf[f].hi > hf[f].lo && hf[f].lo < h1 && a0 < hf[f].hi) return false;
@@ -623,7 +613,10 @@ int launch_tile(const tfdtd_march* d, hipStream_t stream) {
     const int chunk = d->chunk_x > 0 ? d->chunk_x : d->nx;
     a.chunk = chunk;
     a.ytiles = (d->ny + TY - 1) / TY;
-    a.ztiles = (d->nz + 63) / 64;
+    // Shift the z tiles so one tile edge meets the end of the low z slabs (E and H).
+    const int zend = std::max(a.ef.face[2][0].hi, a.hf.face[2][0].hi);
+    a.zshift = zend > 0 ? (63 - zend % 63) % 63 : 0;
+    a.ztiles = (d->nz + a.zshift + 62) / 63;
     const int chunks = (d->nx + chunk - 1) / chunk;
     const long long blocks = (long long)chunks * a.ytiles * a.ztiles;
     if (blocks > 0x7fffffffLL) return tfdtd_fail(TFDTD_EINVAL, "march: too many workgroups");
@@ -631,7 +624,7 @@ int launch_tile(const tfdtd_march* d, hipStream_t stream) {
     free_range(chunks, chunk, 1, d->nx, a.ef.face[0], a.hf.face[0], &a.box_lo[0], &a.box_hi[0]);
     free_range(a.ytiles, TY, 1, d->ny, a.ef.face[1], a.hf.face[1], &a.box_lo[1], &a.box_hi[1]);
     a.nchunks = chunks;
-    free_range(a.ztiles, 64, 1, d->nz, a.ef.face[2], a.hf.face[2], &a.zbox_lo, &a.zbox_hi);
+    free_range(a.ztiles, 63, 1, d->nz, a.ef.face[2], a.hf.face[2], &a.zbox_lo, &a.zbox_hi, a.zshift);
     const long long xy = (long long)(a.box_hi[0] - a.box_lo[0]) * (a.box_hi[1] - a.box_lo[1]);
     const long long pure = xy * (a.zbox_hi - a.zbox_lo), inner = xy * a.ztiles;
     auto lean = [&](auto kernel, long long count, const char* what) {

@@ -184,7 +184,7 @@ def march_family(grid, forward, psi_in, psi_out):
     return f
 
 
-def march_descriptor(grid, e_in, h_in, e_out, h_out, psi_e_in, psi_e_out, sources, counter, tile_y=8, chunk_x=32):
+def march_descriptor(grid, e_in, h_in, e_out, h_out, psi_e_in, psi_e_out, sources, counter, tile_y=8, chunk_x=8):
     """tfdtd_march: one whole step reading (e_in, h_in, psi_e_in) and writing (e_out, h_out, psi_e_out)."""
     d = N.March()
     d.dtype = N.F64 if grid.dtype == torch.float64 else N.F32

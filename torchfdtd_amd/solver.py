@@ -111,7 +111,7 @@ def _json_default(value):
 class Stepper:
     """Device state of one resident run and its step (the reference's closure, solver.py:776-797)."""
 
-    def __init__(self, project, steps=None, kernel=None, one_pass=None, tile_y=8, chunk_x=32):
+    def __init__(self, project, steps=None, kernel=None, one_pass=None, tile_y=8, chunk_x=8):
         p, r = project, project.region
         self.project = p
         self.steps = r.steps if steps is None else steps
