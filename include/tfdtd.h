@@ -75,9 +75,8 @@ typedef struct tfdtd_half_step {
 typedef struct tfdtd_observe {
     int32_t dtype;
     int32_t count;                  /* number of traces */
-    const int64_t* table;           /* device int64[count][2]: family, flat index ((x*ny+y)*nz+z)*3+c */
-    const void* E;
-    const void* H;
+    const int64_t* table;           /* device int64[count][2]: field base address, element index
+                                       ((x*ny+y)*nz+z)*3+c; traces of many grids may share one table */
     void* traces;                   /* real[steps][count] */
     int64_t steps;                  /* rows of traces; records stop beyond */
     int64_t* counter;               /* incremented by one after recording */
@@ -173,6 +172,63 @@ int tfdtd_observe_step(const tfdtd_observe* observe, void* stream);
 int tfdtd_step_fused(const tfdtd_half_step* e_step, const tfdtd_half_step* h_step,
                      const tfdtd_plane* planes, int32_t plane_count,
                      const tfdtd_observe* observe, int32_t n_steps, void* stream);
+
+/* Uneven batches: many grids of any shape advance together, one launch per family.
+ * torchfdtd/cuda_batch.py:19-71 (FusedBatchYeeCUDA, identical-topology cohorts) generalised to
+ * grids of different shapes, CPML layouts and sources.  tfdtd_batch_pack validates `count`
+ * half-step descriptors of one family and dtype, writes the engine's internal table into
+ * host_table (tfdtd_batch_table_bytes bytes) and block_prefix[count+1]; the caller copies the table
+ * and the prefix to device memory once, then launches tfdtd_batch_half_step every step with
+ * total_blocks = block_prefix[count]. */
+size_t tfdtd_batch_table_bytes(int32_t count, int32_t dtype);
+int tfdtd_batch_pack(const tfdtd_half_step* steps, int32_t count, void* host_table, size_t bytes, int32_t* block_prefix);
+int tfdtd_batch_half_step(const void* device_table, const int32_t* device_prefix, int32_t count, int32_t total_blocks,
+                          int32_t dtype, int32_t family, void* stream);
+
+/* The one-pass step of many grids (tfdtd_yee_step_march descriptors of one dtype, tile_y and
+ * eps_stride): three launches per step -- interior, z-face and boundary tiles -- each covering
+ * every grid of the batch.  tfdtd_batch_march_pack writes the internal table (tfdtd_batch_march_bytes
+ * bytes) and block_prefix int32[3][count+1]; the caller copies both to device memory once and passes
+ * totals[v] = block_prefix[v][count] (host memory) to every tfdtd_batch_march call.  A batch of
+ * one grid is also valid and runs the single-grid step from a device-resident descriptor. */
+size_t tfdtd_batch_march_bytes(int32_t count, int32_t dtype);
+int tfdtd_batch_march_pack(const tfdtd_march* steps, int32_t count, void* host_table, size_t bytes, int32_t* block_prefix);
+int tfdtd_batch_march(const void* device_table, const int32_t* device_prefix, int32_t count, int32_t dtype,
+                      int32_t tile_y, int32_t eps_stride, const int32_t* totals, void* stream);
+
+/* The same for frequency planes of many grids (torchfdtd/cuda_batch.py:74-169, FusedBatchIO, and
+ * torchfdtd/cuda_monitors.py:14-155): one launch for every plane of the batch. */
+size_t tfdtd_batch_planes_bytes(int32_t count, int32_t dtype, int32_t spectral);
+int tfdtd_batch_planes_pack(const tfdtd_plane* planes, int32_t count, void* host_table, size_t bytes,
+                            int32_t* max_blocks, int32_t* max_frequencies);
+int tfdtd_batch_dft_update(const void* device_table, int32_t count, int32_t max_blocks, int32_t max_frequencies,
+                           int32_t dtype, int32_t spectral, void* stream);
+
+/* State norms of the run-control diagnostics (torchfdtd/run_control.py:127-159 and the fused
+ * reduction of torchfdtd/cuda_diagnostics.py:63-135), per grid in FP64:
+ *   out[g] = { sum_c vol E_c^2 / inv_eps_c + vol H_c^2,  max |E|,|H|,  1 if any value of E, H
+ *             or the CPML memory is not finite else 0 },
+ * vol = widths[0][x] * widths[1][y] * widths[2][z] (cell widths over the reference step; all
+ * three null: 1).  tfdtd_norm_pack writes the table (tfdtd_norm_table_bytes bytes) for the caller
+ * to copy to device memory once; tfdtd_state_norm then needs partial double[tasks][3] and
+ * out double[count][3] in device memory, tasks = tfdtd_norm_tasks. */
+typedef struct tfdtd_norm_grid {
+    int32_t dtype;
+    int32_t nx, ny, nz;
+    const void* E;
+    const void* H;
+    const void* inverse_permittivity;
+    int32_t eps_stride;             /* 1 per cell, 3 per cell and component */
+    const double* widths[3];
+    const void* psi[2];             /* CPML memory buffers (null: none) */
+    int64_t psi_count[2];           /* elements of each */
+} tfdtd_norm_grid;
+
+int64_t tfdtd_norm_tasks(const tfdtd_norm_grid* grids, int32_t count);
+size_t tfdtd_norm_table_bytes(const tfdtd_norm_grid* grids, int32_t count);
+int tfdtd_norm_pack(const tfdtd_norm_grid* grids, int32_t count, void* host_table, size_t bytes);
+int tfdtd_state_norm(const void* device_table, int32_t count, int64_t tasks, int32_t dtype, double* partial, double* out,
+                     void* stream);
 
 /* Copies the calling thread's last error message; returns its length. */
 size_t tfdtd_last_error(char* buffer, size_t size);

@@ -61,6 +61,8 @@ int main(void) {
   F(tfdtd_march, courant); F(tfdtd_march, e); F(tfdtd_march, h); F(tfdtd_march, nsrc_e); F(tfdtd_march, srcs_h);
   F(tfdtd_march, source_samples); F(tfdtd_march, counter); F(tfdtd_march, chunk_x);
   F(tfdtd_march_family, face); F(tfdtd_march_face, psi); F(tfdtd_march_face, coef);
+  printf("tfdtd_norm_grid %zu\n", sizeof(tfdtd_norm_grid));
+  F(tfdtd_norm_grid, eps_stride); F(tfdtd_norm_grid, widths); F(tfdtd_norm_grid, psi); F(tfdtd_norm_grid, psi_count);
   return 0;
 }
 '''
@@ -74,10 +76,40 @@ def test_ctypes_mirror_matches_the_c_layout(tmp_path):
     lines = dict(line.rsplit(' ', 1) for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split('\n') if line)
     mirror = {'tfdtd_half_step': N.HalfStep, 'tfdtd_plane': N.Plane, 'tfdtd_observe': N.Observe,
               'tfdtd_cpml_segment': N.CpmlSegment, 'tfdtd_source': N.SourceBox, 'tfdtd_march': N.March,
-              'tfdtd_march_face': N.MarchFace, 'tfdtd_march_family': N.MarchFamily}
+              'tfdtd_march_face': N.MarchFace, 'tfdtd_march_family': N.MarchFamily, 'tfdtd_norm_grid': N.NormGrid}
     for key, value in lines.items():
         if '.' in key:
             struct, member = key.split('.')
             assert getattr(mirror[struct], member).offset == int(value), key
         else:
             assert ctypes.sizeof(mirror[key]) == int(value), key
+
+
+def test_batch_and_norm_packing_validate_on_the_host():
+    lib = N.load_library()
+    steps = (N.HalfStep*2)()
+    for k, family in enumerate((0, 1)):
+        d = steps[k]
+        d.dtype, d.family, d.nx, d.ny, d.nz = 0, family, 4, 4, 4
+        d.src = d.dst = d.inverse = 16
+    nbytes = lib.tfdtd_batch_table_bytes(2, 0)
+    host = (ctypes.c_ubyte*nbytes)()
+    prefix = (ctypes.c_int32*3)()
+    assert lib.tfdtd_batch_pack(steps, 2, host, nbytes, prefix) == 1
+    assert 'differs in family' in N.last_error()
+    steps[1].family = 0
+    assert lib.tfdtd_batch_pack(steps, 2, host, nbytes-1, prefix) == 1
+    assert lib.tfdtd_batch_pack(steps, 2, host, nbytes, prefix) == 0
+    assert list(prefix) == [0, 1, 2]    # 64 cells: one 256-thread block per grid
+    grids = (N.NormGrid*1)()
+    grids[0].dtype, grids[0].nx, grids[0].ny, grids[0].nz, grids[0].eps_stride = 0, 0, 4, 4, 1
+    assert lib.tfdtd_norm_pack(grids, 1, host, nbytes) == 1
+    grids[0].nx, grids[0].E, grids[0].H, grids[0].inverse_permittivity = 4, 16, 16, 16
+    grids[0].widths[0] = 16
+    nbytes = lib.tfdtd_norm_table_bytes(grids, 1)
+    host = (ctypes.c_ubyte*nbytes)()
+    assert lib.tfdtd_norm_pack(grids, 1, host, nbytes) == 1
+    assert 'width' in N.last_error()
+    grids[0].widths[0] = None
+    assert lib.tfdtd_norm_pack(grids, 1, host, nbytes) == 0
+    assert lib.tfdtd_norm_tasks(grids, 1) == 2    # E and H, 192 values each

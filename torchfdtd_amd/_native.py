@@ -14,7 +14,7 @@ from pathlib import Path
 PACKAGE = Path(__file__).resolve().parent
 ROOT = PACKAGE.parent
 LIBRARY = PACKAGE/'libtfdtd.so'
-SOURCES = [PACKAGE/'csrc'/name for name in ('yee.hip', 'march.hip', 'batch.hip', 'adjoint.hip', 'conv.hip')]
+SOURCES = [PACKAGE/'csrc'/name for name in ('yee.hip', 'march.hip', 'norm.hip', 'adjoint.hip', 'conv.hip')]
 HEADER = ROOT/'include'/'tfdtd.h'
 
 HIPCC_FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-shared', '-std=c++17',
@@ -68,8 +68,7 @@ class HalfStep(ctypes.Structure):
 
 
 class Observe(ctypes.Structure):
-    _fields_ = [('dtype', i32), ('count', i32), ('table', vp), ('E', vp), ('H', vp),
-                ('traces', vp), ('steps', i64), ('counter', vp)]
+    _fields_ = [('dtype', i32), ('count', i32), ('table', vp), ('traces', vp), ('steps', i64), ('counter', vp)]
 
 
 class Plane(ctypes.Structure):
@@ -99,10 +98,28 @@ class March(ctypes.Structure):
                 ('source_samples', i64), ('counter', vp), ('tile_y', i32), ('chunk_x', i32)]
 
 
+class NormGrid(ctypes.Structure):
+    _fields_ = [('dtype', i32), ('nx', i32), ('ny', i32), ('nz', i32), ('E', vp), ('H', vp),
+                ('inverse_permittivity', vp), ('eps_stride', i32), ('widths', vp*3), ('psi', vp*2), ('psi_count', i64*2)]
+
+
 _lib = None
 
 PROTOTYPES = {
     'tfdtd_yee_step_march': (i32, [ctypes.POINTER(March), vp]),
+    'tfdtd_batch_table_bytes': (ctypes.c_size_t, [i32, i32]),
+    'tfdtd_batch_pack': (i32, [ctypes.POINTER(HalfStep), i32, vp, ctypes.c_size_t, ctypes.POINTER(i32)]),
+    'tfdtd_batch_half_step': (i32, [vp, vp, i32, i32, i32, i32, vp]),
+    'tfdtd_batch_march_bytes': (ctypes.c_size_t, [i32, i32]),
+    'tfdtd_batch_march_pack': (i32, [ctypes.POINTER(March), i32, vp, ctypes.c_size_t, ctypes.POINTER(i32)]),
+    'tfdtd_batch_march': (i32, [vp, vp, i32, i32, i32, i32, ctypes.POINTER(i32), vp]),
+    'tfdtd_batch_planes_bytes': (ctypes.c_size_t, [i32, i32, i32]),
+    'tfdtd_batch_planes_pack': (i32, [ctypes.POINTER(Plane), i32, vp, ctypes.c_size_t, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
+    'tfdtd_batch_dft_update': (i32, [vp, i32, i32, i32, i32, i32, vp]),
+    'tfdtd_norm_tasks': (i64, [ctypes.POINTER(NormGrid), i32]),
+    'tfdtd_norm_table_bytes': (ctypes.c_size_t, [ctypes.POINTER(NormGrid), i32]),
+    'tfdtd_norm_pack': (i32, [ctypes.POINTER(NormGrid), i32, vp, ctypes.c_size_t]),
+    'tfdtd_state_norm': (i32, [vp, i32, i64, i32, vp, vp, vp]),
     'tfdtd_abi_version': (i32, []),
     'tfdtd_last_error': (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
     'tfdtd_yee_half_step': (i32, [ctypes.POINTER(HalfStep), vp]),

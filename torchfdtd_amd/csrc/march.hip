@@ -80,6 +80,7 @@ struct March {
     int zbox_lo, zbox_hi;       // z tiles [lo, hi) free of z CPML rows
     int zshift;                 // z tile tz starts at tz*63 - zshift
     int nchunks;
+    int blocks[3];              // workgroups of the interior, z-face and boundary variants
 };
 
 struct Hit {           // slab membership of a coordinate: face -1 = none
@@ -340,8 +341,9 @@ __device__ __forceinline__ V3<real> h_cell(const March<real>& a, int x, int y, i
 // start at tz*63 - zshift, chosen so a tile edge meets the end of the low z slab.
 // CPML: bit 0 x, bit 1 y, bit 2 z slabs compiled in.  Variant 0 runs the tiles free of every slab,
 // variant 4 the tiles that meet only z slabs, variant 7 the rest.
+// w is the workgroup's index among the G workgroups of its variant and grid.
 template <typename real, int RW, bool EPS3, int CPML>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ? (RW <= 2 ? 3 : 2) : CPML == 4 ? (RW <= 2 ? 4 : 3) : (RW <= 2 ? 6 : 4), 8))) void tile_kernel(const March<real> a) {
+__device__ __forceinline__ void tile_body(const March<real>& a, const int w, const int G) {
     constexpr int TY = 4 * RW, TZ = 63;
     constexpr int HR = TY + 2, HC = TZ + 2;    // sH: rows y0-1 .. y0+TY, columns z0-1 .. z0+63
     constexpr int EC = TZ + 1;                 // sE: rows y0 .. y0+TY, columns z0 .. z0+63
@@ -353,7 +355,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ?
     if (CPML != 7) {
         // Lean tiles: the box of x chunks [lo0, hi0) and y tiles [lo1, hi1), every z tile.  Each XCD
         // (workgroup w runs on XCD w % 8) takes a contiguous range, so neighbouring tiles share L2.
-        const int G = gridDim.x, w = blockIdx.x, full = (G / 8) * 8;
+        const int full = (G / 8) * 8;
         int t = w < full ? (w % 8) * (G / 8) + (w / 8) : w;
         // CPML == 0: z tiles [zlo, zhi) free of z slabs; CPML == 4: the other z tiles.
         const int my = a.box_hi[1] - a.box_lo[1];
@@ -366,7 +368,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ?
         chunk = a.box_lo[0] + t / mz;
     } else {
         // Full tiles: the complement of the box, enumerated as four blocks, round robin over XCDs.
-        int b = blockIdx.x;
+        int b = w;
         const int per_chunk = a.ytiles * a.ztiles, inner = a.box_hi[0] - a.box_lo[0];
         const int nA = a.box_lo[0] * per_chunk, nB = (a.nchunks - a.box_hi[0]) * per_chunk;
         const int nC = inner * a.box_lo[1] * a.ztiles;
@@ -504,6 +506,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ?
     }
 }
 
+#define TFDTD_TILE_BOUNDS \
+    __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ? (RW <= 2 ? 3 : 2) : CPML == 4 ? (RW <= 2 ? 4 : 3) : (RW <= 2 ? 6 : 4), 8)))
+
+template <typename real, int RW, bool EPS3, int CPML>
+__global__ TFDTD_TILE_BOUNDS void tile_kernel(const March<real> a) {
+    tile_body<real, RW, EPS3, CPML>(a, blockIdx.x, gridDim.x);
+}
+
+// Many grids in one launch per variant.  Grid g owns workgroups [prefix[g], prefix[g+1]), a multiple
+// of 8 so that its tiles keep the XCD assignment of a launch of their own; the padding exits at once.
+template <typename real, int RW, bool EPS3, int CPML>
+__global__ TFDTD_TILE_BOUNDS void batch_tile_kernel(const March<real>* __restrict__ table, const int* __restrict__ prefix,
+                                                    int grids) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = grids;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const March<real>& a = table[lo];
+    const int count = a.blocks[CPML == 0 ? 0 : CPML == 4 ? 1 : 2];
+    const int w = b - prefix[lo];
+    if (w >= count) return;
+    tile_body<real, RW, EPS3, CPML>(a, w, count);
+}
+
+#undef TFDTD_TILE_BOUNDS
+
 template <typename real>
 int make_family(const tfdtd_march_family& in, bool split, const int dims[3], Family<real>* out) {
     std::memset(out, 0, sizeof *out);
@@ -604,52 +635,100 @@ inline void free_range(int tiles, int size, int e_extra, int n, const Face* ef, 
     *hi = best_hi;
 }
 
+// Tile layout of one grid: chunking, z shift, the slab-free boxes and the workgroups of each variant.
 template <typename real, int RW>
-int launch_tile(const tfdtd_march* d, hipStream_t stream) {
-    March<real> a;
-    int status = make_march(d, &a);
+int layout_march(const tfdtd_march* d, March<real>* a) {
+    const int status = make_march(d, a);
     if (status) return status;
     constexpr int TY = 4 * RW;
     const int chunk = d->chunk_x > 0 ? d->chunk_x : d->nx;
-    a.chunk = chunk;
-    a.ytiles = (d->ny + TY - 1) / TY;
+    a->chunk = chunk;
+    a->ytiles = (d->ny + TY - 1) / TY;
     // Shift the z tiles so one tile edge meets the end of the low z slabs (E and H).
-    const int zend = std::max(a.ef.face[2][0].hi, a.hf.face[2][0].hi);
-    a.zshift = zend > 0 ? (63 - zend % 63) % 63 : 0;
-    a.ztiles = (d->nz + a.zshift + 62) / 63;
+    const int zend = std::max(a->ef.face[2][0].hi, a->hf.face[2][0].hi);
+    a->zshift = zend > 0 ? (63 - zend % 63) % 63 : 0;
+    a->ztiles = (d->nz + a->zshift + 62) / 63;
     const int chunks = (d->nx + chunk - 1) / chunk;
-    const long long blocks = (long long)chunks * a.ytiles * a.ztiles;
+    const long long blocks = (long long)chunks * a->ytiles * a->ztiles;
     if (blocks > 0x7fffffffLL) return tfdtd_fail(TFDTD_EINVAL, "march: too many workgroups");
     // E planes of chunk c are xa .. xb (xb recomputed), so E extends one row/plane past each tile.
-    free_range(chunks, chunk, 1, d->nx, a.ef.face[0], a.hf.face[0], &a.box_lo[0], &a.box_hi[0]);
-    free_range(a.ytiles, TY, 1, d->ny, a.ef.face[1], a.hf.face[1], &a.box_lo[1], &a.box_hi[1]);
-    a.nchunks = chunks;
-    free_range(a.ztiles, 63, 1, d->nz, a.ef.face[2], a.hf.face[2], &a.zbox_lo, &a.zbox_hi, a.zshift);
-    const long long xy = (long long)(a.box_hi[0] - a.box_lo[0]) * (a.box_hi[1] - a.box_lo[1]);
-    const long long pure = xy * (a.zbox_hi - a.zbox_lo), inner = xy * a.ztiles;
-    auto lean = [&](auto kernel, long long count, const char* what) {
+    free_range(chunks, chunk, 1, d->nx, a->ef.face[0], a->hf.face[0], &a->box_lo[0], &a->box_hi[0]);
+    free_range(a->ytiles, TY, 1, d->ny, a->ef.face[1], a->hf.face[1], &a->box_lo[1], &a->box_hi[1]);
+    a->nchunks = chunks;
+    free_range(a->ztiles, 63, 1, d->nz, a->ef.face[2], a->hf.face[2], &a->zbox_lo, &a->zbox_hi, a->zshift);
+    const long long xy = (long long)(a->box_hi[0] - a->box_lo[0]) * (a->box_hi[1] - a->box_lo[1]);
+    const long long pure = xy * (a->zbox_hi - a->zbox_lo), inner = xy * a->ztiles;
+    if (inner == 0) { a->box_lo[0] = a->box_hi[0] = 0; a->box_lo[1] = a->box_hi[1] = 0; }
+    a->blocks[0] = (int)pure;
+    a->blocks[1] = (int)(inner - pure);
+    a->blocks[2] = (int)(blocks - inner);
+    return TFDTD_OK;
+}
+
+template <typename real, int RW>
+int launch_tile(const tfdtd_march* d, hipStream_t stream) {
+    March<real> a;
+    int status = layout_march<real, RW>(d, &a);
+    if (status) return status;
+    auto launch = [&](auto kernel, int count, const char* what) {
         if (count <= 0 || status) return;
         hipLaunchKernelGGL(kernel, dim3((unsigned)count), dim3(256), 0, stream, a);
         status = tfdtd_check_launch(what);
     };
     if (d->eps_stride == 3) {
-        lean(tile_kernel<real, RW, true, 0>, pure, "march interior");
-        lean(tile_kernel<real, RW, true, 4>, inner - pure, "march z faces");
+        launch(tile_kernel<real, RW, true, 0>, a.blocks[0], "march interior");
+        launch(tile_kernel<real, RW, true, 4>, a.blocks[1], "march z faces");
+        launch(tile_kernel<real, RW, true, 7>, a.blocks[2], "march boundary");
     } else {
-        lean(tile_kernel<real, RW, false, 0>, pure, "march interior");
-        lean(tile_kernel<real, RW, false, 4>, inner - pure, "march z faces");
-    }
-    if (status) return status;
-    if (inner < blocks) {
-        if (inner == 0) { a.box_lo[0] = a.box_hi[0] = 0; a.box_lo[1] = a.box_hi[1] = 0; }
-        const unsigned rest = (unsigned)(blocks - inner);
-        if (d->eps_stride == 3)
-            hipLaunchKernelGGL((tile_kernel<real, RW, true, 7>), dim3(rest), dim3(256), 0, stream, a);
-        else
-            hipLaunchKernelGGL((tile_kernel<real, RW, false, 7>), dim3(rest), dim3(256), 0, stream, a);
-        status = tfdtd_check_launch("march boundary");
+        launch(tile_kernel<real, RW, false, 0>, a.blocks[0], "march interior");
+        launch(tile_kernel<real, RW, false, 4>, a.blocks[1], "march z faces");
+        launch(tile_kernel<real, RW, false, 7>, a.blocks[2], "march boundary");
     }
     return status;
+}
+
+template <typename real, int RW>
+int pack_marches(const tfdtd_march* d, int count, void* host_table, size_t bytes, int32_t* prefix) {
+    if (!host_table || bytes < sizeof(March<real>) * (size_t)count) return tfdtd_fail(TFDTD_EINVAL, "batch march: table buffer too small");
+    March<real>* table = static_cast<March<real>*>(host_table);
+    long long total[3] = {0, 0, 0};
+    for (int k = 0; k < count; ++k) {
+        if (d[k].dtype != d[0].dtype || d[k].tile_y != d[0].tile_y || d[k].eps_stride != d[0].eps_stride)
+            return tfdtd_fail(TFDTD_EINVAL, "batch march: descriptor %d differs in dtype, tile_y or eps_stride", k);
+        const int status = layout_march<real, RW>(d + k, table + k);
+        if (status) return status;
+        for (int v = 0; v < 3; ++v) {
+            prefix[v * (count + 1) + k] = (int32_t)total[v];
+            total[v] += (table[k].blocks[v] + 7) / 8 * 8;
+            if (total[v] > 0x7fffffffLL) return tfdtd_fail(TFDTD_EINVAL, "batch march: too many workgroups for one launch");
+        }
+    }
+    for (int v = 0; v < 3; ++v) prefix[v * (count + 1) + count] = (int32_t)total[v];
+    return TFDTD_OK;
+}
+
+template <typename real, int RW, bool EPS3>
+int launch_batch_tiles(const void* table, const int32_t* prefix, int count, const int32_t* totals, hipStream_t stream) {
+    const March<real>* t = static_cast<const March<real>*>(table);
+    int status = TFDTD_OK;
+    auto launch = [&](auto kernel, int v, const char* what) {
+        if (totals[v] <= 0 || status) return;
+        hipLaunchKernelGGL(kernel, dim3((unsigned)totals[v]), dim3(256), 0, stream, t, prefix + v * (count + 1), count);
+        status = tfdtd_check_launch(what);
+    };
+    launch(batch_tile_kernel<real, RW, EPS3, 0>, 0, "batch march interior");
+    launch(batch_tile_kernel<real, RW, EPS3, 4>, 1, "batch march z faces");
+    launch(batch_tile_kernel<real, RW, EPS3, 7>, 2, "batch march boundary");
+    return status;
+}
+
+template <typename real>
+int batch_march(const void* table, const int32_t* prefix, int count, int tile_y, int eps_stride, const int32_t* totals,
+                hipStream_t stream) {
+    if (tile_y == 8) return eps_stride == 3 ? launch_batch_tiles<real, 2, true>(table, prefix, count, totals, stream)
+                                            : launch_batch_tiles<real, 2, false>(table, prefix, count, totals, stream);
+    return eps_stride == 3 ? launch_batch_tiles<real, 4, true>(table, prefix, count, totals, stream)
+                           : launch_batch_tiles<real, 4, false>(table, prefix, count, totals, stream);
 }
 
 template <typename real>
@@ -668,5 +747,36 @@ extern "C" int tfdtd_yee_step_march(const tfdtd_march* d, void* stream) {
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (d->dtype == TFDTD_F32) return dispatch_march<float>(d, s);
     if (d->dtype == TFDTD_F64) return dispatch_march<double>(d, s);
+    return tfdtd_fail(TFDTD_EINVAL, "march: dtype must be float32 or float64");
+}
+
+extern "C" size_t tfdtd_batch_march_bytes(int32_t count, int32_t dtype) {
+    if (count < 0) return 0;
+    if (dtype == TFDTD_F32) return sizeof(March<float>) * (size_t)count;
+    if (dtype == TFDTD_F64) return sizeof(March<double>) * (size_t)count;
+    return 0;
+}
+
+extern "C" int tfdtd_batch_march_pack(const tfdtd_march* steps, int32_t count, void* host_table, size_t bytes, int32_t* block_prefix) {
+    if (!steps || count < 1 || !block_prefix) return tfdtd_fail(TFDTD_EINVAL, "batch march: at least one descriptor and a prefix array");
+    const int ty = steps[0].tile_y;
+    if (ty != 8 && ty != 16) return tfdtd_fail(TFDTD_EINVAL, "march: tile_y must be 8 or 16");
+    if (steps[0].dtype == TFDTD_F32)
+        return ty == 8 ? pack_marches<float, 2>(steps, count, host_table, bytes, block_prefix)
+                       : pack_marches<float, 4>(steps, count, host_table, bytes, block_prefix);
+    if (steps[0].dtype == TFDTD_F64)
+        return ty == 8 ? pack_marches<double, 2>(steps, count, host_table, bytes, block_prefix)
+                       : pack_marches<double, 4>(steps, count, host_table, bytes, block_prefix);
+    return tfdtd_fail(TFDTD_EINVAL, "march: dtype must be float32 or float64");
+}
+
+extern "C" int tfdtd_batch_march(const void* device_table, const int32_t* device_prefix, int32_t count, int32_t dtype,
+                                 int32_t tile_y, int32_t eps_stride, const int32_t* totals, void* stream) {
+    if (!device_table || !device_prefix || count < 1 || !totals) return tfdtd_fail(TFDTD_EINVAL, "batch march: bad launch arguments");
+    if (tile_y != 8 && tile_y != 16) return tfdtd_fail(TFDTD_EINVAL, "march: tile_y must be 8 or 16");
+    if (eps_stride != 1 && eps_stride != 3) return tfdtd_fail(TFDTD_EINVAL, "march: eps_stride must be 1 or 3");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (dtype == TFDTD_F32) return batch_march<float>(device_table, device_prefix, count, tile_y, eps_stride, totals, s);
+    if (dtype == TFDTD_F64) return batch_march<double>(device_table, device_prefix, count, tile_y, eps_stride, totals, s);
     return tfdtd_fail(TFDTD_EINVAL, "march: dtype must be float32 or float64");
 }

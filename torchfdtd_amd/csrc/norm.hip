@@ -1,0 +1,185 @@
+// State norms for the run-control diagnostics (torchfdtd/run_control.py:127-159,
+// torchfdtd/cuda_diagnostics.py:63-135): per grid, in FP64,
+//   energy = sum_c vol E_c^2 / inv_eps_c + vol H_c^2,   peak = max |E|, |H|,
+// and a flag for any non-finite value of E, H or the CPML memory.
+//
+// Two launches for any number of grids: norm_partial reduces one 4096-element task of one
+// array per workgroup; norm_final reduces each grid's task range with one workgroup per
+// grid.  Every grid's result lands in out[grid][3] (energy, peak, non-finite flag), so a
+// whole batch is measured with one pair of launches and one 24*grids byte copy.
+
+#include "common.h"
+
+namespace {
+
+constexpr int kTask = 4096;
+constexpr int kThreads = 256;
+
+struct NormGrid {
+    int nx, ny, nz, eps_stride;
+    const void* E;
+    const void* H;
+    const void* inverse;
+    const double* widths[3];
+    const void* psi[2];
+    long long psi_count[2];
+    int task_lo, task_hi;
+};
+
+struct NormTask {
+    int grid, part;        // part 0 E, 1 H, 2 and 3 the CPML memory buffers
+    long long start;
+};
+
+__device__ __forceinline__ void block_reduce(double sum, double peak, double bad, double* out) {
+    __shared__ double s[3][kThreads];
+    const int t = threadIdx.x;
+    s[0][t] = sum;
+    s[1][t] = peak;
+    s[2][t] = bad;
+    __syncthreads();
+    for (int k = kThreads / 2; k > 0; k >>= 1) {
+        if (t < k) {
+            s[0][t] += s[0][t + k];
+            s[1][t] = fmax(s[1][t], s[1][t + k]);
+            s[2][t] = fmax(s[2][t], s[2][t + k]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = s[0][0];
+        out[1] = s[1][0];
+        out[2] = s[2][0];
+    }
+}
+
+template <typename real>
+__global__ __launch_bounds__(kThreads) void norm_partial(const NormGrid* __restrict__ grids, const NormTask* __restrict__ tasks,
+                                                        double* __restrict__ partial) {
+    const NormTask task = tasks[blockIdx.x];
+    const NormGrid& g = grids[task.grid];
+    double sum = 0., peak = 0., bad = 0.;
+    if (task.part < 2) {
+        const real* data = static_cast<const real*>(task.part == 0 ? g.E : g.H);
+        const real* inverse = static_cast<const real*>(g.inverse);
+        const long long n = 3LL * g.nx * g.ny * g.nz;
+        const long long end = min(n, task.start + kTask);
+        for (long long i = task.start + threadIdx.x; i < end; i += kThreads) {
+            const double v = (double)data[i];
+            if (!isfinite(v)) bad = 1.;
+            const long long cell = i / 3;
+            double vol = 1.;
+            if (g.widths[0]) {
+                const int z = (int)(cell % g.nz);
+                const int y = (int)((cell / g.nz) % g.ny);
+                const int x = (int)(cell / ((long long)g.nz * g.ny));
+                vol = g.widths[0][x] * g.widths[1][y] * g.widths[2][z];
+            }
+            double w = vol;
+            if (task.part == 0) w = vol / (double)inverse[g.eps_stride == 3 ? i : cell];
+            sum += v * v * w;
+            peak = fmax(peak, fabs(v));
+        }
+    } else {
+        const int k = task.part - 2;
+        const real* data = static_cast<const real*>(g.psi[k]);
+        const long long end = min(g.psi_count[k], task.start + kTask);
+        for (long long i = task.start + threadIdx.x; i < end; i += kThreads)
+            if (!isfinite((double)data[i])) bad = 1.;
+    }
+    block_reduce(sum, peak, bad, partial + 3LL * blockIdx.x);
+}
+
+__global__ __launch_bounds__(kThreads) void norm_final(const NormGrid* __restrict__ grids, const double* __restrict__ partial,
+                                                      double* __restrict__ out) {
+    const NormGrid& g = grids[blockIdx.x];
+    double sum = 0., peak = 0., bad = 0.;
+    for (int t = g.task_lo + threadIdx.x; t < g.task_hi; t += kThreads) {
+        sum += partial[3LL * t];
+        peak = fmax(peak, partial[3LL * t + 1]);
+        bad = fmax(bad, partial[3LL * t + 2]);
+    }
+    block_reduce(sum, peak, bad, out + 3LL * blockIdx.x);
+}
+
+long long tasks_of(long long n) { return (n + kTask - 1) / kTask; }
+
+long long grid_tasks(const tfdtd_norm_grid& d) {
+    const long long n = 3LL * d.nx * d.ny * d.nz;
+    return 2 * tasks_of(n) + tasks_of(d.psi[0] ? d.psi_count[0] : 0) + tasks_of(d.psi[1] ? d.psi_count[1] : 0);
+}
+
+size_t table_bytes(int count, long long tasks) {
+    return sizeof(NormGrid) * (size_t)count + sizeof(NormTask) * (size_t)tasks;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t tfdtd_norm_tasks(const tfdtd_norm_grid* grids, int32_t count) {
+    if (!grids || count < 1) return -1;
+    long long tasks = 0;
+    for (int k = 0; k < count; ++k) tasks += grid_tasks(grids[k]);
+    return tasks;
+}
+
+size_t tfdtd_norm_table_bytes(const tfdtd_norm_grid* grids, int32_t count) {
+    const int64_t tasks = tfdtd_norm_tasks(grids, count);
+    return tasks < 0 ? 0 : table_bytes(count, tasks);
+}
+
+int tfdtd_norm_pack(const tfdtd_norm_grid* grids, int32_t count, void* host_table, size_t bytes) {
+    const int64_t total = tfdtd_norm_tasks(grids, count);
+    if (total < 0) return tfdtd_fail(TFDTD_EINVAL, "norm: at least one grid");
+    if (total > 0x7fffffffLL) return tfdtd_fail(TFDTD_EINVAL, "norm: too many tasks for one launch");
+    if (!host_table || bytes < table_bytes(count, total)) return tfdtd_fail(TFDTD_EINVAL, "norm: table buffer too small");
+    NormGrid* table = static_cast<NormGrid*>(host_table);
+    NormTask* tasks = reinterpret_cast<NormTask*>(table + count);
+    long long t = 0;
+    for (int k = 0; k < count; ++k) {
+        const tfdtd_norm_grid& d = grids[k];
+        if (d.dtype != grids[0].dtype) return tfdtd_fail(TFDTD_EINVAL, "norm: grid %d differs in dtype", k);
+        if (d.nx < 1 || d.ny < 1 || d.nz < 1 || !d.E || !d.H || !d.inverse_permittivity)
+            return tfdtd_fail(TFDTD_EINVAL, "norm: grid %d lacks a shape, a field or the inverse permittivity", k);
+        if (d.eps_stride != 1 && d.eps_stride != 3) return tfdtd_fail(TFDTD_EINVAL, "norm: eps_stride must be 1 or 3");
+        const bool weighted = d.widths[0] || d.widths[1] || d.widths[2];
+        if (weighted && !(d.widths[0] && d.widths[1] && d.widths[2]))
+            return tfdtd_fail(TFDTD_EINVAL, "norm: grid %d: give all three width arrays or none", k);
+        NormGrid& g = table[k];
+        g.nx = d.nx; g.ny = d.ny; g.nz = d.nz; g.eps_stride = d.eps_stride;
+        g.E = d.E; g.H = d.H; g.inverse = d.inverse_permittivity;
+        for (int a = 0; a < 3; ++a) g.widths[a] = weighted ? d.widths[a] : nullptr;
+        for (int j = 0; j < 2; ++j) {
+            g.psi[j] = d.psi[j];
+            g.psi_count[j] = d.psi[j] ? d.psi_count[j] : 0;
+            if (g.psi_count[j] < 0) return tfdtd_fail(TFDTD_EINVAL, "norm: negative CPML memory size");
+        }
+        g.task_lo = (int)t;
+        const long long n = 3LL * d.nx * d.ny * d.nz;
+        for (int part = 0; part < 4; ++part) {
+            const long long size = part < 2 ? n : g.psi_count[part - 2];
+            for (long long s = 0; s < size; s += kTask) tasks[t++] = NormTask{k, part, s};
+        }
+        g.task_hi = (int)t;
+    }
+    return TFDTD_OK;
+}
+
+int tfdtd_state_norm(const void* device_table, int32_t count, int64_t tasks, int32_t dtype, double* partial, double* out,
+                     void* stream) {
+    if (!device_table || count < 1 || tasks < 1 || tasks > 0x7fffffffLL || !partial || !out)
+        return tfdtd_fail(TFDTD_EINVAL, "norm: bad launch arguments");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const NormGrid* grids = static_cast<const NormGrid*>(device_table);
+    const NormTask* list = reinterpret_cast<const NormTask*>(grids + count);
+    if (dtype == TFDTD_F32) hipLaunchKernelGGL(norm_partial<float>, dim3((unsigned)tasks), dim3(kThreads), 0, s, grids, list, partial);
+    else if (dtype == TFDTD_F64) hipLaunchKernelGGL(norm_partial<double>, dim3((unsigned)tasks), dim3(kThreads), 0, s, grids, list, partial);
+    else return tfdtd_fail(TFDTD_EINVAL, "norm: dtype must be float32 or float64");
+    int status = tfdtd_check_launch("norm_partial");
+    if (status) return status;
+    hipLaunchKernelGGL(norm_final, dim3((unsigned)count), dim3(kThreads), 0, s, grids, partial, out);
+    return tfdtd_check_launch("norm_final");
+}
+
+}  // extern "C"

@@ -9,6 +9,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -60,13 +61,10 @@ struct HalfStep {
     const long long* counter;
 };
 
-// One cell per thread.  forward=false: E update from backward differences of H;
+// One cell.  forward=false: E update from backward differences of H;
 // forward=true: H update from forward differences of E.
 template <typename real, bool forward>
-__global__ __launch_bounds__(256) void yee_half_step_kernel(const HalfStep<real> a) {
-    const long long count = (long long)a.nx * a.ny * a.nz;
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
+__device__ __forceinline__ void half_step_cell(const HalfStep<real>& a, const long long i) {
     const int nyz = a.ny * a.nz;
     const int x = (int)(i / nyz);
     const int r = (int)(i - (long long)x * nyz);
@@ -136,12 +134,35 @@ __global__ __launch_bounds__(256) void yee_half_step_kernel(const HalfStep<real>
     for (int c = 0; c < 3; ++c) a.dst[3 * i + c] = out[c];
 }
 
+template <typename real, bool forward>
+__global__ __launch_bounds__(256) void yee_half_step_kernel(const HalfStep<real> a) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)a.nx * a.ny * a.nz) return;
+    half_step_cell<real, forward>(a, i);
+}
+
+// Many grids in one launch: block b belongs to the last grid g with prefix[g] <= b; the grids'
+// descriptors are a table in device memory, read with wave-uniform loads.
+template <typename real, bool forward>
+__global__ __launch_bounds__(256) void batch_half_step_kernel(const HalfStep<real>* __restrict__ table,
+                                                              const int* __restrict__ prefix, int grids) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = grids;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const HalfStep<real>& a = table[lo];
+    const long long i = (long long)(b - prefix[lo]) * blockDim.x + threadIdx.x;
+    if (i >= (long long)a.nx * a.ny * a.nz) return;
+    half_step_cell<real, forward>(a, i);
+}
+
 template <typename real>
 struct Observe {
     int count;
     const long long* table;
-    const real* E;
-    const real* H;
     real* traces;
     long long steps;
     long long* counter;
@@ -154,8 +175,8 @@ __global__ __launch_bounds__(256) void observe_kernel(const Observe<real> a) {
     __syncthreads();
     if (n < a.steps) {
         for (int k = threadIdx.x; k < a.count; k += blockDim.x) {
-            const long long family = a.table[2 * k], index = a.table[2 * k + 1];
-            a.traces[n * a.count + k] = (family ? a.H : a.E)[index];
+            const real* field = reinterpret_cast<const real*>(a.table[2 * k]);
+            a.traces[n * a.count + k] = field[a.table[2 * k + 1]];
         }
     }
     __syncthreads();
@@ -182,7 +203,7 @@ struct Plane {
 // One thread per (point, component) sample; the block first evaluates the nf
 // absolute-time phases exp(i omega (n+1)) in FP64 (cuda_monitors.py:68-84) into LDS.
 template <typename real, typename spec>
-__global__ __launch_bounds__(256) void dft_plane_kernel(const Plane<real, spec> a) {
+__device__ __forceinline__ void dft_plane_block(const Plane<real, spec>& a, long long block) {
     extern __shared__ unsigned char lds_raw[];
     spec* phase = reinterpret_cast<spec*>(lds_raw);
     const long long n = *a.counter;
@@ -199,7 +220,7 @@ __global__ __launch_bounds__(256) void dft_plane_kernel(const Plane<real, spec> 
     }
     __syncthreads();
     const long long samples = a.points * a.nc;
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long i = block * blockDim.x + threadIdx.x;
     if (i >= samples) return;
     const int c = (int)(i % a.nc);
     const long long p = i / a.nc;
@@ -217,6 +238,19 @@ __global__ __launch_bounds__(256) void dft_plane_kernel(const Plane<real, spec> 
         out[0] = out[0] + phase[4 * f + o] * sv;
         out[1] = out[1] + phase[4 * f + o + 1] * sv;
     }
+}
+
+template <typename real, typename spec>
+__global__ __launch_bounds__(256) void dft_plane_kernel(const Plane<real, spec> a) {
+    dft_plane_block(a, blockIdx.x);
+}
+
+// Many planes in one launch: blockIdx.y selects the plane of a device table.
+template <typename real, typename spec>
+__global__ __launch_bounds__(256) void batch_dft_kernel(const Plane<real, spec>* __restrict__ table) {
+    const Plane<real, spec>& a = table[blockIdx.y];
+    if ((long long)blockIdx.x * blockDim.x >= a.points * a.nc) return;
+    dft_plane_block(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------- host side
@@ -289,17 +323,31 @@ int launch_half_step(const tfdtd_half_step* d, hipStream_t stream) {
 template <typename real>
 int launch_observe(const tfdtd_observe* d, hipStream_t stream) {
     if (!d->counter) return fail(TFDTD_EINVAL, "observe needs the device step counter");
-    if (d->count < 0 || (d->count > 0 && (!d->table || !d->traces || !d->E || !d->H)))
-        return fail(TFDTD_EINVAL, "observe: trace table, traces and fields are required");
-    Observe<real> a{d->count, reinterpret_cast<const long long*>(d->table), static_cast<const real*>(d->E),
-                    static_cast<const real*>(d->H), static_cast<real*>(d->traces), (long long)d->steps,
-                    reinterpret_cast<long long*>(d->counter)};
+    if (d->count < 0 || (d->count > 0 && (!d->table || !d->traces)))
+        return fail(TFDTD_EINVAL, "observe: trace table and traces are required");
+    Observe<real> a{d->count, reinterpret_cast<const long long*>(d->table), static_cast<real*>(d->traces),
+                    (long long)d->steps, reinterpret_cast<long long*>(d->counter)};
     hipLaunchKernelGGL((observe_kernel<real>), dim3(1), dim3(256), 0, stream, a);
     return check_launch("observe");
 }
 
 template <typename real, typename spec>
+int make_plane(const tfdtd_plane* d, Plane<real, spec>* out);
+
+template <typename real, typename spec>
 int launch_plane(const tfdtd_plane* d, hipStream_t stream) {
+    Plane<real, spec> a;
+    int status = make_plane(d, &a);
+    if (status) return status;
+    const size_t lds = 4 * sizeof(spec) * (size_t)d->frequencies;
+    const long long blocks = (d->points * d->components + 255) / 256;
+    hipLaunchKernelGGL((dft_plane_kernel<real, spec>), dim3((unsigned)blocks), dim3(256), lds, stream, a);
+    return check_launch("dft_plane");
+}
+
+template <typename real, typename spec>
+int make_plane(const tfdtd_plane* d, Plane<real, spec>* out) {
+    Plane<real, spec>& a = *out;
     if (d->points < 1 || d->frequencies < 1 || d->components < 1 || d->components > 6 || d->electric < 0 ||
         d->electric > d->components || d->corners < 1 || d->corners > 8 || d->time_downsample < 1)
         return fail(TFDTD_EINVAL, "plane: bad sizes");
@@ -307,7 +355,6 @@ int launch_plane(const tfdtd_plane* d, hipStream_t stream) {
         return fail(TFDTD_EINVAL, "plane: null pointer");
     const size_t lds = 4 * sizeof(spec) * (size_t)d->frequencies;
     if (lds > 64 * 1024) return fail(TFDTD_EINVAL, "plane: %d frequencies exceed one launch", d->frequencies);
-    Plane<real, spec> a;
     std::memset(&a, 0, sizeof a);
     a.points = d->points; a.nf = d->frequencies; a.nc = d->components; a.ne = d->electric;
     a.corners = d->corners; a.downsample = d->time_downsample;
@@ -322,10 +369,8 @@ int launch_plane(const tfdtd_plane* d, hipStream_t stream) {
     a.counter = reinterpret_cast<const long long*>(d->counter);
     if (d->samples < 0) return fail(TFDTD_EINVAL, "plane: negative window length");
     a.samples = d->samples;
-    const long long samples = d->points * d->components;
-    const long long blocks = (samples + 255) / 256;
-    hipLaunchKernelGGL((dft_plane_kernel<real, spec>), dim3((unsigned)blocks), dim3(256), lds, stream, a);
-    return check_launch("dft_plane");
+    if ((d->points * d->components + 255) / 256 > 0x7fffffffLL) return fail(TFDTD_EINVAL, "plane: too many samples");
+    return TFDTD_OK;
 }
 
 int half_step(const tfdtd_half_step* d, hipStream_t stream) {
@@ -357,9 +402,115 @@ int observe(const tfdtd_observe* d, hipStream_t stream) {
     return fail(TFDTD_EINVAL, "dtype must be float32 or float64");
 }
 
+template <typename real>
+int pack_half_steps(const tfdtd_half_step* steps, int count, void* host_table, size_t bytes, int32_t* prefix) {
+    if (count < 1 || !steps || !prefix) return fail(TFDTD_EINVAL, "batch: at least one descriptor and a prefix array");
+    if (!host_table || bytes < sizeof(HalfStep<real>) * (size_t)count) return fail(TFDTD_EINVAL, "batch: table buffer too small");
+    HalfStep<real>* table = static_cast<HalfStep<real>*>(host_table);
+    long long total = 0;
+    for (int k = 0; k < count; ++k) {
+        if (steps[k].family != steps[0].family || steps[k].dtype != steps[0].dtype)
+            return fail(TFDTD_EINVAL, "batch: descriptor %d differs in family or dtype", k);
+        const int status = make_half_step(steps + k, table + k);
+        if (status) return status;
+        prefix[k] = (int32_t)total;
+        total += ((long long)steps[k].nx * steps[k].ny * steps[k].nz + 255) / 256;
+        if (total > 0x7fffffffLL) return fail(TFDTD_EINVAL, "batch: too many cells for one launch");
+    }
+    prefix[count] = (int32_t)total;
+    return TFDTD_OK;
+}
+
+template <typename real, typename spec>
+int pack_planes(const tfdtd_plane* planes, int count, void* host_table, size_t bytes, int32_t* max_blocks, int32_t* max_frequencies) {
+    if (count < 1 || !planes || !max_blocks || !max_frequencies) return fail(TFDTD_EINVAL, "batch planes: bad arguments");
+    if (!host_table || bytes < sizeof(Plane<real, spec>) * (size_t)count) return fail(TFDTD_EINVAL, "batch planes: table buffer too small");
+    Plane<real, spec>* table = static_cast<Plane<real, spec>*>(host_table);
+    *max_blocks = 0;
+    *max_frequencies = 0;
+    for (int k = 0; k < count; ++k) {
+        const int status = make_plane(planes + k, table + k);
+        if (status) return status;
+        *max_blocks = std::max<int32_t>(*max_blocks, (int32_t)((planes[k].points * planes[k].components + 255) / 256));
+        *max_frequencies = std::max(*max_frequencies, planes[k].frequencies);
+    }
+    return TFDTD_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+size_t tfdtd_batch_table_bytes(int32_t count, int32_t dtype) {
+    if (count < 0) return 0;
+    return (size_t)count * (dtype == TFDTD_F64 ? sizeof(HalfStep<double>) : sizeof(HalfStep<float>));
+}
+
+int tfdtd_batch_pack(const tfdtd_half_step* steps, int32_t count, void* host_table, size_t bytes, int32_t* block_prefix) {
+    if (count < 1 || !steps) return fail(TFDTD_EINVAL, "batch: at least one descriptor");
+    if (steps[0].dtype == TFDTD_F32) return pack_half_steps<float>(steps, count, host_table, bytes, block_prefix);
+    if (steps[0].dtype == TFDTD_F64) return pack_half_steps<double>(steps, count, host_table, bytes, block_prefix);
+    return fail(TFDTD_EINVAL, "batch: dtype must be float32 or float64");
+}
+
+int tfdtd_batch_half_step(const void* device_table, const int32_t* device_prefix, int32_t count, int32_t total_blocks,
+                          int32_t dtype, int32_t family, void* stream) {
+    if (!device_table || !device_prefix || count < 1 || total_blocks < 0) return fail(TFDTD_EINVAL, "batch: bad launch arguments");
+    if (total_blocks == 0) return TFDTD_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int* prefix = reinterpret_cast<const int*>(device_prefix);
+    if (dtype == TFDTD_F32) {
+        const HalfStep<float>* t = static_cast<const HalfStep<float>*>(device_table);
+        if (family == TFDTD_FAMILY_E) hipLaunchKernelGGL((batch_half_step_kernel<float, false>), dim3(total_blocks), dim3(256), 0, s, t, prefix, count);
+        else hipLaunchKernelGGL((batch_half_step_kernel<float, true>), dim3(total_blocks), dim3(256), 0, s, t, prefix, count);
+    } else if (dtype == TFDTD_F64) {
+        const HalfStep<double>* t = static_cast<const HalfStep<double>*>(device_table);
+        if (family == TFDTD_FAMILY_E) hipLaunchKernelGGL((batch_half_step_kernel<double, false>), dim3(total_blocks), dim3(256), 0, s, t, prefix, count);
+        else hipLaunchKernelGGL((batch_half_step_kernel<double, true>), dim3(total_blocks), dim3(256), 0, s, t, prefix, count);
+    } else {
+        return fail(TFDTD_EINVAL, "batch: dtype must be float32 or float64");
+    }
+    return check_launch("batch_half_step");
+}
+
+size_t tfdtd_batch_planes_bytes(int32_t count, int32_t dtype, int32_t spectral) {
+    if (count < 0) return 0;
+    if (dtype == TFDTD_F32 && spectral == 0) return (size_t)count * sizeof(Plane<float, float>);
+    if (dtype == TFDTD_F32 && spectral == 1) return (size_t)count * sizeof(Plane<float, double>);
+    return (size_t)count * sizeof(Plane<double, double>);
+}
+
+int tfdtd_batch_planes_pack(const tfdtd_plane* p, int32_t count, void* host_table, size_t bytes, int32_t* max_blocks,
+                            int32_t* max_frequencies) {
+    if (count < 1 || !p) return fail(TFDTD_EINVAL, "batch planes: at least one plane");
+    for (int k = 1; k < count; ++k)
+        if (p[k].dtype != p[0].dtype || p[k].spectral != p[0].spectral) return fail(TFDTD_EINVAL, "batch planes: mixed precision");
+    if (p[0].dtype == TFDTD_F32 && p[0].spectral == 0) return pack_planes<float, float>(p, count, host_table, bytes, max_blocks, max_frequencies);
+    if (p[0].dtype == TFDTD_F32 && p[0].spectral == 1) return pack_planes<float, double>(p, count, host_table, bytes, max_blocks, max_frequencies);
+    if (p[0].dtype == TFDTD_F64 && p[0].spectral == 1) return pack_planes<double, double>(p, count, host_table, bytes, max_blocks, max_frequencies);
+    return fail(TFDTD_EINVAL, "batch planes: unsupported dtype/spectral pair");
+}
+
+int tfdtd_batch_dft_update(const void* device_table, int32_t count, int32_t max_blocks, int32_t max_frequencies,
+                           int32_t dtype, int32_t spectral, void* stream) {
+    if (!device_table || count < 1 || max_blocks < 1 || max_frequencies < 1) return fail(TFDTD_EINVAL, "batch planes: bad launch arguments");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const dim3 grid((unsigned)max_blocks, (unsigned)count);
+    if (dtype == TFDTD_F32 && spectral == 0) {
+        const size_t lds = 4 * sizeof(float) * (size_t)max_frequencies;
+        hipLaunchKernelGGL((batch_dft_kernel<float, float>), grid, dim3(256), lds, s, static_cast<const Plane<float, float>*>(device_table));
+    } else if (dtype == TFDTD_F32 && spectral == 1) {
+        const size_t lds = 4 * sizeof(double) * (size_t)max_frequencies;
+        hipLaunchKernelGGL((batch_dft_kernel<float, double>), grid, dim3(256), lds, s, static_cast<const Plane<float, double>*>(device_table));
+    } else if (dtype == TFDTD_F64 && spectral == 1) {
+        const size_t lds = 4 * sizeof(double) * (size_t)max_frequencies;
+        hipLaunchKernelGGL((batch_dft_kernel<double, double>), grid, dim3(256), lds, s, static_cast<const Plane<double, double>*>(device_table));
+    } else {
+        return fail(TFDTD_EINVAL, "batch planes: unsupported dtype/spectral pair");
+    }
+    return check_launch("batch_dft");
+}
+
 
 int tfdtd_abi_version(void) { return TFDTD_ABI_VERSION; }
 

@@ -7,9 +7,6 @@ from __future__ import annotations
 
 import math
 
-import numpy as np
-import torch
-
 from .planning import TAIL_OUTER, pulse_parameters
 
 
@@ -64,18 +61,18 @@ class DecayDecision:
 
 
 class StateDiagnostics:
-    """Sum_c vol E_c^2 / inv_eps_c + vol H_c^2 in FP64, max |E|,|H|, and a finiteness flag of the CPML memory."""
+    """Sum_c vol E_c^2 / inv_eps_c + vol H_c^2 in FP64 and max |E|,|H| of one grid (HIP reduction),
+    raising FloatingPointError when E, H or the CPML memory holds a non-finite value."""
 
     backend = 'hip'
 
     def __init__(self, grid):
+        from .diagnostics import StateNorms
         self.grid = grid
-        r = grid.region
-        widths = [np.diff(nodes)/r.reference_step if n > 1 else np.ones(1) for nodes, n in zip(r.mesh_nodes, r.shape)]
-        self.uniform = all(np.all(w == 1) for w in widths)
-        volume = widths[0][:, None, None]*widths[1][None, :, None]*widths[2][None, None, :]
-        self.volume = torch.as_tensor(volume, device=grid.E.device, dtype=torch.float64)
+        self.norms = StateNorms([grid])
 
     def measure(self):
-        from .diagnostics import state_norm
-        return state_norm(self.grid, self.volume)
+        energy, peak, bad = self.norms.values()[0]
+        if bad:
+            raise FloatingPointError('Non-finite full-domain or CPML state detected.')
+        return float(energy), float(peak)

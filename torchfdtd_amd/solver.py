@@ -108,32 +108,42 @@ def _json_default(value):
     raise TypeError(type(value))
 
 
+def _check_epsilon(eps, region):
+    shape = tuple(region.shape)
+    if eps.shape not in (shape, shape+(3,)):
+        raise ValueError(f'epsilon must have shape {shape} or {shape+(3,)}, not {eps.shape}')
+    if not np.isfinite(eps).all() or (eps <= 0).any():
+        raise ValueError('epsilon must be finite and positive')
+    return eps
+
+
 class Stepper:
     """Device state of one resident run and its step (the reference's closure, solver.py:776-797)."""
 
-    def __init__(self, project, steps=None, kernel=None, one_pass=None, tile_y=8, chunk_x=8):
+    def __init__(self, project, steps=None, kernel=None, one_pass=None, tile_y=8, chunk_x=8, counter=None, epsilon=None):
         p, r = project, project.region
         self.project = p
         self.steps = r.steps if steps is None else steps
         self.kernel = kernel or resolve_cuda_kernel(r)
-        self.eps = voxelize(p)
+        # An explicit relative permittivity (a design already mapped to the grid) replaces the voxelized structures.
+        self.eps = voxelize(p) if epsilon is None else _check_epsilon(np.asarray(epsilon), r)
         self.grid = g = YeeGrid(r)
         g.set_permittivity(self.eps)
         self.lib = N.library()
         device, real = g.E.device, g.dtype
-        self.counter = torch.zeros(1, device=device, dtype=torch.long)
+        # A batch shares one counter among its grids.
+        self.counter = torch.zeros(1, device=device, dtype=torch.long) if counter is None else counter
         terms = source_terms(p, self.steps)
         self.sources = {'E': [], 'H': []}
         for t in terms:
             self.sources[t.family].append((t.comp, t.box, torch.as_tensor(t.values, device=device)))
         self.point_monitors = [p.resolved_monitor(m) for m in p.monitors if m.enabled and m.kind == 'point']
-        table = []
+        self.trace_points = []
         for m in self.point_monitors:
             loc = index_at(m.center, r, m.component)
             flat = ((loc[0]*r.shape[1]+loc[1])*r.shape[2]+loc[2])*3+'xyz'.index(m.component[1].lower())
-            table.append((int(m.component[0] == 'H'), flat))
-        self.trace_table = torch.as_tensor(np.array(table, dtype=np.int64).reshape(-1, 2), device=device)
-        self.traces = torch.zeros((self.steps, len(table)), device=device, dtype=real)
+            self.trace_points.append((m.component[0] == 'H', flat))
+        self.traces = torch.zeros((self.steps, len(self.trace_points)), device=device, dtype=real)
         self.planes = [FrequencyPlane(g, p.resolved_monitor(m)) for m in p.monitors if m.enabled and m.kind == 'field']
         if self.kernel == 'torch':
             configure_cuda_kernel(g, 'torch')
@@ -144,13 +154,17 @@ class Stepper:
         self.tile_y, self.chunk_x = tile_y, chunk_x
         self._describe()
 
-    def _observe(self, E, H):
+    def _trace_table(self, E, H):
+        """int64 (address, element index) of every point trace in the fields E, H."""
+        rows = [((H if magnetic else E).data_ptr(), flat) for magnetic, flat in self.trace_points]
+        return torch.as_tensor(np.array(rows, dtype=np.int64).reshape(-1, 2), device=E.device)
+
+    def _observe(self, table):
         g = self.grid
         o = N.Observe()
         o.dtype = N.F64 if g.dtype == torch.float64 else N.F32
         o.count = len(self.point_monitors)
-        o.table = self.trace_table.data_ptr() if o.count else None
-        o.E, o.H = E.data_ptr(), H.data_ptr()
+        o.table = table.data_ptr() if o.count else None
         o.traces = self.traces.data_ptr() if o.count else None
         o.steps = self.steps
         o.counter = self.counter.data_ptr()
@@ -168,7 +182,8 @@ class Stepper:
         self.e_step = half_step_descriptor(g, False, self.sources['E'] if self.kernel == 'hip' else (), self.counter)
         self.h_step = half_step_descriptor(g, True, self.sources['H'] if self.kernel == 'hip' else (), self.counter)
         self.plane_table = self._planes(g.E, g.H)
-        self.observe = self._observe(g.E, g.H)
+        self.trace_table = self._trace_table(g.E, g.H)
+        self.observe = self._observe(self.trace_table)
         if self.one_pass:
             # Second buffer set of the one-pass step: time n is read from one set, n+1 written to the other.
             self.E_b, self.H_b = torch.empty_like(g.E), torch.empty_like(g.H)
@@ -179,7 +194,8 @@ class Stepper:
                           march_descriptor(g, self.E_b, self.H_b, g.E, g.H, self.psi_e_b, psi_a, self.sources, self.counter,
                                            self.tile_y, self.chunk_x)]
             self.plane_table_b = self._planes(self.E_b, self.H_b)
-            self.observe_b = self._observe(self.E_b, self.H_b)
+            self.trace_table_b = self._trace_table(self.E_b, self.H_b)
+            self.observe_b = self._observe(self.trace_table_b)
 
     def states(self):
         extra = [self.E_b, self.H_b, self.psi_e_b] if self.one_pass else []
@@ -230,8 +246,9 @@ class Stepper:
 class Simulation:
     """Drop-in for the reference's Simulation (solver.py:642-670) on the HIP engine."""
 
-    def __init__(self, project: Project):
+    def __init__(self, project: Project, epsilon=None):
         self.project = Project.model_validate(project.model_dump())
+        self.epsilon = epsilon
 
     def run(self, progress=None, cancel=None, cuda_graph=True, cuda_graph_steps=1):
         with ENGINE_LOCK, torch.no_grad():
@@ -250,24 +267,16 @@ class Simulation:
             raise RuntimeError('The HIP engine requires a ROCm GPU; none is visible.')
         validate_graph_steps(cuda_graph_steps, cuda_graph)
         started = time.perf_counter()
-        run = Stepper(p)
+        run = Stepper(p, epsilon=self.epsilon)
         g = run.grid
         control = r.run_control
         diagnostics = StateDiagnostics(g) if control.auto_shutoff or control.divergence_check else None
         decision = DecayDecision(control, source_end_time(p), g.time_step)
         graph = (StepGraphs(run.step, run.states(), r.steps, cuda_graph_steps, pair=run.step_pair if run.one_pass else None)
                  if cuda_graph else None)
-        axis = 'xyz'.index(r.slice_axis)
-        plane_index = index_at(tuple(r.slice_position if i == axis else 0 for i in range(3)), r, r.field)[axis]
-        component = 'xyz'.index(r.field[1].lower())
-        eps = run.eps
-
-        def plane(a):
-            sl = [slice(None)]*3
-            sl[axis] = plane_index
-            data = a[tuple(sl)]
-            return data[::max(1, math.ceil(data.shape[0]/256)), ::max(1, math.ceil(data.shape[1]/256))]
-        eps_plane = np.array(plane(eps[..., component] if eps.ndim == 4 else eps))
+        view = DisplaySlice(p)
+        plane_index = view.index
+        eps_plane = view.epsilon(run.eps)
         interval = max(r.snapshot_interval, math.ceil(r.steps/100))
         frames, frame_steps = [], []
         torch.cuda.synchronize()
@@ -299,7 +308,7 @@ class Simulation:
             if stop:
                 termination = 'decayed'
             if completed % interval == 0 or completed == r.steps or stop:
-                frame = plane((g.E if r.field[0] == 'E' else g.H)[..., component]).cpu().numpy().copy()
+                frame = view.frame(g)
                 if not np.isfinite(frame).all():
                     raise FloatingPointError('Fields diverged. Check mesh, sources and precision.')
                 frames.append(frame)
@@ -316,21 +325,53 @@ class Simulation:
         e, h = g.E.cpu().numpy().copy(), g.H.cpu().numpy().copy()
         if not np.isfinite(e).all() or not np.isfinite(h).all():
             raise FloatingPointError('Non-finite fields detected.')
-        props = torch.cuda.get_device_properties(g.device)
-        summary = dict(shape=list(r.shape), cells=math.prod(r.shape), dt_fs=r.time_step*1e15,
-                       duration_fs=r.time_step*r.steps*1e15, backend='cuda', precision=r.precision,
-                       gpu=props.name, gpu_arch=getattr(props, 'gcnArchName', None),
-                       cuda_graph=graph is not None, cuda_graph_steps=width, cuda_graph_replays=replays,
-                       cuda_kernel=run.kernel, cuda_monitor_kernel='hip', engine='torchfdtd_amd HIP gfx950',
-                       steps=completed, requested_steps=r.steps, cancelled=termination == 'cancelled',
-                       termination_reason=termination, auto_shutoff=termination == 'decayed',
-                       diagnostics=decision.history, diagnostic_seconds=diagnostic_seconds,
-                       diagnostic_backend=diagnostics.backend if diagnostics is not None else None,
-                       seconds=seconds, setup_seconds=setup_seconds,
-                       mcells_per_second=math.prod(r.shape)*completed/max(seconds, 1e-9)/1e6,
-                       field_peak=float(np.max(np.abs(e))), slice_index=plane_index,
-                       material_sampling=r.material_sampling, boundaries=r.boundaries.model_dump(),
-                       units='geometry: um; time: s; E/H: reduced fields; Bloch phase: rad', warnings=[])
+        summary = run_summary(p, run, e, completed, termination, seconds, setup_seconds,
+                              cuda_graph=graph is not None, cuda_graph_steps=width, cuda_graph_replays=replays,
+                              diagnostics=decision.history, diagnostic_seconds=diagnostic_seconds,
+                              diagnostic_backend=diagnostics.backend if diagnostics is not None else None,
+                              slice_index=plane_index)
         planes = [m.result() for m in run.planes]
         return Result(p, summary, np.array(frames), np.array(frame_steps), eps_plane,
                       run.traces[:completed].cpu().numpy(), np.arange(1, completed+1)*g.time_step, e, h, planes)
+
+
+class DisplaySlice:
+    """The displayed field plane of a run (solver.py:747-756): axis, index and a <=256x256 subsample."""
+
+    def __init__(self, project):
+        r = project.region
+        self.axis = 'xyz'.index(r.slice_axis)
+        self.index = index_at(tuple(r.slice_position if i == self.axis else 0 for i in range(3)), r, r.field)[self.axis]
+        self.component = 'xyz'.index(r.field[1].lower())
+        self.electric = r.field[0] == 'E'
+
+    def plane(self, a):
+        sl = [slice(None)]*3
+        sl[self.axis] = self.index
+        data = a[tuple(sl)]
+        return data[::max(1, math.ceil(data.shape[0]/256)), ::max(1, math.ceil(data.shape[1]/256))]
+
+    def frame(self, grid):
+        return self.plane((grid.E if self.electric else grid.H)[..., self.component]).cpu().numpy().copy()
+
+    def epsilon(self, eps):
+        return np.array(self.plane(eps[..., self.component] if eps.ndim == 4 else eps))
+
+
+def run_summary(project, run, e, completed, termination, seconds, setup_seconds, **extra):
+    """Result.summary of one run (the keys of the reference's solver.py summary)."""
+    r = project.region
+    props = torch.cuda.get_device_properties(run.grid.device)
+    summary = dict(shape=list(r.shape), cells=math.prod(r.shape), dt_fs=r.time_step*1e15,
+                   duration_fs=r.time_step*r.steps*1e15, backend='cuda', precision=r.precision,
+                   gpu=props.name, gpu_arch=getattr(props, 'gcnArchName', None),
+                   cuda_kernel=run.kernel, cuda_monitor_kernel='hip', engine='torchfdtd_amd HIP gfx950',
+                   steps=completed, requested_steps=r.steps, cancelled=termination == 'cancelled',
+                   termination_reason=termination, auto_shutoff=termination == 'decayed',
+                   seconds=seconds, setup_seconds=setup_seconds,
+                   mcells_per_second=math.prod(r.shape)*completed/max(seconds, 1e-9)/1e6,
+                   field_peak=float(np.max(np.abs(e))),
+                   material_sampling=r.material_sampling, boundaries=r.boundaries.model_dump(),
+                   units='geometry: um; time: s; E/H: reduced fields; Bloch phase: rad', warnings=[])
+    summary.update(extra)
+    return summary

@@ -8,6 +8,12 @@
 * ``cube_512``: 512^3 cells, 16-cell CPML on all faces, 500 steps, per-cell
   eps = 1 + 11 s with s = [Gaussian(sigma = 3 cells) * U(0,1) noise > 0.5] (periodic
   filter, fixed seed), Ez dipole at the centre.
+* ``uneven_batch``: 64 3D grids with sizes drawn uniformly from 64x64x32 to 192x160x64
+  cells at 0.04 um, 10-cell CPML, 2000 steps; each has an Ez dipole (1.45-1.65 um
+  broadband pulse around 1.55 um) 12 cells inside its low-x edge, a 16-wavelength
+  float64 DFT plane 12 cells inside its high-x edge over the CPML-free cross-section,
+  and a 32x32x8-cell binary design block at its centre (eps 2.085 or 12.11 from
+  seeded U(0,1) noise, conic filter of radius 0.1 um, threshold 0.5).
 """
 from __future__ import annotations
 
@@ -15,9 +21,11 @@ import math
 
 import numpy as np
 
-from .models import Material, Monitor, Project, Region, Source, SpectrumSettings, Structure
+from .models import FieldMonitor, Material, Monitor, Project, Region, Source, SpectrumSettings, Structure
 
 CUBE_SEED = 20260512
+BATCH_SEED = 20260513
+EPS_LOW, EPS_HIGH = 2.085, 12.11
 
 
 def waveguide_2d(steps=1000, backend='cuda'):
@@ -69,3 +77,71 @@ def cube_inverse_permittivity(mask):
     import torch
     inv12 = float(np.float32(1)/np.float32(12))
     return torch.where(mask, torch.tensor(inv12, device=mask.device), torch.tensor(1., device=mask.device)).float()
+
+
+def conic_filter(density, radius_cells):
+    """Normalised conic filter (1 - d/r)+ over the x-y plane of a 2D density, edge-padded."""
+    reach = int(math.floor(radius_cells))
+    offsets = np.arange(-reach, reach+1)
+    weight = np.maximum(0., 1.-np.hypot(offsets[:, None], offsets[None, :])/radius_cells)
+    weight /= weight.sum()
+    padded = np.pad(density, reach, mode='edge')
+    out = np.zeros_like(density, dtype=np.float64)
+    for i, dx in enumerate(offsets):
+        for j, dy in enumerate(offsets):
+            if weight[i, j]:
+                out += weight[i, j]*padded[reach+dx:reach+dx+density.shape[0], reach+dy:reach+dy+density.shape[1]]
+    return out
+
+
+def binary_design(rng, shape=(32, 32), radius_um=.1, mesh=.04):
+    """0/1 design: U(0,1) noise, conic filter of `radius_um`, threshold 0.5."""
+    return conic_filter(rng.random(shape), radius_um/mesh) > .5
+
+
+def uneven_grid(shape, steps=2000, mesh=.04, pml=10, frequencies=16, precision='float32', backend='cuda', name='grid',
+                inset_cells=12):
+    """One grid of the uneven batch: Ez dipole near low x, float64 DFT plane near high x."""
+    size = tuple(n*mesh for n in shape)
+    inset = inset_cells*mesh
+    return Project(name=name,
+                   region=Region(dimension='3d', size=size, mesh=mesh, steps=steps, pml_cells=pml,
+                                 background_index=math.sqrt(EPS_LOW), backend=backend, precision=precision,
+                                 snapshot_interval=steps),
+                   sources=[Source(id='src', center=(-size[0]/2+inset+mesh/2, 0, 0), component='Ez', pulse='broadband',
+                                   time_definition='wavelength', wavelength_start=1.45, wavelength_stop=1.65)],
+                   monitors=[FieldMonitor(id='plane', center=(size[0]/2-inset-mesh/2, 0, 0), normal='x',
+                                          size=(0, size[1]-2*pml*mesh, size[2]-2*pml*mesh), dft_precision='float64',
+                                          spectrum=SpectrumSettings(sampling='wavelength', frequency_points=frequencies,
+                                                                    wavelength_start=1.45, wavelength_stop=1.65,
+                                                                    apodization='none'))])
+
+
+def design_epsilon(project, design, depth=8):
+    """Per-cell eps of a grid with the binary x-y `design` extruded over `depth` z cells at its centre."""
+    from .planning import voxelize
+    eps = voxelize(project)
+    nx, ny, nz = eps.shape
+    dx, dy = design.shape
+    x0, y0, z0 = (nx-dx)//2, (ny-dy)//2, (nz-depth)//2
+    block = np.where(design, EPS_HIGH, EPS_LOW).astype(eps.dtype)
+    eps[x0:x0+dx, y0:y0+dy, z0:z0+depth] = block[:, :, None]
+    return eps
+
+
+def uneven_shapes(count=64, seed=BATCH_SEED, low=(64, 64, 32), high=(192, 160, 64)):
+    rng = np.random.default_rng(seed)
+    return [tuple(int(rng.integers(lo, hi+1)) for lo, hi in zip(low, high)) for _ in range(count)]
+
+
+def uneven_batch(count=64, steps=2000, seed=BATCH_SEED, frequencies=16, shapes=None, precision='float32'):
+    """BatchCases of the uneven batch (sizes, designs and ids fixed by `seed`)."""
+    from .batch import BatchCase
+    shapes = uneven_shapes(count, seed) if shapes is None else shapes
+    rng = np.random.default_rng(seed+1)
+    cases = []
+    for i, shape in enumerate(shapes):
+        p = uneven_grid(shape, steps, frequencies=frequencies, precision=precision, name=f'uneven {i} {shape}')
+        design = binary_design(rng)
+        cases.append(BatchCase(f'grid-{i:02d}', p, dict(shape=list(shape)), epsilon=design_epsilon(p, design)))
+    return cases
