@@ -22,8 +22,6 @@ Pinning (what checks each part, besides restating the reference's source):
 * ``monitors`` plane DFT: a direct time-domain sum of the sampled trace
   (definition) and trilinear interpolation of linear fields (exact for affine
   fields).
-* ``adjoint``: the dot-product identity <A x, y> = <x, A^T y> on random states and
-  central finite differences of the FP64 forward.
 
 Functions whose header says "parity unpinned" have none of the above.
 """
