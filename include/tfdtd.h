@@ -230,6 +230,68 @@ int tfdtd_norm_pack(const tfdtd_norm_grid* grids, int32_t count, void* host_tabl
 int tfdtd_state_norm(const void* device_table, int32_t count, int64_t tasks, int32_t dtype, double* partial, double* out,
                      void* stream);
 
+/* Forward record for the adjoint: curl H of the E update (backward differences, periodic wrap
+ * included) on the design box cells [box_lo, box_hi), written to
+ * record[counter % modulus][box cell][3] -- read before the step's E update.  No reference
+ * counterpart: the reference recomputes this curl from replayed states
+ * (torchfdtd/differentiable.py:508-509, torchfdtd/cuda_adjoint.py:108-216). */
+typedef struct tfdtd_curl_record {
+    int32_t dtype;
+    int32_t nx, ny, nz;
+    const void* H;
+    int32_t periodic_mask;
+    int32_t box_lo[3], box_hi[3];
+    void* record;
+    int64_t modulus;
+    const int64_t* counter;
+} tfdtd_curl_record;
+
+int tfdtd_record_curl(const tfdtd_curl_record* record, void* stream);
+
+/* One step of the discrete adjoint, backwards (torchfdtd/differentiable.py:496-519 and
+ * torchfdtd/cuda_adjoint.py:218-234, FusedAdjointCUDA.step): for t = *counter,
+ *   e_bar/h_bar[target k] += sum of seeds[t][seed_columns[seed_offsets[k] .. seed_offsets[k+1])]
+ *   e_bar += curl_T(forward)(-(S inv_mu) h_bar)            (H-family memory adjoints e.psi/h.psi)
+ *   gradient[box cell] += -S e_bar . curl_H(t) inv_eps^2   (curl from record slot t % record_modulus)
+ *   h_bar += curl_T(backward)((S inv_eps) e_bar)
+ *   *counter = t - 1.
+ * The memory adjoints use the forward psi layouts of tfdtd_march_family: psi_in holds the
+ * adjoint after the step and psi_out receives the adjoint before it (separate buffers, which
+ * the caller alternates); coef is the forward coefficient block.  seed_targets are
+ * int64[nseed][2] = (0 E | 1 H, element index).  The gradient is double[box cells][stride]
+ * (stride 1 sums the components).  With record == NULL no gradient is formed; a step with t < 0
+ * adds neither seeds nor gradient. */
+typedef struct tfdtd_adjoint {
+    int32_t dtype;
+    int32_t nx, ny, nz;
+    void* e_bar;
+    void* h_bar;
+    const void* inverse_permittivity;
+    int32_t eps_stride;
+    const void* inverse_permeability;   /* one value */
+    double courant;
+    int32_t periodic_mask;
+    int32_t pec_upper_mask;
+    double pec_upper_factor[3];
+    tfdtd_march_family e;
+    tfdtd_march_family h;
+    int32_t nseed;
+    const int64_t* seed_targets;
+    const int32_t* seed_offsets;
+    const int32_t* seed_columns;
+    const void* seeds;                  /* real[seed_rows][seed_width] */
+    int32_t seed_width;
+    int64_t seed_rows;                  /* steps t outside [0, seed_rows) add no seed */
+    int32_t box_lo[3], box_hi[3];
+    const void* record;
+    int64_t record_modulus;
+    double* gradient;
+    int32_t gradient_stride;
+    int64_t* counter;
+} tfdtd_adjoint;
+
+int tfdtd_adjoint_step(const tfdtd_adjoint* step, void* stream);
+
 /* Copies the calling thread's last error message; returns its length. */
 size_t tfdtd_last_error(char* buffer, size_t size);
 

@@ -63,6 +63,11 @@ int main(void) {
   F(tfdtd_march_family, face); F(tfdtd_march_face, psi); F(tfdtd_march_face, coef);
   printf("tfdtd_norm_grid %zu\n", sizeof(tfdtd_norm_grid));
   F(tfdtd_norm_grid, eps_stride); F(tfdtd_norm_grid, widths); F(tfdtd_norm_grid, psi); F(tfdtd_norm_grid, psi_count);
+  printf("tfdtd_curl_record %zu\n", sizeof(tfdtd_curl_record));
+  F(tfdtd_curl_record, box_lo); F(tfdtd_curl_record, record); F(tfdtd_curl_record, modulus); F(tfdtd_curl_record, counter);
+  printf("tfdtd_adjoint %zu\n", sizeof(tfdtd_adjoint));
+  F(tfdtd_adjoint, courant); F(tfdtd_adjoint, e); F(tfdtd_adjoint, h); F(tfdtd_adjoint, nseed); F(tfdtd_adjoint, seeds); F(tfdtd_adjoint, seed_rows);
+  F(tfdtd_adjoint, box_lo); F(tfdtd_adjoint, record_modulus); F(tfdtd_adjoint, gradient_stride); F(tfdtd_adjoint, counter);
   return 0;
 }
 '''
@@ -76,7 +81,8 @@ def test_ctypes_mirror_matches_the_c_layout(tmp_path):
     lines = dict(line.rsplit(' ', 1) for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split('\n') if line)
     mirror = {'tfdtd_half_step': N.HalfStep, 'tfdtd_plane': N.Plane, 'tfdtd_observe': N.Observe,
               'tfdtd_cpml_segment': N.CpmlSegment, 'tfdtd_source': N.SourceBox, 'tfdtd_march': N.March,
-              'tfdtd_march_face': N.MarchFace, 'tfdtd_march_family': N.MarchFamily, 'tfdtd_norm_grid': N.NormGrid}
+              'tfdtd_march_face': N.MarchFace, 'tfdtd_march_family': N.MarchFamily, 'tfdtd_norm_grid': N.NormGrid,
+              'tfdtd_curl_record': N.CurlRecord, 'tfdtd_adjoint': N.Adjoint}
     for key, value in lines.items():
         if '.' in key:
             struct, member = key.split('.')

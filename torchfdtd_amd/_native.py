@@ -103,6 +103,20 @@ class NormGrid(ctypes.Structure):
                 ('inverse_permittivity', vp), ('eps_stride', i32), ('widths', vp*3), ('psi', vp*2), ('psi_count', i64*2)]
 
 
+class CurlRecord(ctypes.Structure):
+    _fields_ = [('dtype', i32), ('nx', i32), ('ny', i32), ('nz', i32), ('H', vp), ('periodic_mask', i32),
+                ('box_lo', i32*3), ('box_hi', i32*3), ('record', vp), ('modulus', i64), ('counter', vp)]
+
+
+class Adjoint(ctypes.Structure):
+    _fields_ = [('dtype', i32), ('nx', i32), ('ny', i32), ('nz', i32), ('e_bar', vp), ('h_bar', vp),
+                ('inverse_permittivity', vp), ('eps_stride', i32), ('inverse_permeability', vp), ('courant', ctypes.c_double),
+                ('periodic_mask', i32), ('pec_upper_mask', i32), ('pec_upper_factor', ctypes.c_double*3),
+                ('e', MarchFamily), ('h', MarchFamily), ('nseed', i32), ('seed_targets', vp), ('seed_offsets', vp),
+                ('seed_columns', vp), ('seeds', vp), ('seed_width', i32), ('seed_rows', i64), ('box_lo', i32*3), ('box_hi', i32*3),
+                ('record', vp), ('record_modulus', i64), ('gradient', vp), ('gradient_stride', i32), ('counter', vp)]
+
+
 _lib = None
 
 PROTOTYPES = {
@@ -120,6 +134,8 @@ PROTOTYPES = {
     'tfdtd_norm_table_bytes': (ctypes.c_size_t, [ctypes.POINTER(NormGrid), i32]),
     'tfdtd_norm_pack': (i32, [ctypes.POINTER(NormGrid), i32, vp, ctypes.c_size_t]),
     'tfdtd_state_norm': (i32, [vp, i32, i64, i32, vp, vp, vp]),
+    'tfdtd_record_curl': (i32, [ctypes.POINTER(CurlRecord), vp]),
+    'tfdtd_adjoint_step': (i32, [ctypes.POINTER(Adjoint), vp]),
     'tfdtd_abi_version': (i32, []),
     'tfdtd_last_error': (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
     'tfdtd_yee_half_step': (i32, [ctypes.POINTER(HalfStep), vp]),

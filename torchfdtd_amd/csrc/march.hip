@@ -28,22 +28,9 @@
 
 #include "../../include/tfdtd.h"
 #include "common.h"
+#include "cpml_faces.h"
 
 namespace {
-
-struct Face {          // one face's slab along an axis: rows [lo, hi), shared by the axis's two terms
-    int lo, hi;        // no slab: lo == hi == 0
-    unsigned psi[2];   // element offsets of the two terms' psi arrays (curl order)
-    unsigned coef;     // b at coef, c at coef + w, inv_k at coef + 2w (w = hi - lo)
-};
-
-template <typename real>
-struct Family {
-    const real* psi_in;
-    real* psi_out;     // equals psi_in for the in-place H family
-    const real* coef;
-    Face face[3][2];   // [axis][low, high]
-};
 
 template <typename real>
 struct MSource {
@@ -82,20 +69,6 @@ struct March {
     int nchunks;
     int blocks[3];              // workgroups of the interior, z-face and boundary variants
 };
-
-struct Hit {           // slab membership of a coordinate: face -1 = none
-    int face, q, w;
-    unsigned psi0, psi1, coef;
-};
-
-template <typename real>
-__device__ __forceinline__ Hit find_hit(const Family<real>& F, int axis, int k) {
-    const Face& f0 = F.face[axis][0];
-    const Face& f1 = F.face[axis][1];
-    if (k >= f0.lo && k < f0.hi) return Hit{0, k - f0.lo, f0.hi - f0.lo, f0.psi[0], f0.psi[1], f0.coef};
-    if (k >= f1.lo && k < f1.hi) return Hit{1, k - f1.lo, f1.hi - f1.lo, f1.psi[0], f1.psi[1], f1.coef};
-    return Hit{-1, 0, 1, 0u, 0u, 0u};
-}
 
 // psi <- psi*b + c*d (stored when `write`), d <- d*inv_k + psi, for slab row q of term k.
 template <typename real>
@@ -534,34 +507,6 @@ __global__ TFDTD_TILE_BOUNDS void batch_tile_kernel(const March<real>* __restric
 }
 
 #undef TFDTD_TILE_BOUNDS
-
-template <typename real>
-int make_family(const tfdtd_march_family& in, bool split, const int dims[3], Family<real>* out) {
-    std::memset(out, 0, sizeof *out);
-    if (!in.psi_in || !in.psi_out || !in.coef) return tfdtd_fail(TFDTD_EINVAL, "march: null CPML buffer");
-    if (split == (in.psi_in == in.psi_out))
-        return tfdtd_fail(TFDTD_EINVAL, split ? "march: E psi needs separate input and output buffers" : "march: H psi updates in place");
-    out->psi_in = static_cast<const real*>(in.psi_in);
-    out->psi_out = static_cast<real*>(in.psi_out);
-    out->coef = static_cast<const real*>(in.coef);
-    for (int ax = 0; ax < 3; ++ax) {
-        for (int f = 0; f < 2; ++f) {
-            const tfdtd_march_face& face = in.face[ax][f];
-            if (face.lo == face.hi) continue;
-            if (face.lo < 0 || face.hi < face.lo || face.hi > dims[ax])
-                return tfdtd_fail(TFDTD_EINVAL, "march: face %d of axis %d has rows outside the axis", f, ax);
-            if (face.psi[0] < 0 || face.psi[1] < 0 || face.coef < 0 || face.psi[0] > 0xffffffffLL ||
-                face.psi[1] > 0xffffffffLL || face.coef > 0xffffffffLL)
-                return tfdtd_fail(TFDTD_EINVAL, "march: CPML offsets must be non-negative 32-bit element offsets");
-            out->face[ax][f] = Face{face.lo, face.hi, {(unsigned)face.psi[0], (unsigned)face.psi[1]}, (unsigned)face.coef};
-        }
-        const Face& lo = out->face[ax][0];
-        const Face& hi = out->face[ax][1];
-        if (lo.hi > lo.lo && hi.hi > hi.lo && lo.hi > hi.lo && hi.hi > lo.lo)
-            return tfdtd_fail(TFDTD_EINVAL, "march: the two faces of axis %d overlap", ax);
-    }
-    return TFDTD_OK;
-}
 
 template <typename real>
 int make_march(const tfdtd_march* d, March<real>* a) {

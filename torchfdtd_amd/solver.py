@@ -109,6 +109,9 @@ def _json_default(value):
 
 
 def _check_epsilon(eps, region):
+    if isinstance(eps, torch.Tensor):
+        eps = eps.detach().cpu().numpy()
+    eps = np.asarray(eps)
     shape = tuple(region.shape)
     if eps.shape not in (shape, shape+(3,)):
         raise ValueError(f'epsilon must have shape {shape} or {shape+(3,)}, not {eps.shape}')
@@ -126,7 +129,7 @@ class Stepper:
         self.steps = r.steps if steps is None else steps
         self.kernel = kernel or resolve_cuda_kernel(r)
         # An explicit relative permittivity (a design already mapped to the grid) replaces the voxelized structures.
-        self.eps = voxelize(p) if epsilon is None else _check_epsilon(np.asarray(epsilon), r)
+        self.eps = voxelize(p) if epsilon is None else _check_epsilon(epsilon, r)
         self.grid = g = YeeGrid(r)
         g.set_permittivity(self.eps)
         self.lib = N.library()
@@ -152,6 +155,12 @@ class Stepper:
         if self.one_pass and not supported:
             raise ValueError('the one-pass kernel needs a 3D grid without periodic axes and at most 8 sources per family')
         self.tile_y, self.chunk_x = tile_y, chunk_x
+        self._describe()
+
+    def set_trace_points(self, points):
+        """Replace the recorded traces by `points`: (magnetic, flat element index) pairs."""
+        self.trace_points = [(bool(m), int(flat)) for m, flat in points]
+        self.traces = torch.zeros((self.steps, len(self.trace_points)), device=self.grid.device, dtype=self.grid.dtype)
         self._describe()
 
     def _trace_table(self, E, H):
