@@ -292,6 +292,32 @@ typedef struct tfdtd_adjoint {
 
 int tfdtd_adjoint_step(const tfdtd_adjoint* step, void* stream);
 
+/* BF16 implicit-GEMM 3D convolution on MFMA with FP32 accumulation (the learned screening
+ * model of the inverse-design loop; no counterpart in the reference, which has no neural
+ * surrogate).  Activations are channels-last bf16 [n][d][h][w][c]; a kernel of ks^3 taps
+ * (ks 1 or 3), stride 1 or 2, zero padding `pad`.  Forward weights are bf16 [co][kp] with
+ * k = ((kd*ks + kh)*ks + kw)*ci + c, zero beyond ks^3*ci up to kp (a multiple of 32). */
+typedef struct tfdtd_conv {
+    int32_t n, di, hi, wi, ci;
+    int32_t co;
+    int32_t ks, stride, pad;
+    int32_t kp;
+} tfdtd_conv;
+
+int tfdtd_conv_output(const tfdtd_conv* conv, int32_t* dout, int32_t* hout, int32_t* wout);
+/* pre[m][co] = sum_k im2col(x)[m][k] W[co][k] + bias[co] (FP32); act = bf16(GELU(pre)) or bf16(pre). */
+int tfdtd_conv_forward(const tfdtd_conv* conv, const void* x, const void* weight, const float* bias, int32_t activation,
+                       float* pre, void* act, void* stream);
+/* dx[m_in][ci] (FP32) from dy bf16 [m_out][co] and weight_t bf16 [ci][ks^3*co] (k = tap*co + c). */
+int tfdtd_conv_dgrad(const tfdtd_conv* conv, const void* dy, const void* weight_t, float* dx, void* stream);
+/* dw[co][kp] (FP32) from dy_t bf16 [co][m_out] and x; K split over `splits` workgroup layers into
+ * partial float[splits][co][kp], then summed in split order. */
+int tfdtd_conv_wgrad(const tfdtd_conv* conv, const void* dy_t, const void* x, int32_t splits, float* partial, float* dw,
+                     void* stream);
+/* dpre = dy * GELU'(pre) (activation 1) or dy (0), in FP32 and bf16. */
+int tfdtd_gelu_backward(const float* dy, const float* pre, int64_t count, int32_t activation, float* dpre, void* dpre16,
+                        void* stream);
+
 /* Copies the calling thread's last error message; returns its length. */
 size_t tfdtd_last_error(char* buffer, size_t size);
 

@@ -67,6 +67,8 @@ int main(void) {
   F(tfdtd_curl_record, box_lo); F(tfdtd_curl_record, record); F(tfdtd_curl_record, modulus); F(tfdtd_curl_record, counter);
   printf("tfdtd_adjoint %zu\n", sizeof(tfdtd_adjoint));
   F(tfdtd_adjoint, courant); F(tfdtd_adjoint, e); F(tfdtd_adjoint, h); F(tfdtd_adjoint, nseed); F(tfdtd_adjoint, seeds); F(tfdtd_adjoint, seed_rows);
+  printf("tfdtd_conv %zu\n", sizeof(tfdtd_conv));
+  F(tfdtd_conv, co); F(tfdtd_conv, kp);
   F(tfdtd_adjoint, box_lo); F(tfdtd_adjoint, record_modulus); F(tfdtd_adjoint, gradient_stride); F(tfdtd_adjoint, counter);
   return 0;
 }
@@ -82,7 +84,8 @@ def test_ctypes_mirror_matches_the_c_layout(tmp_path):
     mirror = {'tfdtd_half_step': N.HalfStep, 'tfdtd_plane': N.Plane, 'tfdtd_observe': N.Observe,
               'tfdtd_cpml_segment': N.CpmlSegment, 'tfdtd_source': N.SourceBox, 'tfdtd_march': N.March,
               'tfdtd_march_face': N.MarchFace, 'tfdtd_march_family': N.MarchFamily, 'tfdtd_norm_grid': N.NormGrid,
-              'tfdtd_curl_record': N.CurlRecord, 'tfdtd_adjoint': N.Adjoint}
+              'tfdtd_curl_record': N.CurlRecord, 'tfdtd_adjoint': N.Adjoint,
+              'tfdtd_conv': __import__('torchfdtd_amd.surrogate', fromlist=['ConvShape']).ConvShape}
     for key, value in lines.items():
         if '.' in key:
             struct, member = key.split('.')

@@ -136,6 +136,11 @@ PROTOTYPES = {
     'tfdtd_state_norm': (i32, [vp, i32, i64, i32, vp, vp, vp]),
     'tfdtd_record_curl': (i32, [ctypes.POINTER(CurlRecord), vp]),
     'tfdtd_adjoint_step': (i32, [ctypes.POINTER(Adjoint), vp]),
+    'tfdtd_conv_output': (i32, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]),
+    'tfdtd_conv_forward': (i32, [vp, vp, vp, vp, i32, vp, vp, vp]),
+    'tfdtd_conv_dgrad': (i32, [vp, vp, vp, vp, vp]),
+    'tfdtd_conv_wgrad': (i32, [vp, vp, vp, i32, vp, vp, vp]),
+    'tfdtd_gelu_backward': (i32, [vp, vp, i64, i32, vp, vp, vp]),
     'tfdtd_abi_version': (i32, []),
     'tfdtd_last_error': (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
     'tfdtd_yee_half_step': (i32, [ctypes.POINTER(HalfStep), vp]),

@@ -1,0 +1,144 @@
+"""Learned surrogate for inverse-design screening: a 3D conv encoder on BF16 MFMA kernels.
+
+A density rho (64 x 64 x 8 design cells) maps to the transmission at 32 wavelengths through
+four stride-2 3x3x3 convolutions (1 -> 32 -> 64 -> 128 -> 256 channels, GELU), a mean over
+the remaining positions and an MLP head (256 -> 256 GELU -> 32): about 1.24 M parameters.
+Every convolution and linear layer runs on the library's implicit-GEMM kernels
+(``tfdtd_conv_forward`` / ``tfdtd_conv_dgrad`` / ``tfdtd_conv_wgrad``, bf16 operands, FP32
+accumulation); weights are kept in FP32 (the optimiser's master copy) and rounded to bf16
+per call.  The reference has no neural surrogate: this is the learned screen of the
+inverse-design loop, with the FDTD solve as the judge (``screen``).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import torch
+from torch import nn
+
+from . import _native as N
+
+
+class ConvShape(ctypes.Structure):
+    _fields_ = [('n', ctypes.c_int32), ('di', ctypes.c_int32), ('hi', ctypes.c_int32), ('wi', ctypes.c_int32),
+                ('ci', ctypes.c_int32), ('co', ctypes.c_int32), ('ks', ctypes.c_int32), ('stride', ctypes.c_int32),
+                ('pad', ctypes.c_int32), ('kp', ctypes.c_int32)]
+
+
+def padded_k(ks, ci):
+    return (ks**3*ci+31)//32*32
+
+
+def conv_shape(x_shape, ci, co, ks, stride, pad):
+    n, d, h, w, c = x_shape
+    if c != ci:
+        raise ValueError(f'input has {c} channels, the layer expects {ci}')
+    s = ConvShape(n, d, h, w, ci, co, ks, stride, pad, padded_k(ks, ci))
+    out = (ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32())
+    N.check(N.library().tfdtd_conv_output(ctypes.byref(s), *(ctypes.byref(v) for v in out)), 'tfdtd_conv_output')
+    return s, tuple(v.value for v in out)
+
+
+def forward_weight(weight):
+    """FP32 [co][ks][ks][ks][ci] -> bf16 [co][kp] with zero padding."""
+    co = weight.shape[0]
+    k = weight[0].numel()
+    w16 = torch.zeros((co, padded_k(weight.shape[1], weight.shape[-1])), device=weight.device, dtype=torch.bfloat16)
+    w16[:, :k] = weight.reshape(co, k).to(torch.bfloat16)
+    return w16
+
+
+class _Conv(torch.autograd.Function):
+    """y = act(conv(x) + b), x and y FP32 channels-last; bf16 operands on MFMA inside."""
+
+    @staticmethod
+    def forward(ctx, x, weight, bias, stride, pad, activation):
+        lib = N.library()
+        co, ks, ci = weight.shape[0], weight.shape[1], weight.shape[-1]
+        shape, (do, ho, wo) = conv_shape(tuple(x.shape), ci, co, ks, stride, pad)
+        x16 = x.detach().to(torch.bfloat16).contiguous()
+        w16 = forward_weight(weight.detach())
+        m = x.shape[0]*do*ho*wo
+        pre = torch.empty((m, co), device=x.device, dtype=torch.float32)
+        act = torch.empty((m, co), device=x.device, dtype=torch.bfloat16)
+        b = bias.detach().float().contiguous()
+        N.check(lib.tfdtd_conv_forward(ctypes.byref(shape), x16.data_ptr(), w16.data_ptr(), ctypes.c_void_p(b.data_ptr()),
+                                       int(activation), ctypes.c_void_p(pre.data_ptr()), act.data_ptr(), N.stream_handle()),
+                'tfdtd_conv_forward')
+        ctx.save_for_backward(x16, w16, pre)
+        ctx.config = (shape, ks, ci, co, int(activation), tuple(x.shape))
+        return act.float().view(x.shape[0], do, ho, wo, co)
+
+    @staticmethod
+    def backward(ctx, grad):
+        lib = N.library()
+        x16, w16, pre = ctx.saved_tensors
+        shape, ks, ci, co, activation, x_shape = ctx.config
+        stream = N.stream_handle()
+        m = pre.shape[0]
+        dpre = torch.empty_like(pre)
+        dpre16 = torch.empty(pre.shape, device=pre.device, dtype=torch.bfloat16)
+        dy = grad.contiguous().view(m, co).float()
+        N.check(lib.tfdtd_gelu_backward(ctypes.c_void_p(dy.data_ptr()), ctypes.c_void_p(pre.data_ptr()), m*co, activation,
+                                        ctypes.c_void_p(dpre.data_ptr()), dpre16.data_ptr(), stream), 'tfdtd_gelu_backward')
+        grad_bias = dpre.sum(0)
+        dy_t = dpre16.t().contiguous()
+        splits = max(1, min(64, m//4096))
+        partial = torch.empty((splits, co, shape.kp), device=pre.device, dtype=torch.float32)
+        dw = torch.empty((co, shape.kp), device=pre.device, dtype=torch.float32)
+        N.check(lib.tfdtd_conv_wgrad(ctypes.byref(shape), dy_t.data_ptr(), x16.data_ptr(), splits,
+                                     ctypes.c_void_p(partial.data_ptr()), ctypes.c_void_p(dw.data_ptr()), stream), 'tfdtd_conv_wgrad')
+        taps = ks**3
+        grad_weight = dw[:, :taps*ci].reshape(co, ks, ks, ks, ci)
+        grad_x = None
+        if ctx.needs_input_grad[0]:
+            wt = w16[:, :taps*ci].reshape(co, taps, ci).permute(2, 1, 0).contiguous()
+            dx = torch.empty((x_shape[0]*x_shape[1]*x_shape[2]*x_shape[3], ci), device=pre.device, dtype=torch.float32)
+            N.check(lib.tfdtd_conv_dgrad(ctypes.byref(shape), dpre16.data_ptr(), wt.data_ptr(), ctypes.c_void_p(dx.data_ptr()),
+                                         stream), 'tfdtd_conv_dgrad')
+            grad_x = dx.view(x_shape)
+        return grad_x, grad_weight, grad_bias, None, None, None
+
+
+def conv3d(x, weight, bias, stride=1, pad=0, activation=True):
+    """Channels-last FP32 x [n,d,h,w,ci], weight [co,ks,ks,ks,ci] -> [n,do,ho,wo,co] (bf16 MFMA inside)."""
+    return _Conv.apply(x, weight, bias, stride, pad, activation)
+
+
+def linear(x, weight, bias, activation=True):
+    """[n, ci] -> [n, co] through the same kernels (a 1x1x1 convolution)."""
+    y = _Conv.apply(x.reshape(x.shape[0], 1, 1, 1, x.shape[1]), weight.reshape(weight.shape[0], 1, 1, 1, weight.shape[1]),
+                    bias, 1, 0, activation)
+    return y.reshape(x.shape[0], weight.shape[0])
+
+
+class SurrogateNet(nn.Module):
+    """rho [n, 64, 64, 8] -> transmission [n, 32]."""
+
+    def __init__(self, channels=(32, 64, 128, 256), hidden=256, outputs=32, seed=0):
+        super().__init__()
+        gen = torch.Generator().manual_seed(seed)
+        self.weights, self.biases = nn.ParameterList(), nn.ParameterList()
+        ci = 1
+        for co in channels:
+            bound = 1/math.sqrt(27*ci)
+            self.weights.append(nn.Parameter((torch.rand((co, 3, 3, 3, ci), generator=gen)*2-1)*bound))
+            self.biases.append(nn.Parameter((torch.rand(co, generator=gen)*2-1)*bound))
+            ci = co
+        self.head = nn.ParameterList()
+        for a, b in ((ci, hidden), (hidden, outputs)):
+            bound = 1/math.sqrt(a)
+            self.head.append(nn.Parameter((torch.rand((b, a), generator=gen)*2-1)*bound))
+            self.head.append(nn.Parameter((torch.rand(b, generator=gen)*2-1)*bound))
+
+    def forward(self, rho):
+        x = rho.unsqueeze(-1).float()
+        for w, b in zip(self.weights, self.biases):
+            x = conv3d(x, w, b, stride=2, pad=1, activation=True)
+        x = x.mean(dim=(1, 2, 3))
+        x = linear(x, self.head[0], self.head[1], activation=True)
+        return linear(x, self.head[2], self.head[3], activation=False)
+
+    def parameter_count(self):
+        return sum(p.numel() for p in self.parameters())
