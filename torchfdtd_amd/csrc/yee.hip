@@ -183,6 +183,18 @@ __global__ __launch_bounds__(256) void observe_kernel(const Observe<real> a) {
     if (threadIdx.x == 0) a.counter[0] = n + 1;
 }
 
+// Many traces: one trace per thread over the grid (the counter advances in a launch of its own).
+template <typename real>
+__global__ __launch_bounds__(256) void observe_wide_kernel(const Observe<real> a) {
+    const long long n = *a.counter;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= a.steps || k >= a.count) return;
+    const real* field = reinterpret_cast<const real*>(a.table[2 * k]);
+    a.traces[n * a.count + k] = field[a.table[2 * k + 1]];
+}
+
+__global__ void advance_counter_kernel(long long* counter) { counter[0] = counter[0] + 1; }
+
 template <typename real, typename spec>
 struct Plane {
     long long points;
@@ -327,8 +339,16 @@ int launch_observe(const tfdtd_observe* d, hipStream_t stream) {
         return fail(TFDTD_EINVAL, "observe: trace table and traces are required");
     Observe<real> a{d->count, reinterpret_cast<const long long*>(d->table), static_cast<real*>(d->traces),
                     (long long)d->steps, reinterpret_cast<long long*>(d->counter)};
-    hipLaunchKernelGGL((observe_kernel<real>), dim3(1), dim3(256), 0, stream, a);
-    return check_launch("observe");
+    constexpr int kOneBlock = 4096;
+    if (d->count <= kOneBlock) {
+        hipLaunchKernelGGL((observe_kernel<real>), dim3(1), dim3(256), 0, stream, a);
+        return check_launch("observe");
+    }
+    hipLaunchKernelGGL((observe_wide_kernel<real>), dim3((unsigned)((d->count + 255) / 256)), dim3(256), 0, stream, a);
+    const int status = check_launch("observe");
+    if (status) return status;
+    hipLaunchKernelGGL(advance_counter_kernel, dim3(1), dim3(1), 0, stream, a.counter);
+    return check_launch("observe counter");
 }
 
 template <typename real, typename spec>

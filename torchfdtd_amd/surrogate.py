@@ -14,6 +14,7 @@ from __future__ import annotations
 import ctypes
 import math
 
+import numpy as np
 import torch
 from torch import nn
 
@@ -142,3 +143,146 @@ class SurrogateNet(nn.Module):
 
     def parameter_count(self):
         return sum(p.numel() for p in self.parameters())
+
+    def figure_of_merit(self, transmission):
+        """The design objective evaluated on predicted spectra (set by the caller, e.g. a band filter)."""
+        if self.objective is None:
+            raise ValueError('set SurrogateNet.objective to a spectrum -> figure of merit function first')
+        return self.objective(transmission)
+
+    objective = None
+
+
+# ------------------------------------------------------------------ data from the batched solver and the screen
+
+def _plane_flux(plane):
+    """Signed flux through a z-normal plane per frequency, on the device (field_monitors.py:167-183)."""
+    comps = plane.components
+    v = plane.value
+    ex, ey = v[..., comps.index('Ex')], v[..., comps.index('Ey')]
+    hx, hy = v[..., comps.index('Hx')], v[..., comps.index('Hy')]
+    density = .5*(ex*hy.conj()-ey*hx.conj()).real
+    weights = torch.as_tensor(plane.plan['weights'], device=v.device, dtype=density.dtype)
+    return density@weights
+
+
+def slab_flux(densities, *, steps=2000, frequencies=32, cohort=64, launch='auto', width=50, progress=None):
+    """FDTD flux spectra (count, frequencies) of design-slab densities, in batched cohorts.
+
+    Every cohort runs all its grids together (``batch.Cohort``) in HIP graphs of `width` steps;
+    only the plane accumulators are reduced, on the device, to flux spectra."""
+    from .batch import Cohort, resolve_launch
+    from .workloads import design_slab, slab_epsilon
+    project = design_slab(steps, frequencies)
+    monitor = project.monitors[0]
+    monitor.record_fields = ('Ex', 'Ey', 'Hx', 'Hy')
+    monitor.record_poynting = ('z',)
+    out = []
+    with torch.no_grad():
+        for start in range(0, len(densities), cohort):
+            members = densities[start:start+cohort]
+            projects = [project]*len(members)
+            chosen = resolve_launch(projects, launch)
+            group = Cohort(projects, chosen, epsilons=[slab_epsilon(project, rho) for rho in members])
+            graphs = group.graphs(min(width, steps))
+            done = 0
+            while done < steps:
+                advance = graphs.width if steps-done >= graphs.width else 1
+                graphs.replay(advance)
+                done += advance
+            out.append(torch.stack([_plane_flux(run.planes[0]) for run in group.runs]).cpu())
+            del graphs, group
+            if progress:
+                progress(start+len(members))
+    return torch.cat(out).numpy()
+
+
+def slab_transmission(densities, reference_flux, **kwargs):
+    return slab_flux(densities, **kwargs)/reference_flux
+
+
+def train(net, densities, targets, *, epochs=20, batch=128, lr=3e-4, weight_decay=1e-2, seed=0, log=None):
+    """AdamW on mean-squared error; densities (n, 64, 64, 8), targets (n, 32), both on the device."""
+    gen = torch.Generator(device='cpu').manual_seed(seed)
+    opt = torch.optim.AdamW(net.parameters(), lr=lr, weight_decay=weight_decay)
+    n = densities.shape[0]
+    history = []
+    for epoch in range(epochs):
+        order = torch.randperm(n, generator=gen)
+        total = 0.
+        for i in range(0, n, batch):
+            idx = order[i:i+batch].to(densities.device)
+            opt.zero_grad(set_to_none=True)
+            loss = torch.nn.functional.mse_loss(net(densities[idx]), targets[idx])
+            loss.backward()
+            opt.step()
+            total += loss.item()*len(idx)
+        history.append(total/n)
+        if log:
+            log(epoch, history[-1])
+    return history
+
+
+@torch.no_grad()
+def predict(net, densities, batch=512):
+    return torch.cat([net(densities[i:i+batch]) for i in range(0, densities.shape[0], batch)])
+
+
+def band_figure_of_merit(transmission, wavelengths_um, band=(1.50, 1.60)):
+    """Mean transmission inside `band` minus mean outside: a band-pass figure of merit per design."""
+    t = torch.as_tensor(transmission)
+    w = torch.as_tensor(wavelengths_um, dtype=t.dtype)
+    inside = (w >= band[0]) & (w <= band[1])
+    return t[..., inside].mean(-1)-t[..., ~inside].mean(-1)
+
+
+def propose(rng, count, best=None, flip=.08):
+    """Candidate densities: fresh smoothed-threshold patterns, and (with a best design) mutations of it that
+    re-threshold the best pattern plus filtered noise."""
+    from .workloads import SLAB_DESIGN, binary_design, conic_filter
+    out = np.empty((count, *SLAB_DESIGN), dtype=np.float32)
+    for i in range(count):
+        if best is None or i % 2 == 0:
+            out[i] = binary_design(rng, SLAB_DESIGN[:2])[:, :, None]
+        else:
+            base = best[:, :, 0].astype(np.float64)
+            noisy = conic_filter(np.clip(base+flip*rng.normal(size=base.shape)*8, 0, 1), 2.5)
+            out[i] = (noisy > .5)[:, :, None]
+    return out
+
+
+def screen(net, evaluate, *, iterations=6, candidates=4096, verify=32, seed=0):
+    """Surrogate-screened design loop: each iteration proposes `candidates` designs, ranks them by the
+    surrogate's figure of merit and judges only the top `verify` with the FDTD `evaluate`
+    (densities -> figure of merit).  Returns the FDTD history [(solves, best FOM)] and the surrogate's
+    error on every judged design."""
+    rng = np.random.default_rng(seed)
+    device = next(net.parameters()).device
+    best, best_fom, solves, history, errors = None, -np.inf, 0, [], []
+    for _ in range(iterations):
+        pool = propose(rng, candidates, best)
+        predicted = net.figure_of_merit(predict(net, torch.as_tensor(pool, device=device))).cpu().numpy()
+        top = np.argsort(-predicted)[:verify]
+        judged = evaluate(pool[top])
+        solves += verify
+        errors.extend((judged-predicted[top]).tolist())
+        k = int(np.argmax(judged))
+        if judged[k] > best_fom:
+            best, best_fom = pool[top][k], float(judged[k])
+        history.append((solves, best_fom))
+    return history, np.asarray(errors), best
+
+
+def unscreened(evaluate, *, iterations=24, per_iteration=32, seed=0):
+    """The same loop without the surrogate: every proposed design is judged by FDTD."""
+    rng = np.random.default_rng(seed)
+    best, best_fom, solves, history = None, -np.inf, 0, []
+    for _ in range(iterations):
+        pool = propose(rng, per_iteration, best)
+        judged = evaluate(pool)
+        solves += per_iteration
+        k = int(np.argmax(judged))
+        if judged[k] > best_fom:
+            best, best_fom = pool[k], float(judged[k])
+        history.append((solves, best_fom))
+    return history, best

@@ -14,6 +14,10 @@
   float64 DFT plane 12 cells inside its high-x edge over the CPML-free cross-section,
   and a 32x32x8-cell binary design block at its centre (eps 2.085 or 12.11 from
   seeded U(0,1) noise, conic filter of radius 0.1 um, threshold 0.5).
+* ``design_slab``: the surrogate's data and judge.  A 64x64x8-cell density slab
+  (eps = 2.085 + 10.025 rho) in a 96x96x52-cell grid at 0.04 um with 10-cell CPML, lit by an
+  Ex plane source below it (1.45-1.65 um broadband) and measured by a 32-wavelength float64
+  DFT plane above it; the transmission is the plane's flux over that of the empty slab.
 """
 from __future__ import annotations
 
@@ -145,3 +149,47 @@ def uneven_batch(count=64, steps=2000, seed=BATCH_SEED, frequencies=16, shapes=N
         design = binary_design(rng)
         cases.append(BatchCase(f'grid-{i:02d}', p, dict(shape=list(shape)), epsilon=design_epsilon(p, design)))
     return cases
+
+
+SLAB_SHAPE = (96, 96, 52)
+SLAB_DESIGN = (64, 64, 8)
+
+
+def design_slab(steps=2000, frequencies=32, mesh=.04, precision='float32', name='design slab'):
+    """The design-slab project (no design: background eps 2.085 everywhere)."""
+    nx, ny, nz = SLAB_SHAPE
+    size = (nx*mesh, ny*mesh, nz*mesh)
+    pml = 10
+    inner = ((nx-2*pml)*mesh, (ny-2*pml)*mesh)
+    spectrum = SpectrumSettings(sampling='wavelength', frequency_points=frequencies, wavelength_start=1.45,
+                                wavelength_stop=1.65, apodization='none')
+    return Project(name=name,
+                   region=Region(dimension='3d', size=size, mesh=mesh, steps=steps, pml_cells=pml,
+                                 background_index=math.sqrt(EPS_LOW), backend='cuda', precision=precision,
+                                 snapshot_interval=steps),
+                   sources=[Source(id='plane', kind='plane', center=(0, 0, -size[2]/2+(pml+4.5)*mesh), size=(*inner, 0),
+                                   normal='z', component='Ex', pulse='broadband', time_definition='wavelength',
+                                   wavelength_start=1.45, wavelength_stop=1.65)],
+                   monitors=[FieldMonitor(id='out', center=(0, 0, size[2]/2-(pml+4.5)*mesh), normal='z',
+                                          size=(*inner, 0), dft_precision='float64', spectrum=spectrum)])
+
+
+def slab_epsilon(project, rho):
+    """Per-cell eps of the slab grid with density rho (64, 64, 8) at its centre: 2.085 + 10.025 rho."""
+    from .planning import voxelize
+    eps = voxelize(project).astype(np.float64)
+    rho = np.asarray(rho, dtype=np.float64)
+    nx, ny, nz = eps.shape
+    dx, dy, dz = rho.shape
+    x0, y0, z0 = (nx-dx)//2, (ny-dy)//2, (nz-dz)//2
+    eps[x0:x0+dx, y0:y0+dy, z0:z0+dz] = EPS_LOW+(EPS_HIGH-EPS_LOW)*rho
+    return eps.astype(np.float32 if project.region.precision == 'float32' else np.float64)
+
+
+def slab_densities(count, seed, radius_um=.1, mesh=.04):
+    """`count` binary slab densities: smoothed-threshold noise patterns (64x64) extruded over 8 cells."""
+    rng = np.random.default_rng(seed)
+    out = np.empty((count, *SLAB_DESIGN), dtype=np.float32)
+    for i in range(count):
+        out[i] = binary_design(rng, SLAB_DESIGN[:2], radius_um, mesh)[:, :, None]
+    return out
