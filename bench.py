@@ -67,6 +67,62 @@ def cpu_baseline(seconds_budget, mask):
                        f'{elapsed:.1f} s')
 
 
+def batched_line(steps, count, loop_grids, width=50):
+    """The uneven batch: 64 grids of 64x64x32 .. 192x160x64 cells, 2000 steps, autotuned launch,
+    against the per-grid loop of Simulation.run (both with graphs of `width` steps)."""
+    from torchfdtd_amd import workloads as W
+    from torchfdtd_amd.batch import Launch, run_tensor_batch, tune_tensor_batch
+    from torchfdtd_amd.solver import Simulation
+    cases = W.uneven_batch(count, steps)
+    cells = sum(math.prod(c.project.region.shape) for c in cases)
+    print(f'bench: uneven batch, {count} grids, {cells} cells', file=sys.stderr, flush=True)
+    tuning = tune_tensor_batch(cases, candidates=(Launch('split'), Launch('march', 8, 8), Launch('march', 8, 16),
+                                                  Launch('march', 16, 8)), repeats=2, trial_steps=100)
+    report = run_tensor_batch(cases, launch=tuning, cuda_graph_steps=width, keep_results=False)
+    batched = cells*steps/report.plan['loop_seconds']/1e9
+    print(f'bench: batched {batched:.2f} Gcell/s with {tuning.launch}; per-grid loop over {loop_grids} grids',
+          file=sys.stderr, flush=True)
+    loop_seconds, loop_cells = 0., 0
+    for c in cases[:loop_grids]:
+        result = Simulation(c.project, epsilon=c.epsilon).run(cuda_graph_steps=width)
+        loop_seconds += result.summary['seconds']
+        loop_cells += math.prod(c.project.region.shape)
+    loop = loop_cells*steps/loop_seconds/1e9
+    return dict(workload=f'{count} uneven 3D grids 64x64x32..192x160x64, 10-cell CPML, {steps} steps, Ez dipole, '
+                         '16-frequency float64 DFT plane, binary design block', cells=cells,
+                value=round(batched, 3), unit='Gcell-updates/s', loop_value=round(loop, 3), loop_grids=loop_grids,
+                speedup_vs_loop=round(batched/loop, 3), launch=tuning.launch.as_dict(),
+                tuning={k: round(v['median_seconds'], 5) for k, v in tuning.timings.items()},
+                loop_seconds=round(report.plan['loop_seconds'], 4))
+
+
+def adjoint_line(steps=2000):
+    """Discrete-adjoint gradient on a 200x160x60 grid, 50x50x6 density design, |DFT Ez|^2 on a plane at 8 wavelengths."""
+    import torch
+    from torchfdtd_amd.adjoint import AdjointOptions, DifferentiablePlaneSimulation
+    from torchfdtd_amd.density import cell_density_map
+    from torchfdtd_amd.workloads import adjoint_case
+    project, box, rho, frequency = adjoint_case(steps)
+    model = DifferentiablePlaneSimulation(project, AdjointOptions(design_box=box))
+    base = torch.full(tuple(project.region.shape), 2.085, device='cuda')
+    rho = torch.as_tensor(rho, device='cuda').requires_grad_(True)
+    started = time.perf_counter()
+    eps = cell_density_map(rho, base, box, background_epsilon=2.085, design_epsilon=12.11)
+    planes = model(eps, frequency)
+    objective = (planes['out'].fields[..., 2].abs()**2).sum()
+    objective.backward()
+    torch.cuda.synchronize()
+    total = time.perf_counter()-started
+    report = planes['out'].report
+    return dict(workload=f'200x160x60 cells at 0.04 um, {steps} steps, 12-cell CPML, 50x50x6 density design, '
+                         'objective sum |DFT Ez|^2 over a plane at 8 wavelengths 1.50-1.60 um',
+                seconds_per_gradient=round(total, 4), forward_seconds=round(report['forward_seconds'], 4),
+                backward_seconds=round(report['backward_seconds'], 4),
+                backward_over_forward=round(report['backward_seconds']/report['forward_seconds'], 3),
+                checkpoint_interval=report['checkpoint_interval'], record_bytes=report['record_bytes'],
+                observers=report['observers'], gradient_norm=float(rho.grad.norm()))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -76,12 +132,27 @@ def main():
     ap.add_argument('--graph-width', type=int, default=50)
     ap.add_argument('--cpu-seconds', type=float, default=15.)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--no-batched', action='store_true')
+    ap.add_argument('--no-adjoint', action='store_true')
+    ap.add_argument('--batch-grids', type=int, default=64)
+    ap.add_argument('--loop-grids', type=int, default=64)
+    ap.add_argument('--copy-gib', type=float, default=0,
+                    help='only copy a buffer of this many GiB three times (the calibration of HBM byte counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py measures one GPU: the fused Yee step runs the whole grid on one MI355X '
               f'(--gpus {args.gpus} is not run; see the sharding paragraph of the design notes).', file=sys.stderr)
         return 2
     import torch
+    if args.copy_gib:
+        count = int(args.copy_gib*2**30)//4
+        src = torch.ones(count, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_bytes_read_per_launch': count*4, 'copy_bytes_written_per_launch': count*4, 'launches': 3}))
+        return 0
     from torchfdtd_amd import _native as N
     from torchfdtd_amd.graphs import StepGraphs
     from torchfdtd_amd.solver import Stepper
@@ -164,6 +235,11 @@ def main():
                                    f'{region.steps - 8} steps of which {args.steps} timed',
                        'cells': cells, 'graph_steps': width, 'wall_s': round(wall, 3)},
             'roofline': roofline, 'cpu_baseline': baseline}
+    if not args.no_batched:
+        line['batched'] = batched_line(2000, args.batch_grids, args.loop_grids)
+    if not args.no_adjoint:
+        print('bench: adjoint gradient', file=sys.stderr, flush=True)
+        line['adjoint'] = adjoint_line()
     print(json.dumps(line), flush=True)
     return 0
 

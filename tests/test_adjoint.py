@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 
 
 def point_project(precision='float64', layout='pml', steps=36):
-    r = Region(dimension='3d', size=(1.4, 1.2, 1.0), mesh=.1, pml_cells=3, precision=precision, steps=steps, backend='cuda')
+    r = Region(dimension='3d', size=(1.4, 1.2, 1.2), mesh=.1, pml_cells=3, precision=precision, steps=steps, backend='cuda')
     if layout == 'mixed':
         r.boundaries.y_min = BoundaryFace(kind='periodic')
         r.boundaries.y_max = BoundaryFace(kind='periodic')
@@ -102,7 +102,7 @@ def test_checkpointed_replay_is_bitwise_identical(gpu):
 
 
 def plane_project(precision='float64', steps=60):
-    r = Region(dimension='3d', size=(1.6, 1.2, 1.0), mesh=.1, pml_cells=3, precision=precision, steps=steps, backend='cuda')
+    r = Region(dimension='3d', size=(1.6, 1.2, 1.2), mesh=.1, pml_cells=3, precision=precision, steps=steps, backend='cuda')
     spectrum = SpectrumSettings(sampling='wavelength', frequency_points=3, wavelength_start=1.5, wavelength_stop=1.6,
                                 apodization='none')
     return Project(region=r, sources=[Source(id='s', center=(-.3, 0, 0), component='Ez', pulse_length=3e-15,

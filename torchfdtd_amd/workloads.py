@@ -14,6 +14,10 @@
   float64 DFT plane 12 cells inside its high-x edge over the CPML-free cross-section,
   and a 32x32x8-cell binary design block at its centre (eps 2.085 or 12.11 from
   seeded U(0,1) noise, conic filter of radius 0.1 um, threshold 0.5).
+* ``adjoint_case``: 200x160x60 cells at 0.04 um, 2000 steps, 12-cell CPML, an Ez dipole
+  2.4 um left of centre, a 50x50x6-cell design box at the centre with seeded uniform density
+  rho (eps = 2.085 + 10.025 rho) and an x-normal float64 DFT plane 2.4 um right of centre over
+  the CPML-free cross-section, sampled at 8 wavelengths 1.50-1.60 um.
 * ``design_slab``: the surrogate's data and judge.  A 64x64x8-cell density slab
   (eps = 2.085 + 10.025 rho) in a 96x96x52-cell grid at 0.04 um with 10-cell CPML, lit by an
   Ex plane source below it (1.45-1.65 um broadband) and measured by a 32-wavelength float64
@@ -193,3 +197,28 @@ def slab_densities(count, seed, radius_um=.1, mesh=.04):
     for i in range(count):
         out[i] = binary_design(rng, SLAB_DESIGN[:2], radius_um, mesh)[:, :, None]
     return out
+
+
+ADJOINT_SEED = 20260514
+
+
+def adjoint_case(steps=2000, seed=ADJOINT_SEED, precision='float32'):
+    """(project, design box cells, rho (50, 50, 6), frequencies in Hz) of the adjoint benchmark."""
+    mesh, shape, pml = .04, (200, 160, 60), 12
+    size = tuple(n*mesh for n in shape)
+    spectrum = SpectrumSettings(sampling='wavelength', frequency_points=8, wavelength_start=1.5, wavelength_stop=1.6,
+                                apodization='none')
+    project = Project(name='adjoint 200x160x60',
+                      region=Region(dimension='3d', size=size, mesh=mesh, steps=steps, pml_cells=pml,
+                                    background_index=math.sqrt(EPS_LOW), backend='cuda', precision=precision,
+                                    snapshot_interval=steps),
+                      sources=[Source(id='src', center=(-2.4, 0, 0), component='Ez', pulse='broadband',
+                                      time_definition='wavelength', wavelength_start=1.5, wavelength_stop=1.6)],
+                      monitors=[FieldMonitor(id='out', center=(2.4+mesh/2, 0, 0), normal='x',
+                                             size=(0, size[1]-2*pml*mesh, size[2]-2*pml*mesh), dft_precision='float64',
+                                             spectrum=spectrum)])
+    design = (50, 50, 6)
+    box = tuple(((n-d)//2, (n-d)//2+d) for n, d in zip(shape, design))
+    rho = np.random.default_rng(seed).random(design).astype(np.float32 if precision == 'float32' else np.float64)
+    from .monitors import frequency_samples
+    return project, box, rho, frequency_samples(project.monitors[0].spectrum)
