@@ -103,6 +103,10 @@ typedef struct tfdtd_plane {
     void* value;                    /* spectral complex[nf][n][nc], accumulated */
     int64_t samples;                /* length of the windows; no accumulation at or beyond it */
     const int64_t* counter;
+    int32_t block;                  /* > 1: stage the samples of `block` steps and accumulate them together
+                                       (FP64 sums of the block's products); 0 or 1: every step */
+    void* stage;                    /* real[block][n][nc] */
+    int64_t* flushed;               /* step up to which tfdtd_dft_flush accumulated (zero it with the state) */
 } tfdtd_plane;
 
 /* CPML slabs of one family for the one-pass step.  Every psi array of the family lies
@@ -164,6 +168,10 @@ int tfdtd_yee_half_step(const tfdtd_half_step* step, void* stream);
  * absolute counter in FP64, eight-corner sampling and accumulation, one launch per plane. */
 int tfdtd_dft_update(const tfdtd_plane* planes, int32_t count, void* stream);
 
+/* Accumulate the staged samples of an unfinished block (blocked planes; nothing for block <= 1):
+ * call before reading `value` mid-window or after a run that stopped early. */
+int tfdtd_dft_flush(const tfdtd_plane* planes, int32_t count, void* stream);
+
 /* torchfdtd/solver.py:791-797: trace record (traces.index_copy_) and counter.add_(1). */
 int tfdtd_observe_step(const tfdtd_observe* observe, void* stream);
 
@@ -200,8 +208,9 @@ int tfdtd_batch_march(const void* device_table, const int32_t* device_prefix, in
  * torchfdtd/cuda_monitors.py:14-155): one launch for every plane of the batch. */
 size_t tfdtd_batch_planes_bytes(int32_t count, int32_t dtype, int32_t spectral);
 int tfdtd_batch_planes_pack(const tfdtd_plane* planes, int32_t count, void* host_table, size_t bytes,
-                            int32_t* max_blocks, int32_t* max_frequencies);
-int tfdtd_batch_dft_update(const void* device_table, int32_t count, int32_t max_blocks, int32_t max_frequencies,
+                            int32_t* max_blocks, int32_t* max_phases);
+/* max_phases: the largest frequencies x max(block, 1) of the packed planes (the launch's LDS). */
+int tfdtd_batch_dft_update(const void* device_table, int32_t count, int32_t max_blocks, int32_t max_phases,
                            int32_t dtype, int32_t spectral, void* stream);
 
 /* State norms of the run-control diagnostics (torchfdtd/run_control.py:127-159 and the fused

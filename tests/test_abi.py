@@ -54,6 +54,7 @@ int main(void) {
   printf("tfdtd_source %zu\n", sizeof(tfdtd_source));
   F(tfdtd_half_step, courant); F(tfdtd_half_step, seg); F(tfdtd_half_step, srcs); F(tfdtd_half_step, source_samples); F(tfdtd_half_step, counter);
   F(tfdtd_plane, points); F(tfdtd_plane, indices); F(tfdtd_plane, samples); F(tfdtd_plane, counter);
+  F(tfdtd_plane, block); F(tfdtd_plane, stage); F(tfdtd_plane, flushed);
   F(tfdtd_observe, steps); F(tfdtd_observe, counter);
   printf("tfdtd_march %zu\n", sizeof(tfdtd_march));
   printf("tfdtd_march_face %zu\n", sizeof(tfdtd_march_face));

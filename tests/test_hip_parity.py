@@ -237,3 +237,29 @@ def test_plane_dft_matches_oracle(gpu, dft_precision):
         assert scale > 0
         tol = 3e-6 if dft_precision == 'field' else 1e-12
         np.testing.assert_allclose(plane['fields']/scale, fields/scale, rtol=tol, atol=tol)
+
+
+def test_blocked_plane_flush_mid_window(gpu):
+    """A run stopped inside an accumulation block: the flush adds exactly the staged steps, once."""
+    from torchfdtd_amd import FieldMonitor, Project, Region, Source, SpectrumSettings
+    from torchfdtd_amd.solver import Stepper
+    project = Project(region=Region(dimension='3d', size=(2.4, 2.0, 2.0), mesh=.1, pml_cells=4, steps=80, backend='cuda'),
+                      sources=[Source(center=(-.5, 0, 0))],
+                      monitors=[FieldMonitor(id='p', normal='x', center=(.45, 0, 0), size=(0, .9, .8), dft_precision='float64',
+                                             spectrum=SpectrumSettings(sampling='frequency', frequency_points=5, apodization='none'))])
+    run = Stepper(project)
+    plane = run.planes[0]
+    assert plane.block > 1
+    run.advance(45)
+    first = plane.result()['fields']
+    again = plane.result()['fields']      # a second flush at the same step adds nothing
+    ref = simulate.run(project.model_dump(), steps=45)['planes'][0]
+    names = list(ref.components)
+    expected = ref.value[..., [names.index(c) for c in plane.monitor.record_fields]]
+    scale = np.abs(expected).max()
+    np.testing.assert_allclose(first/scale, expected/scale, rtol=1e-12, atol=1e-12)
+    np.testing.assert_array_equal(first, again)
+    run.advance(35)                       # the rest of the window: the block ending at step 64 starts at the flush
+    ref = simulate.run(project.model_dump())['planes'][0]
+    expected = ref.value[..., [names.index(c) for c in plane.monitor.record_fields]]
+    np.testing.assert_allclose(plane.result()['fields']/scale, expected/scale, rtol=1e-12, atol=1e-12)

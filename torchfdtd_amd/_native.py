@@ -76,7 +76,7 @@ class Plane(ctypes.Structure):
                 ('components', i32), ('electric', i32), ('corners', i32), ('time_downsample', i32),
                 ('E', vp), ('H', vp), ('indices', vp*6), ('weights', vp*6),
                 ('window_e', vp), ('window_h', vp), ('omega', vp), ('half', vp), ('value', vp),
-                ('samples', i64), ('counter', vp)]
+                ('samples', i64), ('counter', vp), ('block', i32), ('stage', vp), ('flushed', vp)]
 
 
 class MarchFace(ctypes.Structure):
@@ -145,6 +145,7 @@ PROTOTYPES = {
     'tfdtd_last_error': (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
     'tfdtd_yee_half_step': (i32, [ctypes.POINTER(HalfStep), vp]),
     'tfdtd_dft_update': (i32, [ctypes.POINTER(Plane), i32, vp]),
+    'tfdtd_dft_flush': (i32, [ctypes.POINTER(Plane), i32, vp]),
     'tfdtd_observe_step': (i32, [ctypes.POINTER(Observe), vp]),
     'tfdtd_step_fused': (i32, [ctypes.POINTER(HalfStep), ctypes.POINTER(HalfStep), ctypes.POINTER(Plane), i32,
                                ctypes.POINTER(Observe), i32, vp]),

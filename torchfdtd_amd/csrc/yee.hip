@@ -210,30 +210,28 @@ struct Plane {
     spec* value;        // interleaved re, im
     long long samples;
     const long long* counter;
+    int block;          // > 1: samples of `block` steps are staged and accumulated together
+    real* stage;        // [block][points * nc]
+    const long long* flushed;
 };
 
-// One thread per (point, component) sample; the block first evaluates the nf
-// absolute-time phases exp(i omega (n+1)) in FP64 (cuda_monitors.py:68-84) into LDS.
+// exp(i omega (step+1)) for E and the same times exp(i pi f dt) for H, from the absolute step in
+// FP64 (cuda_monitors.py:68-84): p[0..1] E (re, im), p[2..3] H.
 template <typename real, typename spec>
-__device__ __forceinline__ void dft_plane_block(const Plane<real, spec>& a, long long block) {
-    extern __shared__ unsigned char lds_raw[];
-    spec* phase = reinterpret_cast<spec*>(lds_raw);
-    const long long n = *a.counter;
-    if (n >= a.samples || n % a.downsample) return;
-    for (int f = threadIdx.x; f < a.nf; f += blockDim.x) {
-        double si, co;
-        sincos(a.omega[f] * (double)(n + 1), &si, &co);
-        const spec er = (spec)co, ei = (spec)si;
-        const spec hr = a.half[2 * f], hi = a.half[2 * f + 1];
-        phase[4 * f + 0] = er;
-        phase[4 * f + 1] = ei;
-        phase[4 * f + 2] = er * hr - ei * hi;
-        phase[4 * f + 3] = er * hi + ei * hr;
-    }
-    __syncthreads();
-    const long long samples = a.points * a.nc;
-    const long long i = block * blockDim.x + threadIdx.x;
-    if (i >= samples) return;
+__device__ __forceinline__ void phases(const Plane<real, spec>& a, long long step, int f, spec* p) {
+    double si, co;
+    sincos(a.omega[f] * (double)(step + 1), &si, &co);
+    const spec er = (spec)co, ei = (spec)si;
+    const spec hr = a.half[2 * f], hi = a.half[2 * f + 1];
+    p[0] = er;
+    p[1] = ei;
+    p[2] = er * hr - ei * hi;
+    p[3] = er * hi + ei * hr;
+}
+
+// Interpolated sample of (point, component) i at step n, times the window.
+template <typename real, typename spec>
+__device__ __forceinline__ real plane_sample(const Plane<real, spec>& a, long long i, long long n) {
     const int c = (int)(i % a.nc);
     const long long p = i / a.nc;
     const bool electric = c < a.ne;
@@ -242,14 +240,82 @@ __device__ __forceinline__ void dft_plane_block(const Plane<real, spec>& a, long
     const real* w = a.weights[c];
     real v = real(0);
     for (int j = 0; j < a.corners; ++j) v = v + field[idx[j * a.points + p]] * w[j * a.points + p];
-    v = v * (electric ? a.window_e : a.window_h)[n];
-    const spec sv = (spec)v;
-    const int o = electric ? 0 : 2;
+    return v * (electric ? a.window_e : a.window_h)[n];
+}
+
+// Accumulate the staged samples of steps [lo, hi) (one block at most): the phases of those steps go
+// to LDS, each thread sums its sample's products in FP64 and adds the sum to its accumulators.
+template <typename real, typename spec>
+__device__ __forceinline__ void flush_range(const Plane<real, spec>& a, long long i, long long lo, long long hi) {
+    extern __shared__ unsigned char lds_raw[];
+    spec* phase = reinterpret_cast<spec*>(lds_raw);
+    const int count = (int)(hi - lo);
+    for (int e = threadIdx.x; e < count * a.nf; e += blockDim.x) phases(a, lo + e / a.nf, e % a.nf, phase + 4 * e);
+    __syncthreads();
+    const long long samples = a.points * a.nc;
+    if (i >= samples) return;
+    const int o = (int)(i % a.nc) < a.ne ? 0 : 2;
+    for (int f = 0; f < a.nf; ++f) {
+        double sr = 0., si = 0.;
+        for (int j = 0; j < count; ++j) {
+            const long long step = lo + j;
+            if (step % a.downsample) continue;
+            const spec v = (spec)a.stage[(step % a.block) * samples + i];
+            const spec* p = phase + 4 * (j * a.nf + f) + o;
+            sr += (double)(p[0] * v);
+            si += (double)(p[1] * v);
+        }
+        spec* out = a.value + 2 * ((long long)f * samples + i);
+        out[0] = out[0] + (spec)sr;
+        out[1] = out[1] + (spec)si;
+    }
+}
+
+// One thread per (point, component) sample.  Per-step accumulation (block <= 1): the block first
+// evaluates the nf phases of this step into LDS.  Blocked accumulation: the sample is staged and
+// the block's last step (or the window's last sample) accumulates the staged block.
+template <typename real, typename spec>
+__device__ __forceinline__ void dft_plane_block(const Plane<real, spec>& a, long long block) {
+    extern __shared__ unsigned char lds_raw[];
+    spec* phase = reinterpret_cast<spec*>(lds_raw);
+    const long long n = *a.counter;
+    if (n >= a.samples) return;
+    const long long samples = a.points * a.nc;
+    const long long i = block * blockDim.x + threadIdx.x;
+    if (a.block > 1) {
+        if (n % a.downsample == 0 && i < samples) a.stage[(n % a.block) * samples + i] = plane_sample(a, i, n);
+        if ((n + 1) % a.block == 0 || n + 1 == a.samples) {
+            const long long first = n - n % a.block;
+            flush_range(a, i, first > *a.flushed ? first : *a.flushed, n + 1);
+        }
+        return;
+    }
+    if (n % a.downsample) return;
+    for (int f = threadIdx.x; f < a.nf; f += blockDim.x) phases(a, n, f, phase + 4 * f);
+    __syncthreads();
+    if (i >= samples) return;
+    const spec sv = (spec)plane_sample(a, i, n);
+    const int o = (int)(i % a.nc) < a.ne ? 0 : 2;
     for (int f = 0; f < a.nf; ++f) {
         spec* out = a.value + 2 * ((long long)f * samples + i);
         out[0] = out[0] + phase[4 * f + o] * sv;
         out[1] = out[1] + phase[4 * f + o + 1] * sv;
     }
+}
+
+// Accumulate the staged steps of the unfinished block: [max(flushed, first of block), counter).
+template <typename real, typename spec>
+__global__ __launch_bounds__(256) void dft_flush_kernel(const Plane<real, spec> a) {
+    const long long n = *a.counter < a.samples ? *a.counter : a.samples;
+    if (a.block <= 1 || n == a.samples) return;     // a finished window was accumulated by its last step
+    const long long first = n - n % a.block;
+    const long long lo = first > *a.flushed ? first : *a.flushed;
+    if (lo >= n) return;
+    flush_range(a, (long long)blockIdx.x * blockDim.x + threadIdx.x, lo, n);
+}
+
+__global__ void set_flushed_kernel(const long long* counter, long long samples, long long* flushed) {
+    flushed[0] = counter[0] < samples ? counter[0] : samples;
 }
 
 template <typename real, typename spec>
@@ -354,15 +420,34 @@ int launch_observe(const tfdtd_observe* d, hipStream_t stream) {
 template <typename real, typename spec>
 int make_plane(const tfdtd_plane* d, Plane<real, spec>* out);
 
+// LDS of one plane launch: the phases of one step, or of every step of a block.
+template <typename spec>
+size_t plane_lds(const tfdtd_plane* d) {
+    return 4 * sizeof(spec) * (size_t)d->frequencies * (size_t)(d->block > 1 ? d->block : 1);
+}
+
 template <typename real, typename spec>
 int launch_plane(const tfdtd_plane* d, hipStream_t stream) {
     Plane<real, spec> a;
     int status = make_plane(d, &a);
     if (status) return status;
-    const size_t lds = 4 * sizeof(spec) * (size_t)d->frequencies;
+    const size_t lds = plane_lds<spec>(d);
     const long long blocks = (d->points * d->components + 255) / 256;
     hipLaunchKernelGGL((dft_plane_kernel<real, spec>), dim3((unsigned)blocks), dim3(256), lds, stream, a);
     return check_launch("dft_plane");
+}
+
+template <typename real, typename spec>
+int launch_flush(const tfdtd_plane* d, hipStream_t stream) {
+    Plane<real, spec> a;
+    int status = make_plane(d, &a);
+    if (status) return status;
+    const long long blocks = (d->points * d->components + 255) / 256;
+    hipLaunchKernelGGL((dft_flush_kernel<real, spec>), dim3((unsigned)blocks), dim3(256), plane_lds<spec>(d), stream, a);
+    status = check_launch("dft_flush");
+    if (status) return status;
+    hipLaunchKernelGGL(set_flushed_kernel, dim3(1), dim3(1), 0, stream, a.counter, a.samples, const_cast<long long*>(a.flushed));
+    return check_launch("dft_flush counter");
 }
 
 template <typename real, typename spec>
@@ -373,8 +458,10 @@ int make_plane(const tfdtd_plane* d, Plane<real, spec>* out) {
         return fail(TFDTD_EINVAL, "plane: bad sizes");
     if (!d->E || !d->H || !d->window_e || !d->window_h || !d->omega || !d->half || !d->value || !d->counter)
         return fail(TFDTD_EINVAL, "plane: null pointer");
-    const size_t lds = 4 * sizeof(spec) * (size_t)d->frequencies;
-    if (lds > 64 * 1024) return fail(TFDTD_EINVAL, "plane: %d frequencies exceed one launch", d->frequencies);
+    if (d->block < 0) return fail(TFDTD_EINVAL, "plane: negative block");
+    if (plane_lds<spec>(d) > 64 * 1024)
+        return fail(TFDTD_EINVAL, "plane: %d frequencies x block %d exceed one launch", d->frequencies, d->block);
+    if (d->block > 1 && (!d->stage || !d->flushed)) return fail(TFDTD_EINVAL, "plane: a block needs stage and flushed buffers");
     std::memset(&a, 0, sizeof a);
     a.points = d->points; a.nf = d->frequencies; a.nc = d->components; a.ne = d->electric;
     a.corners = d->corners; a.downsample = d->time_downsample;
@@ -389,6 +476,9 @@ int make_plane(const tfdtd_plane* d, Plane<real, spec>* out) {
     a.counter = reinterpret_cast<const long long*>(d->counter);
     if (d->samples < 0) return fail(TFDTD_EINVAL, "plane: negative window length");
     a.samples = d->samples;
+    a.block = d->block > 1 ? d->block : 1;
+    a.stage = static_cast<real*>(d->stage);
+    a.flushed = reinterpret_cast<const long long*>(d->flushed);
     if ((d->points * d->components + 255) / 256 > 0x7fffffffLL) return fail(TFDTD_EINVAL, "plane: too many samples");
     return TFDTD_OK;
 }
@@ -442,17 +532,17 @@ int pack_half_steps(const tfdtd_half_step* steps, int count, void* host_table, s
 }
 
 template <typename real, typename spec>
-int pack_planes(const tfdtd_plane* planes, int count, void* host_table, size_t bytes, int32_t* max_blocks, int32_t* max_frequencies) {
-    if (count < 1 || !planes || !max_blocks || !max_frequencies) return fail(TFDTD_EINVAL, "batch planes: bad arguments");
+int pack_planes(const tfdtd_plane* planes, int count, void* host_table, size_t bytes, int32_t* max_blocks, int32_t* max_phases) {
+    if (count < 1 || !planes || !max_blocks || !max_phases) return fail(TFDTD_EINVAL, "batch planes: bad arguments");
     if (!host_table || bytes < sizeof(Plane<real, spec>) * (size_t)count) return fail(TFDTD_EINVAL, "batch planes: table buffer too small");
     Plane<real, spec>* table = static_cast<Plane<real, spec>*>(host_table);
     *max_blocks = 0;
-    *max_frequencies = 0;
+    *max_phases = 0;
     for (int k = 0; k < count; ++k) {
         const int status = make_plane(planes + k, table + k);
         if (status) return status;
         *max_blocks = std::max<int32_t>(*max_blocks, (int32_t)((planes[k].points * planes[k].components + 255) / 256));
-        *max_frequencies = std::max(*max_frequencies, planes[k].frequencies);
+        *max_phases = std::max(*max_phases, planes[k].frequencies * std::max(planes[k].block, 1));
     }
     return TFDTD_OK;
 }
@@ -501,29 +591,29 @@ size_t tfdtd_batch_planes_bytes(int32_t count, int32_t dtype, int32_t spectral) 
 }
 
 int tfdtd_batch_planes_pack(const tfdtd_plane* p, int32_t count, void* host_table, size_t bytes, int32_t* max_blocks,
-                            int32_t* max_frequencies) {
+                            int32_t* max_phases) {
     if (count < 1 || !p) return fail(TFDTD_EINVAL, "batch planes: at least one plane");
     for (int k = 1; k < count; ++k)
         if (p[k].dtype != p[0].dtype || p[k].spectral != p[0].spectral) return fail(TFDTD_EINVAL, "batch planes: mixed precision");
-    if (p[0].dtype == TFDTD_F32 && p[0].spectral == 0) return pack_planes<float, float>(p, count, host_table, bytes, max_blocks, max_frequencies);
-    if (p[0].dtype == TFDTD_F32 && p[0].spectral == 1) return pack_planes<float, double>(p, count, host_table, bytes, max_blocks, max_frequencies);
-    if (p[0].dtype == TFDTD_F64 && p[0].spectral == 1) return pack_planes<double, double>(p, count, host_table, bytes, max_blocks, max_frequencies);
+    if (p[0].dtype == TFDTD_F32 && p[0].spectral == 0) return pack_planes<float, float>(p, count, host_table, bytes, max_blocks, max_phases);
+    if (p[0].dtype == TFDTD_F32 && p[0].spectral == 1) return pack_planes<float, double>(p, count, host_table, bytes, max_blocks, max_phases);
+    if (p[0].dtype == TFDTD_F64 && p[0].spectral == 1) return pack_planes<double, double>(p, count, host_table, bytes, max_blocks, max_phases);
     return fail(TFDTD_EINVAL, "batch planes: unsupported dtype/spectral pair");
 }
 
-int tfdtd_batch_dft_update(const void* device_table, int32_t count, int32_t max_blocks, int32_t max_frequencies,
+int tfdtd_batch_dft_update(const void* device_table, int32_t count, int32_t max_blocks, int32_t max_phases,
                            int32_t dtype, int32_t spectral, void* stream) {
-    if (!device_table || count < 1 || max_blocks < 1 || max_frequencies < 1) return fail(TFDTD_EINVAL, "batch planes: bad launch arguments");
+    if (!device_table || count < 1 || max_blocks < 1 || max_phases < 1) return fail(TFDTD_EINVAL, "batch planes: bad launch arguments");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const dim3 grid((unsigned)max_blocks, (unsigned)count);
     if (dtype == TFDTD_F32 && spectral == 0) {
-        const size_t lds = 4 * sizeof(float) * (size_t)max_frequencies;
+        const size_t lds = 4 * sizeof(float) * (size_t)max_phases;
         hipLaunchKernelGGL((batch_dft_kernel<float, float>), grid, dim3(256), lds, s, static_cast<const Plane<float, float>*>(device_table));
     } else if (dtype == TFDTD_F32 && spectral == 1) {
-        const size_t lds = 4 * sizeof(double) * (size_t)max_frequencies;
+        const size_t lds = 4 * sizeof(double) * (size_t)max_phases;
         hipLaunchKernelGGL((batch_dft_kernel<float, double>), grid, dim3(256), lds, s, static_cast<const Plane<float, double>*>(device_table));
     } else if (dtype == TFDTD_F64 && spectral == 1) {
-        const size_t lds = 4 * sizeof(double) * (size_t)max_frequencies;
+        const size_t lds = 4 * sizeof(double) * (size_t)max_phases;
         hipLaunchKernelGGL((batch_dft_kernel<double, double>), grid, dim3(256), lds, s, static_cast<const Plane<double, double>*>(device_table));
     } else {
         return fail(TFDTD_EINVAL, "batch planes: unsupported dtype/spectral pair");
@@ -552,6 +642,22 @@ int tfdtd_dft_update(const tfdtd_plane* p, int32_t count, void* stream) {
 }
 
 int tfdtd_observe_step(const tfdtd_observe* d, void* stream) { return observe(d, static_cast<hipStream_t>(stream)); }
+
+int tfdtd_dft_flush(const tfdtd_plane* p, int32_t count, void* stream) {
+    if (count < 0 || (count > 0 && !p)) return fail(TFDTD_EINVAL, "bad plane table");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    for (int k = 0; k < count; ++k) {
+        const tfdtd_plane* d = p + k;
+        if (d->block <= 1) continue;
+        int status;
+        if (d->dtype == TFDTD_F32 && d->spectral == 0) status = launch_flush<float, float>(d, s);
+        else if (d->dtype == TFDTD_F32 && d->spectral == 1) status = launch_flush<float, double>(d, s);
+        else if (d->dtype == TFDTD_F64 && d->spectral == 1) status = launch_flush<double, double>(d, s);
+        else status = fail(TFDTD_EINVAL, "plane %d: unsupported dtype/spectral pair", k);
+        if (status) return status;
+    }
+    return TFDTD_OK;
+}
 
 int tfdtd_step_fused(const tfdtd_half_step* e_step, const tfdtd_half_step* h_step, const tfdtd_plane* p,
                      int32_t plane_count, const tfdtd_observe* obs, int32_t n_steps, void* stream) {

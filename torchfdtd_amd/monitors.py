@@ -184,6 +184,14 @@ class FrequencyPlane:
         mask = (np.arange(r.steps) % monitor.time_downsample == 0)*monitor.time_downsample
         self.windows = [torch.as_tensor(apodization_window(times+shift, monitor.spectrum)*r.time_step*mask, device=device, dtype=real)
                         for shift in (0, r.time_step/2)]
+        # Blocked accumulation: the samples of `block` steps are staged and their phase products summed
+        # in FP64 once per block, so the accumulators are read and written once per block, not per step.
+        spec_bytes = 8 if self.complex_dtype == torch.complex128 else 4
+        self.block = max(1, min(32, 65536//(4*spec_bytes*len(self.frequency))))
+        self.stage = torch.zeros((self.block, shape[1]*shape[2]), device=device, dtype=real)
+        self.flushed = torch.zeros(1, device=device, dtype=torch.long)
+        grid.memory_states.append(self.flushed)
+        self.counter = None
 
     def descriptor(self, counter, E=None, H=None):
         """tfdtd_plane sampling `E`/`H` (the grid's fields by default)."""
@@ -205,9 +213,20 @@ class FrequencyPlane:
         d.omega, d.half, d.value = self.omega.data_ptr(), self.half_phase.data_ptr(), self.value.data_ptr()
         d.samples = self.windows[0].numel()
         d.counter = counter.data_ptr()
+        d.block = self.block
+        d.stage, d.flushed = self.stage.data_ptr(), self.flushed.data_ptr()
+        self.counter = counter
         return d
 
+    def flush(self):
+        """Accumulate the staged steps of an unfinished block (before reading the accumulators)."""
+        if self.counter is None or self.block <= 1:
+            return
+        d = self.descriptor(self.counter)
+        N.check(N.library().tfdtd_dft_flush(ctypes.byref(d), 1, N.stream_handle(self.grid.device)), 'tfdtd_dft_flush')
+
     def result(self):
+        self.flush()
         return plane_result(self.monitor, self.plan, self.frequency, self.components,
                             self.value.detach().cpu().numpy(), self.grid.region.dimension)
 

@@ -157,6 +157,7 @@ class SurrogateNet(nn.Module):
 
 def _plane_flux(plane):
     """Signed flux through a z-normal plane per frequency, on the device (field_monitors.py:167-183)."""
+    plane.flush()
     comps = plane.components
     v = plane.value
     ex, ey = v[..., comps.index('Ex')], v[..., comps.index('Ey')]
