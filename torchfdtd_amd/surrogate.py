@@ -287,3 +287,100 @@ def unscreened(evaluate, *, iterations=24, per_iteration=32, seed=0):
             best, best_fom = pool[k], float(judged[k])
         history.append((solves, best_fom))
     return history, best
+
+
+# ------------------------------------------------------------------ the design study (data, training, screen)
+
+SLAB_WAVELENGTHS_UM = None
+
+
+def slab_wavelengths(frequencies=32):
+    """Wavelengths (um) of the design-slab plane's frequency samples, in sample order."""
+    from .monitors import frequency_samples
+    from .workloads import design_slab
+    return 299792458e6/np.asarray(frequency_samples(design_slab(frequencies=frequencies).monitors[0].spectrum))
+
+
+def generate(start, count, *, seed=2026, steps=2000, frequencies=32, cohort=64, log=print):
+    """Transmission spectra of designs start .. start+count of the seeded design sequence."""
+    from .workloads import slab_densities
+    densities = slab_densities(start+count, seed)[start:]
+    reference = slab_flux(np.zeros((1, *densities.shape[1:]), np.float32), steps=steps, frequencies=frequencies)[0]
+    flux = slab_flux(densities, steps=steps, frequencies=frequencies, cohort=cohort,
+                     progress=lambda done: log(f'generate: {done}/{count} designs'))
+    return flux/reference, reference
+
+
+def design_study(data, *, seed=2026, epochs=30, batch=128, lr=3e-4, candidates=4096, verify=32, iterations=6,
+                 baseline_iterations=24, steps=2000, frequencies=32, band=(1.50, 1.60), log=print):
+    """Train the surrogate on `data` (transmission [n, 32] of the first n seeded designs), then run the
+    screened and the unscreened design loops with the FDTD solve as the judge."""
+    import time
+    from .workloads import slab_densities
+    device = torch.device('cuda')
+    n = data.shape[0]
+    densities = torch.as_tensor(slab_densities(n, seed), device=device)
+    targets = torch.as_tensor(data, device=device, dtype=torch.float32)
+    held = max(1, n//10)
+    net = SurrogateNet(seed=seed).to(device)
+    started = time.perf_counter()
+    history = train(net, densities[held:], targets[held:], epochs=epochs, batch=batch, lr=lr, seed=seed,
+                    log=lambda e, v: log(f'train: epoch {e} mse {v:.5f}'))
+    train_seconds = time.perf_counter()-started
+    predicted = predict(net, densities[:held])
+    error = float(((predicted-targets[:held])**2).mean().sqrt())
+    wavelengths = slab_wavelengths(frequencies)
+    net.objective = lambda t: band_figure_of_merit(t.cpu(), wavelengths, band).to(t.device)
+    reference = slab_flux(np.zeros((1, *densities.shape[1:]), np.float32), steps=steps, frequencies=frequencies)[0]
+
+    def judge(pool):
+        t = slab_flux(pool, steps=steps, frequencies=frequencies)/reference
+        return band_figure_of_merit(t, wavelengths, band).numpy()
+    screened, errors, best = screen(net, judge, iterations=iterations, candidates=candidates, verify=verify, seed=seed+1)
+    plain, plain_best = unscreened(judge, iterations=baseline_iterations, per_iteration=verify, seed=seed+1)
+    target = screened[-1][1]
+    reached = next((solves for solves, fom in plain if fom >= target), None)
+    return dict(samples=n, parameters=net.parameter_count(), epochs=epochs, train_seconds=train_seconds,
+                final_train_mse=history[-1], heldout_rmse=error, screened=screened, unscreened=plain,
+                screened_solves=screened[-1][0], screened_best=target,
+                unscreened_solves_to_match=reached, unscreened_best=plain[-1][1],
+                surrogate_fom_rmse=float(np.sqrt(np.mean(errors**2))), band_um=list(band))
+
+
+def _main(argv=None):
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(prog='python -m torchfdtd_amd.surrogate')
+    sub = ap.add_subparsers(dest='command', required=True)
+    g = sub.add_parser('generate', help='FDTD transmission of seeded designs (batched cohorts)')
+    g.add_argument('--start', type=int, default=0)
+    g.add_argument('--count', type=int, default=1024)
+    g.add_argument('--steps', type=int, default=2000)
+    g.add_argument('--out', required=True)
+    s = sub.add_parser('study', help='train on generated data, then the screened and unscreened design loops')
+    s.add_argument('--data', nargs='+', required=True)
+    s.add_argument('--epochs', type=int, default=30)
+    s.add_argument('--iterations', type=int, default=6)
+    s.add_argument('--baseline-iterations', type=int, default=24)
+    s.add_argument('--steps', type=int, default=2000)
+    s.add_argument('--out', required=True)
+    args = ap.parse_args(argv)
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    if args.command == 'generate':
+        t, reference = generate(args.start, args.count, steps=args.steps, log=log)
+        np.savez(args.out, transmission=t.astype(np.float32), reference_flux=reference, start=args.start, steps=args.steps)
+        log(f'generate: wrote {args.out}')
+        return 0
+    parts = sorted((np.load(p) for p in args.data), key=lambda z: int(z['start']))
+    data = np.concatenate([z['transmission'] for z in parts])
+    result = design_study(data, epochs=args.epochs, iterations=args.iterations, baseline_iterations=args.baseline_iterations,
+                          steps=args.steps, log=log)
+    with open(args.out, 'w') as f:
+        json.dump(result, f, indent=1, default=float)
+    print(json.dumps({k: v for k, v in result.items() if k not in ('screened', 'unscreened')}, default=float))
+    return 0
+
+
+if __name__ == '__main__':
+    raise SystemExit(_main())
