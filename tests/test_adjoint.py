@@ -140,6 +140,7 @@ def test_plane_objective_gradient_matches_central_differences(gpu):
     epsilon = torch.tensor(eps, device='cuda', requires_grad=True)
     objective(epsilon).backward()
     gradient = epsilon.grad.cpu().numpy()
+    assert np.abs(gradient).max() > 0
     h = 1e-5
     cells = [(8, 6, 5), (10, 5, 4), tuple(int(v) for v in np.unravel_index(np.argmax(np.abs(gradient)), gradient.shape))]
     for cell in cells:

@@ -172,7 +172,7 @@ class Stepper:
         g = self.grid
         o = N.Observe()
         o.dtype = N.F64 if g.dtype == torch.float64 else N.F32
-        o.count = len(self.point_monitors)
+        o.count = len(self.trace_points)
         o.table = table.data_ptr() if o.count else None
         o.traces = self.traces.data_ptr() if o.count else None
         o.steps = self.steps
