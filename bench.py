@@ -67,7 +67,19 @@ def cpu_baseline(seconds_budget, mask):
                        f'{elapsed:.1f} s')
 
 
-def batched_line(steps, count, loop_grids, width=50):
+def counted_traffic(n):
+    """HBM bytes per one-pass step of this cube from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE per dispatch, scaled by a plain copy's counts), or None for another size."""
+    path = os.path.join(ROOT, 'profiles', 'march_512cube_hbm_counters.md')
+    if n != 512 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith('traffic_bytes_per_step:'):
+            return int(float(line.split(':')[1]))
+    return None
+
+
+def batched_line(steps, count, loop_grids, launch='tune', width=50):
     """The uneven batch: 64 grids of 64x64x32 .. 192x160x64 cells, 2000 steps, autotuned launch,
     against the per-grid loop of Simulation.run (both with graphs of `width` steps)."""
     from torchfdtd_amd import workloads as W
@@ -76,23 +88,27 @@ def batched_line(steps, count, loop_grids, width=50):
     cases = W.uneven_batch(count, steps)
     cells = sum(math.prod(c.project.region.shape) for c in cases)
     print(f'bench: uneven batch, {count} grids, {cells} cells', file=sys.stderr, flush=True)
-    tuning = tune_tensor_batch(cases, candidates=(Launch('split'), Launch('march', 8, 8), Launch('march', 8, 16),
-                                                  Launch('march', 16, 8)), repeats=2, trial_steps=100)
-    report = run_tensor_batch(cases, launch=tuning, cuda_graph_steps=width, keep_results=False)
+    if launch == 'tune':
+        tuning = tune_tensor_batch(cases, candidates=(Launch('split'), Launch('march', 8, 8), Launch('march', 8, 16),
+                                                      Launch('march', 16, 8)), repeats=2, trial_steps=100)
+        chosen, timings = tuning.launch, {k: round(v['median_seconds'], 5) for k, v in tuning.timings.items()}
+    else:
+        chosen, timings = Launch(launch), None
+    report = run_tensor_batch(cases, launch=chosen, cuda_graph_steps=width, keep_results=False)
     batched = cells*steps/report.plan['loop_seconds']/1e9
-    print(f'bench: batched {batched:.2f} Gcell/s with {tuning.launch}; per-grid loop over {loop_grids} grids',
+    print(f'bench: batched {batched:.2f} Gcell/s with {chosen}; per-grid loop over {loop_grids} grids',
           file=sys.stderr, flush=True)
     loop_seconds, loop_cells = 0., 0
     for c in cases[:loop_grids]:
         result = Simulation(c.project, epsilon=c.epsilon).run(cuda_graph_steps=width)
         loop_seconds += result.summary['seconds']
         loop_cells += math.prod(c.project.region.shape)
-    loop = loop_cells*steps/loop_seconds/1e9
+    loop = loop_cells*steps/loop_seconds/1e9 if loop_grids else None
     return dict(workload=f'{count} uneven 3D grids 64x64x32..192x160x64, 10-cell CPML, {steps} steps, Ez dipole, '
                          '16-frequency float64 DFT plane, binary design block', cells=cells,
-                value=round(batched, 3), unit='Gcell-updates/s', loop_value=round(loop, 3), loop_grids=loop_grids,
-                speedup_vs_loop=round(batched/loop, 3), launch=tuning.launch.as_dict(),
-                tuning={k: round(v['median_seconds'], 5) for k, v in tuning.timings.items()},
+                value=round(batched, 3), unit='Gcell-updates/s', loop_value=round(loop, 3) if loop else None,
+                loop_grids=loop_grids, speedup_vs_loop=round(batched/loop, 3) if loop else None, launch=chosen.as_dict(),
+                tuning=timings,
                 loop_seconds=round(report.plan['loop_seconds'], 4))
 
 
@@ -135,6 +151,9 @@ def main():
     ap.add_argument('--no-batched', action='store_true')
     ap.add_argument('--no-adjoint', action='store_true')
     ap.add_argument('--batch-grids', type=int, default=64)
+    ap.add_argument('--batch-steps', type=int, default=2000)
+    ap.add_argument('--batch-launch', choices=('tune', 'split', 'march'), default='tune')
+    ap.add_argument('--no-cube', action='store_true', help='skip the 512^3 cube (profiling the other parts alone)')
     ap.add_argument('--loop-grids', type=int, default=64)
     ap.add_argument('--copy-gib', type=float, default=0,
                     help='only copy a buffer of this many GiB three times (the calibration of HBM byte counters)')
@@ -152,6 +171,14 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({'copy_bytes_read_per_launch': count*4, 'copy_bytes_written_per_launch': count*4, 'launches': 3}))
+        return 0
+    if args.no_cube:
+        line = {}
+        if not args.no_batched:
+            line['batched'] = batched_line(args.batch_steps, args.batch_grids, args.loop_grids, args.batch_launch)
+        if not args.no_adjoint:
+            line['adjoint'] = adjoint_line()
+        print(json.dumps(line), flush=True)
         return 0
     from torchfdtd_amd import _native as N
     from torchfdtd_amd.graphs import StepGraphs
@@ -220,7 +247,7 @@ def main():
         per_step_ms, kernel = kernel_ms['yee_half_step_E']+kernel_ms['yee_half_step_H'], 'yee_half_step E+H'
     achieved = step_bytes/(per_step_ms/1e3)/1e9
     roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
-                    frac=round(achieved/HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved/HBM_PEAK_GBS, 4), traffic=counted_traffic(args.n) if run.one_pass else None,
                     kernel=kernel, kernel_ms=kernel_ms,
                     bytes_per_cell_update=round(step_bytes/cells, 2), psi_per_cell=round(psi_per_cell, 4))
     baseline = None
@@ -236,7 +263,7 @@ def main():
                        'cells': cells, 'graph_steps': width, 'wall_s': round(wall, 3)},
             'roofline': roofline, 'cpu_baseline': baseline}
     if not args.no_batched:
-        line['batched'] = batched_line(2000, args.batch_grids, args.loop_grids)
+        line['batched'] = batched_line(args.batch_steps, args.batch_grids, args.loop_grids, args.batch_launch)
     if not args.no_adjoint:
         print('bench: adjoint gradient', file=sys.stderr, flush=True)
         line['adjoint'] = adjoint_line()

@@ -173,3 +173,28 @@ def test_uneven_workload_is_seeded_and_within_bounds():
     assert design.shape == (32, 32) and .2 < design.mean() < .8
     # The conic filter is normalised: a constant density passes unchanged.
     assert np.allclose(W.conic_filter(np.full((9, 9), .3), 2.5), .3)
+
+
+@pytest.mark.gpu
+def test_cohort_reset_reproduces_fresh_cohorts(gpu):
+    from torchfdtd_amd.batch import Cohort
+    cases = uneven_cases(steps=30)
+    projects = [c.project for c in cases]
+    cohort = Cohort(projects, Launch('march', 8, 8), epsilons=[c.epsilon for c in cases])
+    graphs = cohort.graphs(8)
+
+    def run():
+        for _ in range(3):
+            graphs.replay(8)
+        for _ in range(6):
+            graphs.replay(1)
+        return cohort.digest()
+    first = run()
+    cohort.reset([c.epsilon for c in cases])
+    assert run() == first
+    other = [np.flip(c.epsilon, 0).copy() for c in cases]
+    cohort.reset(other)
+    changed = run()
+    fresh = Cohort(projects, Launch('march', 8, 8), epsilons=other)
+    fresh.advance(30)
+    assert changed != first and fresh.digest() == changed

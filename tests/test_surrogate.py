@@ -95,3 +95,14 @@ def test_surrogate_network_gradients_and_training(gpu):
         opt.step()
         losses.append(value.item())
     assert losses[-1] < .5*losses[0]
+
+
+def test_slab_flux_cohorts_match_single_runs(gpu):
+    from torchfdtd_amd.surrogate import slab_flux
+    from torchfdtd_amd.workloads import slab_densities
+    rho = slab_densities(3, 11)
+    together = slab_flux(rho, steps=120, frequencies=8, cohort=3)
+    alone = np.concatenate([slab_flux(rho[i:i+1], steps=120, frequencies=8, cohort=1) for i in range(3)])
+    assert together.shape == (3, 8) and np.isfinite(together).all() and np.abs(together).max() > 0
+    assert np.array_equal(together, alone)
+    assert np.array_equal(slab_flux(rho, steps=120, frequencies=8, cohort=3), together)   # reused cohort, reset state

@@ -268,6 +268,20 @@ class Cohort:
             packed.append((table, len(members), blocks.value, frequencies.value, dtype, spectral))
         return packed
 
+    def reset(self, epsilons):
+        """Start a fresh run of the same cohort with new permittivities (same shapes and layouts): every
+        state returns to zero and the descriptor tables, planes and captured graphs stay valid."""
+        epsilons = list(epsilons)
+        if len(epsilons) != len(self.runs):
+            raise ValueError('one permittivity per grid')
+        for run, eps in zip(self.runs, epsilons):
+            layout = run.grid.inverse_permittivity
+            run.grid.set_permittivity(np.asarray(eps))
+            if run.grid.inverse_permittivity is not layout:
+                raise ValueError("a reset keeps each grid's permittivity layout (per-cell or per-component)")
+        for value in self.states():
+            value.zero_()
+
     def states(self):
         states = [self.counter, self.traces]
         for run in self.runs:

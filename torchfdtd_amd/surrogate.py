@@ -167,12 +167,16 @@ def _plane_flux(plane):
     return density@weights
 
 
-def slab_flux(densities, *, steps=2000, frequencies=32, cohort=64, launch='auto', width=50, progress=None):
+_COHORTS = {}
+
+
+def slab_flux(densities, *, steps=2000, frequencies=32, cohort=64, launch='split', width=50, progress=None):
     """FDTD flux spectra (count, frequencies) of design-slab densities, in batched cohorts.
 
-    Every cohort runs all its grids together (``batch.Cohort``) in HIP graphs of `width` steps;
-    only the plane accumulators are reduced, on the device, to flux spectra."""
-    from .batch import Cohort, resolve_launch
+    One cohort of `cohort` grids (``batch.Cohort``, its step loop captured in HIP graphs of `width`
+    steps) is built once per size and reused: each group of designs only resets the permittivities
+    and the state.  Only the plane accumulators are reduced, on the device, to flux spectra."""
+    from .batch import Cohort, Launch
     from .workloads import design_slab, slab_epsilon
     project = design_slab(steps, frequencies)
     monitor = project.monitors[0]
@@ -182,17 +186,22 @@ def slab_flux(densities, *, steps=2000, frequencies=32, cohort=64, launch='auto'
     with torch.no_grad():
         for start in range(0, len(densities), cohort):
             members = densities[start:start+cohort]
-            projects = [project]*len(members)
-            chosen = resolve_launch(projects, launch)
-            group = Cohort(projects, chosen, epsilons=[slab_epsilon(project, rho) for rho in members])
-            graphs = group.graphs(min(width, steps))
+            key = (len(members), steps, frequencies, launch, width)
+            if key not in _COHORTS:
+                while _COHORTS:
+                    _COHORTS.popitem()
+                torch.cuda.empty_cache()
+                group = Cohort([project]*len(members), Launch(launch) if isinstance(launch, str) else launch,
+                               epsilons=[slab_epsilon(project, rho) for rho in members])
+                _COHORTS[key] = (group, group.graphs(min(width, steps)))
+            group, graphs = _COHORTS[key]
+            group.reset([slab_epsilon(project, rho) for rho in members])
             done = 0
             while done < steps:
                 advance = graphs.width if steps-done >= graphs.width else 1
                 graphs.replay(advance)
                 done += advance
             out.append(torch.stack([_plane_flux(run.planes[0]) for run in group.runs]).cpu())
-            del graphs, group
             if progress:
                 progress(start+len(members))
     return torch.cat(out).numpy()
