@@ -106,3 +106,31 @@ def test_slab_flux_cohorts_match_single_runs(gpu):
     assert together.shape == (3, 8) and np.isfinite(together).all() and np.abs(together).max() > 0
     assert np.array_equal(together, alone)
     assert np.array_equal(slab_flux(rho, steps=120, frequencies=8, cohort=3), together)   # reused cohort, reset state
+
+
+def test_trained_surrogate_screens_better_than_unscreened_picks(gpu):
+    """A small version of the design study: solver data -> training -> ranking of held-out designs.
+
+    The judge is the FDTD transmission the batched solver computed for every design; the method
+    without the surrogate picks designs blindly.  The surrogate's top picks must beat the blind
+    average figure of merit and rank the held-out designs with a positive correlation."""
+    from torchfdtd_amd.surrogate import SurrogateNet, band_figure_of_merit, predict, slab_flux, slab_wavelengths, train
+    from torchfdtd_amd.workloads import slab_densities
+    n, held, steps, frequencies = 512, 96, 600, 8
+    rho = slab_densities(n, 5)
+    reference = slab_flux(np.zeros((1, *rho.shape[1:]), np.float32), steps=steps, frequencies=frequencies)[0]
+    t = slab_flux(rho, steps=steps, frequencies=frequencies)/reference
+    wavelengths = slab_wavelengths(frequencies)
+    fom = band_figure_of_merit(torch.as_tensor(t), wavelengths, (1.50, 1.60)).numpy()
+    x = torch.as_tensor(rho, device='cuda')
+    y = torch.as_tensor(t, device='cuda', dtype=torch.float32)
+    scale = y[held:].std()
+    net = SurrogateNet(outputs=frequencies, seed=2).cuda()
+    history = train(net, x[held:], (y[held:]-1)/scale, epochs=40, batch=64, lr=1e-3)
+    assert history[-1] < history[0]
+    predicted = predict(net, x[:held])*scale+1
+    score = band_figure_of_merit(predicted.cpu(), wavelengths, (1.50, 1.60)).numpy()
+    top = np.argsort(-score)[:12]
+    assert fom[:held][top].mean() > fom[:held].mean()
+    ranks = lambda v: np.argsort(np.argsort(v))
+    assert np.corrcoef(ranks(score), ranks(fom[:held]))[0, 1] > .2

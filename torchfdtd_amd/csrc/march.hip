@@ -93,8 +93,12 @@ template <typename real>
 __device__ __forceinline__ Hit zexpand(const Family<real>& F, int code) {
     if (code < 0) return Hit{-1, 0, 1, 0u, 0u, 0u};
     const int face = code >> 20, q = code & 0xfffff;
-    const Face& f = face ? F.face[2][1] : F.face[2][0];
-    return Hit{face, q, f.hi - f.lo, f.psi[0], f.psi[1], f.coef};
+    // Select values, not addresses: a runtime-chosen reference into the descriptor would force
+    // a private copy of it.
+    const Face& f0 = F.face[2][0];
+    const Face& f1 = F.face[2][1];
+    return Hit{face, q, face ? f1.hi - f1.lo : f0.hi - f0.lo, face ? f1.psi[0] : f0.psi[0], face ? f1.psi[1] : f0.psi[1],
+               face ? f1.coef : f0.coef};
 }
 
 template <typename real>
