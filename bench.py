@@ -136,7 +136,8 @@ def adjoint_line(steps=2000):
                 backward_seconds=round(report['backward_seconds'], 4),
                 backward_over_forward=round(report['backward_seconds']/report['forward_seconds'], 3),
                 checkpoint_interval=report['checkpoint_interval'], record_bytes=report['record_bytes'],
-                observers=report['observers'], gradient_norm=float(rho.grad.norm()))
+                observers=report['observers'], objective=float(objective),
+                gradient_norm=float(rho.grad.double().norm()))
 
 
 def main():
@@ -233,10 +234,9 @@ def main():
         e0.record()
         for k in range(reps):
             if launch is None:
-                status = lib.tfdtd_yee_step_march(ctypes.byref(run.march[k % 2]), stream)
+                run.march_step(k % 2, stream)
             else:
-                status = launch()
-            N.check(status, name)
+                N.check(launch(), name)
         e1.record()
         torch.cuda.synchronize()
         kernel_ms[name] = e0.elapsed_time(e1)/reps

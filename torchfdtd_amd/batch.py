@@ -41,7 +41,7 @@ from .planning import periodic_axes
 from .models import Project
 from .diagnostics import StateNorms
 from .run_control import DecayDecision, source_end_time
-from .solver import ENGINE_LOCK, DisplaySlice, Result, Stepper, run_summary
+from .solver import ENGINE_LOCK, DisplaySlice, Result, Stepper, pack_table as _pack, run_summary
 
 
 @dataclass
@@ -155,21 +155,6 @@ def _admit(project):
         raise ValueError('Tensor batch currently requires real fields.')
     if r.run_control.auto_shutoff:
         raise ValueError('Tensor batch requires fixed-duration runs. Set auto_shutoff=False.')
-
-
-def _pack(count_bytes, pack, descriptors, prefix_rows, device):
-    """Host-pack descriptors with the library, then copy the table and block prefix to the device."""
-    count = len(descriptors)
-    array = (type(descriptors[0])*count)(*descriptors)
-    nbytes = count_bytes(count)
-    if nbytes == 0:
-        raise ValueError('unsupported descriptor dtype')
-    host = (ctypes.c_ubyte*nbytes)()
-    prefix = (ctypes.c_int32*(prefix_rows*(count+1)))()
-    N.check(pack(array, count, host, nbytes, prefix), 'batch pack')
-    table = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(device)
-    prefix_host = np.array(prefix, dtype=np.int32).reshape(prefix_rows, count+1)
-    return table, torch.as_tensor(prefix_host, device=device), prefix_host
 
 
 class Cohort:

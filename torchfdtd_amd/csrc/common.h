@@ -11,6 +11,15 @@
 
 #include "../../include/tfdtd.h"
 
+// The global-memory view of a device pointer.  Pointers that kernels read from descriptor tables
+// in device memory are generic, and generic loads and stores compile to flat instructions (which
+// also count against the LDS wait counter); reading them through a global pointer gives global
+// instructions.  Kernel-argument pointers are inferred global already.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* global(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 namespace tfdtd_detail {
 inline thread_local std::string last_error;
 }

@@ -74,11 +74,12 @@ struct March {
 template <typename real>
 __device__ __forceinline__ real cpml(real d, const Family<real>& F, const Hit& hit, int k, unsigned p, bool write) {
     const unsigned o = (k ? hit.psi1 : hit.psi0) + p;
-    const real b = F.coef[hit.coef + hit.q];
-    const real c = F.coef[hit.coef + hit.w + hit.q];
-    const real ik = F.coef[hit.coef + 2 * hit.w + hit.q];
-    const real m = F.psi_in[o] * b + c * d;
-    if (write) F.psi_out[o] = m;
+    const auto coef = global(F.coef);
+    const real b = coef[hit.coef + hit.q];
+    const real c = coef[hit.coef + hit.w + hit.q];
+    const real ik = coef[hit.coef + 2 * hit.w + hit.q];
+    const real m = global(F.psi_in)[o] * b + c * d;
+    if (write) global(F.psi_out)[o] = m;
     return d * ik + m;
 }
 
@@ -110,8 +111,8 @@ template <typename real>
 __device__ __forceinline__ real cpml_lane(real d, const Family<real>& F, const Hit& hit, const ZCoef<real>& zc, int k,
                                           unsigned p, bool write) {
     const unsigned o = (k ? hit.psi1 : hit.psi0) + p;
-    const real m = F.psi_in[o] * zc.b + zc.c * d;
-    if (write) F.psi_out[o] = m;
+    const real m = global(F.psi_in)[o] * zc.b + zc.c * d;
+    if (write) global(F.psi_out)[o] = m;
     return d * zc.ik + m;
 }
 
@@ -119,9 +120,10 @@ template <typename real>
 __device__ __forceinline__ ZCoef<real> lane_coefficients(const Family<real>& F, const Hit& hit) {
     ZCoef<real> zc{real(0), real(0), real(1)};
     if (hit.face >= 0) {
-        zc.b = F.coef[hit.coef + hit.q];
-        zc.c = F.coef[hit.coef + hit.w + hit.q];
-        zc.ik = F.coef[hit.coef + 2 * hit.w + hit.q];
+        const auto coef = global(F.coef);
+        zc.b = coef[hit.coef + hit.q];
+        zc.c = coef[hit.coef + hit.w + hit.q];
+        zc.ik = coef[hit.coef + 2 * hit.w + hit.q];
     }
     return zc;
 }
@@ -134,7 +136,7 @@ __device__ __forceinline__ void add_sources(const MSource<real>* table, int coun
     for (int k = 0; k < count; ++k) {
         const MSource<real>& s = table[k];
         if (x >= s.lo[0] && x < s.hi[0] && y >= s.lo[1] && y < s.hi[1] && z >= s.lo[2] && z < s.hi[2]) {
-            const real v = s.values[n];
+            const real v = global(s.values)[n];
             vx = s.comp == 0 ? vx + v : vx;
             vy = s.comp == 1 ? vy + v : vy;
             vz = s.comp == 2 ? vz + v : vz;
@@ -156,15 +158,20 @@ template <typename real>
 __device__ __forceinline__ V3<real> zero3() { return V3<real>{real(0), real(0), real(0)}; }
 
 // Field offsets are 32-bit element indices (3 x cells < 2^32 is checked on the host).  A cell's
-// three components are one 12-byte (24-byte) access per lane.
+// three components are one 12-byte (24-byte) access per lane, through a global pointer (the
+// batched kernel reads its field pointers from a device table).
 template <typename real>
 __device__ __forceinline__ V3<real> load3(const real* __restrict__ base, unsigned i) {
-    return *reinterpret_cast<const V3<real>*>(base + i);
+    const auto p = global(base + i);
+    return V3<real>{p[0], p[1], p[2]};
 }
 
 template <typename real>
 __device__ __forceinline__ void store3(real* __restrict__ base, unsigned i, V3<real> v) {
-    *reinterpret_cast<V3<real>*>(base + i) = v;
+    const auto p = global(base + i);
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
 }
 
 // E_new at (x, y, z) from backward differences of H (terms in CURL_TERMS order).
@@ -371,13 +378,13 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int y0 = ty * TY, z0 = tz * TZ - a.zshift;
     const int xa = chunk * a.chunk, xb = min(nx, xa + a.chunk);
-    const long long n = *a.counter;
+    const long long n = *global(a.counter);
     const real* __restrict__ e_in = a.e_in;
     const real* __restrict__ h_in = a.h_in;
     real* __restrict__ e_out = a.e_out;
     real* __restrict__ h_out = a.h_out;
     const real* __restrict__ inv_eps = a.inv_eps;
-    const real hcoef = a.courant * a.inv_mu[0];
+    const real hcoef = a.courant * global(a.inv_mu)[0];
     const unsigned plane = (unsigned)ny * nz;
 
     const int z = z0 + lane;
@@ -412,7 +419,7 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
 #pragma unroll
                 for (int r = 0; r < HR; ++r) {
                     const int y = min(max(y0 - 1 + r, 0), ny - 1);
-                    sH[r * 3 * HC + tid] = h_in[3u * (base + (unsigned)y * nz) + row_offset];
+                    sH[r * 3 * HC + tid] = global(h_in)[3u * (base + (unsigned)y * nz) + row_offset];
                 }
             }
 #pragma unroll
@@ -422,7 +429,7 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
                 if (EPS3) {
                     inv[k] = load3(inv_eps, 3u * cell);
                 } else {
-                    const real v = inv_eps[cell];
+                    const real v = global(inv_eps)[cell];
                     inv[k] = V3<real>{v, v, v};
                 }
             }

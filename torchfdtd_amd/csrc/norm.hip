@@ -60,8 +60,8 @@ __global__ __launch_bounds__(kThreads) void norm_partial(const NormGrid* __restr
     const NormGrid& g = grids[task.grid];
     double sum = 0., peak = 0., bad = 0.;
     if (task.part < 2) {
-        const real* data = static_cast<const real*>(task.part == 0 ? g.E : g.H);
-        const real* inverse = static_cast<const real*>(g.inverse);
+        const auto data = global(static_cast<const real*>(task.part == 0 ? g.E : g.H));
+        const auto inverse = global(static_cast<const real*>(g.inverse));
         const long long n = 3LL * g.nx * g.ny * g.nz;
         const long long end = min(n, task.start + kTask);
         for (long long i = task.start + threadIdx.x; i < end; i += kThreads) {
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kThreads) void norm_partial(const NormGrid* __restr
                 const int z = (int)(cell % g.nz);
                 const int y = (int)((cell / g.nz) % g.ny);
                 const int x = (int)(cell / ((long long)g.nz * g.ny));
-                vol = g.widths[0][x] * g.widths[1][y] * g.widths[2][z];
+                vol = global(g.widths[0])[x] * global(g.widths[1])[y] * global(g.widths[2])[z];
             }
             double w = vol;
             if (task.part == 0) w = vol / (double)inverse[g.eps_stride == 3 ? i : cell];
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kThreads) void norm_partial(const NormGrid* __restr
         }
     } else {
         const int k = task.part - 2;
-        const real* data = static_cast<const real*>(g.psi[k]);
+        const auto data = global(static_cast<const real*>(g.psi[k]));
         const long long end = min(g.psi_count[k], task.start + kTask);
         for (long long i = task.start + threadIdx.x; i < end; i += kThreads)
             if (!isfinite((double)data[i])) bad = 1.;

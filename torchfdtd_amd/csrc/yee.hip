@@ -73,6 +73,9 @@ __device__ __forceinline__ void half_step_cell(const HalfStep<real>& a, const lo
     const int coord[3] = {x, y, z};
     const int dims[3] = {a.nx, a.ny, a.nz};
     const long long stride[3] = {(long long)nyz, (long long)a.nz, 1};
+    const auto src = global(a.src);
+    const auto dst = global(a.dst);
+    const auto inverse = global(a.inverse);
     real curl[3] = {real(0), real(0), real(0)};
 #pragma unroll
     for (int t = 0; t < 6; ++t) {
@@ -83,8 +86,8 @@ __device__ __forceinline__ void half_step_cell(const HalfStep<real>& a, const lo
         const long long s = stride[axis];
         real d = real(0);
         if (forward ? (k < n - 1) : (k > 0)) {
-            d = forward ? (a.src[3 * (i + s) + ic] - a.src[3 * i + ic])
-                        : (a.src[3 * i + ic] - a.src[3 * (i - s) + ic]);
+            d = forward ? (src[3 * (i + s) + ic] - src[3 * i + ic])
+                        : (src[3 * i + ic] - src[3 * (i - s) + ic]);
 #pragma unroll
             for (int side = 0; side < 2; ++side) {
                 const Slab<real>& g = a.slab[2 * t + side];
@@ -95,17 +98,18 @@ __device__ __forceinline__ void half_step_cell(const HalfStep<real>& a, const lo
                     if (axis == 0) p = ((long long)q * a.ny + y) * a.nz + z;
                     else if (axis == 1) p = ((long long)x * w + q) * a.nz + z;
                     else p = ((long long)x * a.ny + y) * w + q;
-                    const real memory = g.psi[p] * g.b[q] + g.c[q] * d;
-                    g.psi[p] = memory;
-                    d = d * g.inv_k[q] + memory;
+                    const auto psi = global(g.psi);
+                    const real memory = psi[p] * global(g.b)[q] + global(g.c)[q] * d;
+                    psi[p] = memory;
+                    d = d * global(g.inv_k)[q] + memory;
                 }
             }
         } else if ((a.periodic_mask >> axis) & 1) {
             const long long span = (long long)(n - 1) * s;
-            d = forward ? (a.src[3 * (i - span) + ic] - a.src[3 * i + ic])
-                        : (a.src[3 * i + ic] - a.src[3 * (i + span) + ic]);
+            d = forward ? (src[3 * (i - span) + ic] - src[3 * i + ic])
+                        : (src[3 * i + ic] - src[3 * (i + span) + ic]);
         } else if (forward && ((a.pec_mask >> axis) & 1)) {
-            d = -a.pec_factor[axis] * a.src[3 * i + ic];
+            d = -a.pec_factor[axis] * src[3 * i + ic];
         }
         curl[oc] = term_sign(t) > 0 ? curl[oc] + d : curl[oc] - d;
     }
@@ -113,25 +117,25 @@ __device__ __forceinline__ void half_step_cell(const HalfStep<real>& a, const lo
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const long long j = a.inverse_stride == 3 ? 3 * i + c : a.inverse_stride == 1 ? i : 0;
-        const real coefficient = a.courant * a.inverse[j];
-        out[c] = forward ? a.dst[3 * i + c] - coefficient * curl[c] : a.dst[3 * i + c] + coefficient * curl[c];
+        const real coefficient = a.courant * inverse[j];
+        out[c] = forward ? dst[3 * i + c] - coefficient * curl[c] : dst[3 * i + c] + coefficient * curl[c];
     }
     if (a.nsrc > 0 && x >= a.box_lo[0] && x < a.box_hi[0] && y >= a.box_lo[1] && y < a.box_hi[1] &&
         z >= a.box_lo[2] && z < a.box_hi[2]) {
-        const long long n = *a.counter;
+        const long long n = *global(a.counter);
         for (int k = 0; k < a.nsrc && n < a.source_samples; ++k) {
-            const Source<real>& src = a.srcs[k];
-            if (x >= src.lo[0] && x < src.hi[0] && y >= src.lo[1] && y < src.hi[1] && z >= src.lo[2] &&
-                z < src.hi[2]) {
-                const real v = src.values[n];
-                if (src.comp == 0) out[0] = out[0] + v;
-                else if (src.comp == 1) out[1] = out[1] + v;
+            const Source<real>& box = a.srcs[k];
+            if (x >= box.lo[0] && x < box.hi[0] && y >= box.lo[1] && y < box.hi[1] && z >= box.lo[2] &&
+                z < box.hi[2]) {
+                const real v = global(box.values)[n];
+                if (box.comp == 0) out[0] = out[0] + v;
+                else if (box.comp == 1) out[1] = out[1] + v;
                 else out[2] = out[2] + v;
             }
         }
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) a.dst[3 * i + c] = out[c];
+    for (int c = 0; c < 3; ++c) dst[3 * i + c] = out[c];
 }
 
 template <typename real, bool forward>
@@ -175,7 +179,7 @@ __global__ __launch_bounds__(256) void observe_kernel(const Observe<real> a) {
     __syncthreads();
     if (n < a.steps) {
         for (int k = threadIdx.x; k < a.count; k += blockDim.x) {
-            const real* field = reinterpret_cast<const real*>(a.table[2 * k]);
+            const auto field = global(reinterpret_cast<const real*>(a.table[2 * k]));
             a.traces[n * a.count + k] = field[a.table[2 * k + 1]];
         }
     }
@@ -189,7 +193,7 @@ __global__ __launch_bounds__(256) void observe_wide_kernel(const Observe<real> a
     const long long n = *a.counter;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= a.steps || k >= a.count) return;
-    const real* field = reinterpret_cast<const real*>(a.table[2 * k]);
+    const auto field = global(reinterpret_cast<const real*>(a.table[2 * k]));
     a.traces[n * a.count + k] = field[a.table[2 * k + 1]];
 }
 
@@ -220,9 +224,9 @@ struct Plane {
 template <typename real, typename spec>
 __device__ __forceinline__ void phases(const Plane<real, spec>& a, long long step, int f, spec* p) {
     double si, co;
-    sincos(a.omega[f] * (double)(step + 1), &si, &co);
+    sincos(global(a.omega)[f] * (double)(step + 1), &si, &co);
     const spec er = (spec)co, ei = (spec)si;
-    const spec hr = a.half[2 * f], hi = a.half[2 * f + 1];
+    const spec hr = global(a.half)[2 * f], hi = global(a.half)[2 * f + 1];
     p[0] = er;
     p[1] = ei;
     p[2] = er * hr - ei * hi;
@@ -235,12 +239,12 @@ __device__ __forceinline__ real plane_sample(const Plane<real, spec>& a, long lo
     const int c = (int)(i % a.nc);
     const long long p = i / a.nc;
     const bool electric = c < a.ne;
-    const real* field = electric ? a.E : a.H;
-    const long long* idx = a.indices[c];
-    const real* w = a.weights[c];
+    const auto field = global(electric ? a.E : a.H);
+    const auto idx = global(a.indices[c]);
+    const auto w = global(a.weights[c]);
     real v = real(0);
     for (int j = 0; j < a.corners; ++j) v = v + field[idx[j * a.points + p]] * w[j * a.points + p];
-    return v * (electric ? a.window_e : a.window_h)[n];
+    return v * global(electric ? a.window_e : a.window_h)[n];
 }
 
 // Accumulate the staged samples of steps [lo, hi) (one block at most): the phases of those steps go
@@ -260,12 +264,12 @@ __device__ __forceinline__ void flush_range(const Plane<real, spec>& a, long lon
         for (int j = 0; j < count; ++j) {
             const long long step = lo + j;
             if (step % a.downsample) continue;
-            const spec v = (spec)a.stage[(step % a.block) * samples + i];
+            const spec v = (spec)global(a.stage)[(step % a.block) * samples + i];
             const spec* p = phase + 4 * (j * a.nf + f) + o;
             sr += (double)(p[0] * v);
             si += (double)(p[1] * v);
         }
-        spec* out = a.value + 2 * ((long long)f * samples + i);
+        const auto out = global(a.value + 2 * ((long long)f * samples + i));
         out[0] = out[0] + (spec)sr;
         out[1] = out[1] + (spec)si;
     }
@@ -278,15 +282,16 @@ template <typename real, typename spec>
 __device__ __forceinline__ void dft_plane_block(const Plane<real, spec>& a, long long block) {
     extern __shared__ unsigned char lds_raw[];
     spec* phase = reinterpret_cast<spec*>(lds_raw);
-    const long long n = *a.counter;
+    const long long n = *global(a.counter);
     if (n >= a.samples) return;
     const long long samples = a.points * a.nc;
     const long long i = block * blockDim.x + threadIdx.x;
     if (a.block > 1) {
-        if (n % a.downsample == 0 && i < samples) a.stage[(n % a.block) * samples + i] = plane_sample(a, i, n);
+        if (n % a.downsample == 0 && i < samples) global(a.stage)[(n % a.block) * samples + i] = plane_sample(a, i, n);
         if ((n + 1) % a.block == 0 || n + 1 == a.samples) {
             const long long first = n - n % a.block;
-            flush_range(a, i, first > *a.flushed ? first : *a.flushed, n + 1);
+            const long long flushed = *global(a.flushed);
+            flush_range(a, i, first > flushed ? first : flushed, n + 1);
         }
         return;
     }
@@ -297,7 +302,7 @@ __device__ __forceinline__ void dft_plane_block(const Plane<real, spec>& a, long
     const spec sv = (spec)plane_sample(a, i, n);
     const int o = (int)(i % a.nc) < a.ne ? 0 : 2;
     for (int f = 0; f < a.nf; ++f) {
-        spec* out = a.value + 2 * ((long long)f * samples + i);
+        const auto out = global(a.value + 2 * ((long long)f * samples + i));
         out[0] = out[0] + phase[4 * f + o] * sv;
         out[1] = out[1] + phase[4 * f + o + 1] * sv;
     }

@@ -120,6 +120,21 @@ def _check_epsilon(eps, region):
     return eps
 
 
+def pack_table(count_bytes, pack, descriptors, prefix_rows, device):
+    """Host-pack descriptors with the library, then copy the table and block prefix to the device."""
+    count = len(descriptors)
+    array = (type(descriptors[0])*count)(*descriptors)
+    nbytes = count_bytes(count)
+    if nbytes == 0:
+        raise ValueError('unsupported descriptor dtype')
+    host = (ctypes.c_ubyte*nbytes)()
+    prefix = (ctypes.c_int32*(prefix_rows*(count+1)))()
+    N.check(pack(array, count, host, nbytes, prefix), 'batch pack')
+    table = torch.frombuffer(bytearray(host), dtype=torch.uint8).to(device)
+    prefix_host = np.array(prefix, dtype=np.int32).reshape(prefix_rows, count+1)
+    return table, torch.as_tensor(prefix_host, device=device), prefix_host
+
+
 class Stepper:
     """Device state of one resident run and its step (the reference's closure, solver.py:776-797)."""
 
@@ -202,6 +217,13 @@ class Stepper:
                                            self.tile_y, self.chunk_x),
                           march_descriptor(g, self.E_b, self.H_b, g.E, g.H, self.psi_e_b, psi_a, self.sources, self.counter,
                                            self.tile_y, self.chunk_x)]
+            # The step reads its descriptor from device memory (a batch of one grid): the kernel then
+            # loads tile parameters on demand instead of holding the whole descriptor in registers.
+            self.march_tables = []
+            for d in self.march:
+                table, prefix, host = pack_table(lambda n: self.lib.tfdtd_batch_march_bytes(n, dtype),
+                                                 self.lib.tfdtd_batch_march_pack, [d], 3, g.device)
+                self.march_tables.append((table, prefix, (ctypes.c_int32*3)(*host[:, 1].tolist())))
             self.plane_table_b = self._planes(self.E_b, self.H_b)
             self.trace_table_b = self._trace_table(self.E_b, self.H_b)
             self.observe_b = self._observe(self.trace_table_b)
@@ -210,13 +232,19 @@ class Stepper:
         extra = [self.E_b, self.H_b, self.psi_e_b] if self.one_pass else []
         return [self.grid.E, self.grid.H, self.counter, self.traces, *self.grid.memory_states, *extra]
 
+    def march_step(self, k, stream):
+        """One one-pass step from buffer set k (0: grid buffers -> second set, 1: back)."""
+        table, prefix, totals = self.march_tables[k]
+        d = self.march[k]
+        N.check(self.lib.tfdtd_batch_march(table.data_ptr(), prefix.data_ptr(), 1, d.dtype, self.tile_y, d.eps_stride,
+                                           totals, stream), 'tfdtd_batch_march')
+
     def step_pair(self):
         """Two steps with the one-pass kernel: grid buffers -> second set -> grid buffers."""
         stream = N.stream_handle(self.grid.device)
         lib = self.lib
-        for march, planes, observe in ((self.march[0], self.plane_table_b, self.observe_b),
-                                       (self.march[1], self.plane_table, self.observe)):
-            N.check(lib.tfdtd_yee_step_march(ctypes.byref(march), stream), 'tfdtd_yee_step_march')
+        for k, planes, observe in ((0, self.plane_table_b, self.observe_b), (1, self.plane_table, self.observe)):
+            self.march_step(k, stream)
             if self.planes:
                 N.check(lib.tfdtd_dft_update(planes, len(self.planes), stream), 'tfdtd_dft_update')
             N.check(lib.tfdtd_observe_step(ctypes.byref(observe), stream), 'tfdtd_observe_step')

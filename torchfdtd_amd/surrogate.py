@@ -247,13 +247,13 @@ def band_figure_of_merit(transmission, wavelengths_um, band=(1.50, 1.60)):
 
 
 def propose(rng, count, best=None, flip=.08):
-    """Candidate densities: fresh smoothed-threshold patterns, and (with a best design) mutations of it that
-    re-threshold the best pattern plus filtered noise."""
-    from .workloads import SLAB_DESIGN, binary_design, conic_filter
+    """Candidate densities: fresh smoothed-threshold patterns (fill fractions drawn from SLAB_FILL), and
+    (with a best design) mutations of it that re-threshold the best pattern plus filtered noise."""
+    from .workloads import SLAB_DESIGN, SLAB_FILL, binary_design, conic_filter
     out = np.empty((count, *SLAB_DESIGN), dtype=np.float32)
     for i in range(count):
         if best is None or i % 2 == 0:
-            out[i] = binary_design(rng, SLAB_DESIGN[:2])[:, :, None]
+            out[i] = binary_design(rng, SLAB_DESIGN[:2], fill=rng.uniform(*SLAB_FILL))[:, :, None]
         else:
             base = best[:, :, 0].astype(np.float64)
             noisy = conic_filter(np.clip(base+flip*rng.normal(size=base.shape)*8, 0, 1), 2.5)

@@ -102,9 +102,14 @@ def conic_filter(density, radius_cells):
     return out
 
 
-def binary_design(rng, shape=(32, 32), radius_um=.1, mesh=.04):
-    """0/1 design: U(0,1) noise, conic filter of `radius_um`, threshold 0.5."""
-    return conic_filter(rng.random(shape), radius_um/mesh) > .5
+def binary_design(rng, shape=(32, 32), radius_um=.1, mesh=.04, fill=None):
+    """0/1 design: U(0,1) noise, conic filter of `radius_um`, threshold 0.5 (or, with `fill`, the
+    quantile that sets that fraction of the cells to 1)."""
+    smooth = conic_filter(rng.random(shape), radius_um/mesh)
+    return smooth > (.5 if fill is None else np.quantile(smooth, 1-fill))
+
+
+SLAB_FILL = (.2, .8)
 
 
 def uneven_grid(shape, steps=2000, mesh=.04, pml=10, frequencies=16, precision='float32', backend='cuda', name='grid',
@@ -191,11 +196,13 @@ def slab_epsilon(project, rho):
 
 
 def slab_densities(count, seed, radius_um=.1, mesh=.04):
-    """`count` binary slab densities: smoothed-threshold noise patterns (64x64) extruded over 8 cells."""
+    """`count` binary slab densities: smoothed-threshold noise patterns (64x64) extruded over 8 cells,
+    each with a fill fraction drawn uniformly from SLAB_FILL."""
     rng = np.random.default_rng(seed)
     out = np.empty((count, *SLAB_DESIGN), dtype=np.float32)
     for i in range(count):
-        out[i] = binary_design(rng, SLAB_DESIGN[:2], radius_um, mesh)[:, :, None]
+        fill = rng.uniform(*SLAB_FILL)
+        out[i] = binary_design(rng, SLAB_DESIGN[:2], radius_um, mesh, fill)[:, :, None]
     return out
 
 
