@@ -61,8 +61,21 @@ struct HalfStep {
     const long long* counter;
 };
 
+// The three components of one cell, read as one 12-byte (24-byte) access through a global pointer.
+template <typename real>
+struct Cell {
+    real v[3];
+};
+
+template <typename real>
+__device__ __forceinline__ Cell<real> cell_at(const real* base, long long i) {
+    const auto p = global(base + 3 * i);
+    return Cell<real>{{p[0], p[1], p[2]}};
+}
+
 // One cell.  forward=false: E update from backward differences of H;
-// forward=true: H update from forward differences of E.
+// forward=true: H update from forward differences of E.  The cell and its three neighbours
+// (x, y, z, one row back or ahead) are read whole; the terms then follow CURL_TERMS order.
 template <typename real, bool forward>
 __device__ __forceinline__ void half_step_cell(const HalfStep<real>& a, const long long i) {
     const int nyz = a.ny * a.nz;
@@ -73,9 +86,21 @@ __device__ __forceinline__ void half_step_cell(const HalfStep<real>& a, const lo
     const int coord[3] = {x, y, z};
     const int dims[3] = {a.nx, a.ny, a.nz};
     const long long stride[3] = {(long long)nyz, (long long)a.nz, 1};
-    const auto src = global(a.src);
-    const auto dst = global(a.dst);
-    const auto inverse = global(a.inverse);
+    const Cell<real> c = cell_at(a.src, i);
+    Cell<real> nb[3];
+    bool inside[3];
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        const int n = dims[axis], k = coord[axis];
+        const long long s = stride[axis];
+        inside[axis] = n > 1 && (forward ? (k < n - 1) : (k > 0));
+        nb[axis] = Cell<real>{{real(0), real(0), real(0)}};
+        if (inside[axis]) nb[axis] = cell_at(a.src, forward ? i + s : i - s);
+        else if (n > 1 && ((a.periodic_mask >> axis) & 1)) {
+            const long long span = (long long)(n - 1) * s;
+            nb[axis] = cell_at(a.src, forward ? i - span : i + span);
+        }
+    }
     real curl[3] = {real(0), real(0), real(0)};
 #pragma unroll
     for (int t = 0; t < 6; ++t) {
@@ -83,11 +108,9 @@ __device__ __forceinline__ void half_step_cell(const HalfStep<real>& a, const lo
         const int n = dims[axis];
         if (n == 1) continue;
         const int k = coord[axis];
-        const long long s = stride[axis];
         real d = real(0);
-        if (forward ? (k < n - 1) : (k > 0)) {
-            d = forward ? (src[3 * (i + s) + ic] - src[3 * i + ic])
-                        : (src[3 * i + ic] - src[3 * (i - s) + ic]);
+        if (inside[axis]) {
+            d = forward ? (nb[axis].v[ic] - c.v[ic]) : (c.v[ic] - nb[axis].v[ic]);
 #pragma unroll
             for (int side = 0; side < 2; ++side) {
                 const Slab<real>& g = a.slab[2 * t + side];
@@ -105,20 +128,24 @@ __device__ __forceinline__ void half_step_cell(const HalfStep<real>& a, const lo
                 }
             }
         } else if ((a.periodic_mask >> axis) & 1) {
-            const long long span = (long long)(n - 1) * s;
-            d = forward ? (src[3 * (i - span) + ic] - src[3 * i + ic])
-                        : (src[3 * i + ic] - src[3 * (i + span) + ic]);
+            d = forward ? (nb[axis].v[ic] - c.v[ic]) : (c.v[ic] - nb[axis].v[ic]);
         } else if (forward && ((a.pec_mask >> axis) & 1)) {
-            d = -a.pec_factor[axis] * src[3 * i + ic];
+            d = -a.pec_factor[axis] * c.v[ic];
         }
         curl[oc] = term_sign(t) > 0 ? curl[oc] + d : curl[oc] - d;
     }
+    const Cell<real> old = cell_at(a.dst, i);
+    Cell<real> inv;
+    if (a.inverse_stride == 3) inv = cell_at(a.inverse, i);
+    else {
+        const real v = global(a.inverse)[a.inverse_stride == 1 ? i : 0];
+        inv = Cell<real>{{v, v, v}};
+    }
     real out[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const long long j = a.inverse_stride == 3 ? 3 * i + c : a.inverse_stride == 1 ? i : 0;
-        const real coefficient = a.courant * inverse[j];
-        out[c] = forward ? dst[3 * i + c] - coefficient * curl[c] : dst[3 * i + c] + coefficient * curl[c];
+    for (int m = 0; m < 3; ++m) {
+        const real coefficient = a.courant * inv.v[m];
+        out[m] = forward ? old.v[m] - coefficient * curl[m] : old.v[m] + coefficient * curl[m];
     }
     if (a.nsrc > 0 && x >= a.box_lo[0] && x < a.box_hi[0] && y >= a.box_lo[1] && y < a.box_hi[1] &&
         z >= a.box_lo[2] && z < a.box_hi[2]) {
@@ -134,8 +161,10 @@ __device__ __forceinline__ void half_step_cell(const HalfStep<real>& a, const lo
             }
         }
     }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) dst[3 * i + c] = out[c];
+    const auto cell = global(a.dst + 3 * i);
+    cell[0] = out[0];
+    cell[1] = out[1];
+    cell[2] = out[2];
 }
 
 template <typename real, bool forward>

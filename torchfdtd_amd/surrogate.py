@@ -211,10 +211,12 @@ def slab_transmission(densities, reference_flux, **kwargs):
     return slab_flux(densities, **kwargs)/reference_flux
 
 
-def train(net, densities, targets, *, epochs=20, batch=128, lr=3e-4, weight_decay=1e-2, seed=0, log=None):
-    """AdamW on mean-squared error; densities (n, 64, 64, 8), targets (n, 32), both on the device."""
+def train(net, densities, targets, *, epochs=20, batch=128, lr=3e-4, weight_decay=1e-2, seed=0, log=None, cosine=False):
+    """AdamW on mean-squared error; densities (n, 64, 64, 8), targets (n, outputs), both on the device.
+    `cosine`: decay the learning rate to zero over the epochs."""
     gen = torch.Generator(device='cpu').manual_seed(seed)
     opt = torch.optim.AdamW(net.parameters(), lr=lr, weight_decay=weight_decay)
+    schedule = torch.optim.lr_scheduler.CosineAnnealingLR(opt, epochs) if cosine else None
     n = densities.shape[0]
     history = []
     for epoch in range(epochs):
@@ -227,6 +229,8 @@ def train(net, densities, targets, *, epochs=20, batch=128, lr=3e-4, weight_deca
             loss.backward()
             opt.step()
             total += loss.item()*len(idx)
+        if schedule:
+            schedule.step()
         history.append(total/n)
         if log:
             log(epoch, history[-1])
@@ -261,35 +265,64 @@ def propose(rng, count, best=None, flip=.08):
     return out
 
 
-def screen(net, evaluate, *, iterations=6, candidates=4096, verify=32, seed=0):
+class Surrogate:
+    """The network with its target normalisation: spectra are learned as (t - mean) / std per frequency."""
+
+    def __init__(self, net, mean, std):
+        self.net, self.mean, self.std = net, mean, std
+
+    def fit(self, densities, spectra, **kwargs):
+        return train(self.net, densities, (spectra-self.mean)/self.std, **kwargs)
+
+    def __call__(self, densities):
+        return predict(self.net, densities)*self.std+self.mean
+
+
+def screen(surrogate, solve, fom, *, iterations=6, candidates=4096, verify=32, seed=0, replay=None, refine_epochs=0,
+           log=None):
     """Surrogate-screened design loop: each iteration proposes `candidates` designs, ranks them by the
-    surrogate's figure of merit and judges only the top `verify` with the FDTD `evaluate`
-    (densities -> figure of merit).  Returns the FDTD history [(solves, best FOM)] and the surrogate's
-    error on every judged design."""
+    surrogate's figure of merit and solves only the top `verify` with FDTD (`solve`: densities ->
+    spectra [n, k] on the device; `fom`: spectra -> figure of merit).  With `refine_epochs`, the
+    surrogate is refitted after each iteration on every design solved so far plus `replay` (a
+    (densities, spectra) sample of its training set).  Returns the FDTD history [(solves, best FOM)],
+    the surrogate's FOM error on every solved design and the best design."""
     rng = np.random.default_rng(seed)
-    device = next(net.parameters()).device
+    device = surrogate.mean.device
     best, best_fom, solves, history, errors = None, -np.inf, 0, [], []
-    for _ in range(iterations):
+    seen_x, seen_y = [], []
+    for it in range(iterations):
         pool = propose(rng, candidates, best)
-        predicted = net.figure_of_merit(predict(net, torch.as_tensor(pool, device=device))).cpu().numpy()
+        x = torch.as_tensor(pool, device=device)
+        predicted = fom(surrogate(x)).cpu().numpy()
         top = np.argsort(-predicted)[:verify]
-        judged = evaluate(pool[top])
+        spectra = solve(pool[top])
+        judged = fom(spectra).cpu().numpy()
         solves += verify
         errors.extend((judged-predicted[top]).tolist())
         k = int(np.argmax(judged))
         if judged[k] > best_fom:
             best, best_fom = pool[top][k], float(judged[k])
         history.append((solves, best_fom))
+        if log:
+            log(f'screen: iteration {it}: {solves} solves, best {best_fom:.4f}, '
+                f'surrogate error {np.sqrt(np.mean((judged-predicted[top])**2)):.4f}')
+        if refine_epochs and it+1 < iterations:
+            seen_x.append(x[torch.as_tensor(top, device=device)])
+            seen_y.append(spectra)
+            xs, ys = torch.cat(seen_x), torch.cat(seen_y)
+            if replay is not None:
+                xs, ys = torch.cat([xs, replay[0]]), torch.cat([ys, replay[1]])
+            surrogate.fit(xs, ys, epochs=refine_epochs, batch=64, lr=3e-4, weight_decay=1e-4, seed=seed+it)
     return history, np.asarray(errors), best
 
 
-def unscreened(evaluate, *, iterations=24, per_iteration=32, seed=0):
-    """The same loop without the surrogate: every proposed design is judged by FDTD."""
+def unscreened(solve, fom, *, iterations=24, per_iteration=32, seed=0):
+    """The same loop without the surrogate: every proposed design is solved with FDTD."""
     rng = np.random.default_rng(seed)
     best, best_fom, solves, history = None, -np.inf, 0, []
     for _ in range(iterations):
         pool = propose(rng, per_iteration, best)
-        judged = evaluate(pool)
+        judged = fom(solve(pool)).cpu().numpy()
         solves += per_iteration
         k = int(np.argmax(judged))
         if judged[k] > best_fom:
@@ -320,8 +353,8 @@ def generate(start, count, *, seed=2026, steps=2000, frequencies=32, cohort=64, 
     return flux/reference, reference
 
 
-def design_study(data, *, seed=2026, epochs=30, batch=128, lr=3e-4, candidates=4096, verify=32, iterations=6,
-                 baseline_iterations=24, steps=2000, frequencies=32, band=(1.50, 1.60), log=print):
+def design_study(data, *, seed=2026, epochs=150, batch=128, lr=1e-3, candidates=4096, verify=32, iterations=6,
+                 baseline_iterations=48, refine_epochs=4, steps=2000, frequencies=32, band=(1.50, 1.60), log=print):
     """Train the surrogate on `data` (transmission [n, 32] of the first n seeded designs), then run the
     screened and the unscreened design loops with the FDTD solve as the judge."""
     import time
@@ -331,28 +364,37 @@ def design_study(data, *, seed=2026, epochs=30, batch=128, lr=3e-4, candidates=4
     densities = torch.as_tensor(slab_densities(n, seed), device=device)
     targets = torch.as_tensor(data, device=device, dtype=torch.float32)
     held = max(1, n//10)
-    net = SurrogateNet(seed=seed).to(device)
+    mean, std = targets[held:].mean(0), targets[held:].std(0)
+    surrogate = Surrogate(SurrogateNet(seed=seed).to(device), mean, std)
     started = time.perf_counter()
-    history = train(net, densities[held:], targets[held:], epochs=epochs, batch=batch, lr=lr, seed=seed,
-                    log=lambda e, v: log(f'train: epoch {e} mse {v:.5f}'))
+    history = surrogate.fit(densities[held:], targets[held:], epochs=epochs, batch=batch, lr=lr, weight_decay=1e-4,
+                            seed=seed, cosine=True, log=lambda e, v: log(f'train: epoch {e} mse {v:.5f}'))
     train_seconds = time.perf_counter()-started
-    predicted = predict(net, densities[:held])
+    predicted = surrogate(densities[:held])
     error = float(((predicted-targets[:held])**2).mean().sqrt())
     wavelengths = slab_wavelengths(frequencies)
-    net.objective = lambda t: band_figure_of_merit(t.cpu(), wavelengths, band).to(t.device)
+    fom = lambda t: band_figure_of_merit(t.cpu(), wavelengths, band)
+    heldout_fom = fom(targets[:held]).numpy()
+    heldout_fom_error = float(np.sqrt(np.mean((fom(predicted).numpy()-heldout_fom)**2)))
     reference = slab_flux(np.zeros((1, *densities.shape[1:]), np.float32), steps=steps, frequencies=frequencies)[0]
 
-    def judge(pool):
-        t = slab_flux(pool, steps=steps, frequencies=frequencies)/reference
-        return band_figure_of_merit(t, wavelengths, band).numpy()
-    screened, errors, best = screen(net, judge, iterations=iterations, candidates=candidates, verify=verify, seed=seed+1)
-    plain, plain_best = unscreened(judge, iterations=baseline_iterations, per_iteration=verify, seed=seed+1)
+    def solve(pool):
+        return torch.as_tensor(slab_flux(pool, steps=steps, frequencies=frequencies)/reference, device=device,
+                               dtype=torch.float32)
+    gen = torch.Generator(device='cpu').manual_seed(seed)
+    sample = torch.randperm(n-held, generator=gen)[:1024].to(device)+held
+    screened, errors, best = screen(surrogate, solve, fom, iterations=iterations, candidates=candidates, verify=verify,
+                                    seed=seed+1, replay=(densities[sample], targets[sample]), refine_epochs=refine_epochs,
+                                    log=log)
+    plain, plain_best = unscreened(solve, fom, iterations=baseline_iterations, per_iteration=verify, seed=seed+1)
     target = screened[-1][1]
-    reached = next((solves for solves, fom in plain if fom >= target), None)
-    return dict(samples=n, parameters=net.parameter_count(), epochs=epochs, train_seconds=train_seconds,
-                final_train_mse=history[-1], heldout_rmse=error, screened=screened, unscreened=plain,
+    reached = next((solves for solves, value in plain if value >= target), None)
+    return dict(samples=n, parameters=surrogate.net.parameter_count(), epochs=epochs, train_seconds=train_seconds,
+                final_train_mse=history[-1], heldout_rmse=error, heldout_fom_std=float(heldout_fom.std()),
+                heldout_fom_rmse=heldout_fom_error, refine_epochs=refine_epochs, screened=screened, unscreened=plain,
                 screened_solves=screened[-1][0], screened_best=target,
                 unscreened_solves_to_match=reached, unscreened_best=plain[-1][1],
+                solves_ratio=(reached/screened[-1][0]) if reached else None,
                 surrogate_fom_rmse=float(np.sqrt(np.mean(errors**2))), band_um=list(band))
 
 
@@ -369,9 +411,10 @@ def _main(argv=None):
     g.add_argument('--out', required=True)
     s = sub.add_parser('study', help='train on generated data, then the screened and unscreened design loops')
     s.add_argument('--data', nargs='+', required=True)
-    s.add_argument('--epochs', type=int, default=30)
+    s.add_argument('--epochs', type=int, default=150)
     s.add_argument('--iterations', type=int, default=6)
-    s.add_argument('--baseline-iterations', type=int, default=24)
+    s.add_argument('--baseline-iterations', type=int, default=48)
+    s.add_argument('--refine-epochs', type=int, default=4)
     s.add_argument('--steps', type=int, default=2000)
     s.add_argument('--out', required=True)
     args = ap.parse_args(argv)
@@ -384,7 +427,7 @@ def _main(argv=None):
     parts = sorted((np.load(p) for p in args.data), key=lambda z: int(z['start']))
     data = np.concatenate([z['transmission'] for z in parts])
     result = design_study(data, epochs=args.epochs, iterations=args.iterations, baseline_iterations=args.baseline_iterations,
-                          steps=args.steps, log=log)
+                          refine_epochs=args.refine_epochs, steps=args.steps, log=log)
     with open(args.out, 'w') as f:
         json.dump(result, f, indent=1, default=float)
     print(json.dumps({k: v for k, v in result.items() if k not in ('screened', 'unscreened')}, default=float))
