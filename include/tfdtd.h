@@ -157,7 +157,16 @@ typedef struct tfdtd_march {
 
 /* torchfdtd/reversible_cuda_fused.py:259-325 (FusedEHMarching) generalised to the
  * CPML, PEC and source layouts of torchfdtd/cuda_kernels.py:150-259: E and H of one
- * step in one pass over HBM. */
+ * step in one pass over HBM.
+ *
+ * The step's three tile launches (interior, z-face and boundary tiles: disjoint cells, time-n
+ * inputs only) run concurrently: the library forks two side streams of its own from `stream`
+ * with events and joins them back, so the work is ordered on `stream` as one launch would be
+ * and can be captured into a graph.  The side streams (default priority, one pair per device)
+ * are created by the first call on a device made outside stream capture; calls captured
+ * before that run the three launches one after another on `stream`.  Calls on one device
+ * from several host threads at once must be serialised by the caller.  The same holds for
+ * tfdtd_batch_march. */
 int tfdtd_yee_step_march(const tfdtd_march* step, void* stream);
 
 /* torchfdtd/cuda_kernels.py:349-365 (FusedYeeCUDA.update_E / update_H), with the

@@ -25,6 +25,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 
 #include "../../include/tfdtd.h"
 #include "common.h"
@@ -490,6 +491,12 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
     }
 }
 
+#ifndef TFDTD_SERIAL_VARIANTS
+constexpr bool kConcurrentVariants = true;
+#else
+constexpr bool kConcurrentVariants = false;
+#endif
+
 #define TFDTD_TILE_BOUNDS \
     __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ? (RW <= 2 ? 3 : 2) : CPML == 4 ? (RW <= 2 ? 4 : 3) : (RW <= 2 ? 6 : 4), 8)))
 
@@ -600,9 +607,18 @@ int layout_march(const tfdtd_march* d, March<real>* a) {
     const int chunk = d->chunk_x > 0 ? d->chunk_x : d->nx;
     a->chunk = chunk;
     a->ytiles = (d->ny + TY - 1) / TY;
-    // Shift the z tiles so one tile edge meets the end of the low z slabs (E and H).
+    // Shift the z tiles so one tile edge meets the end of the low z slabs (E and H), when that
+    // lowers the cost of a z column: slab-free tiles weigh 10, tiles with z slab lanes 17 (their
+    // measured relative durations at 512^3).  A short z axis keeps one unshifted tile.
     const int zend = std::max(a->ef.face[2][0].hi, a->hf.face[2][0].hi);
-    a->zshift = zend > 0 ? (63 - zend % 63) % 63 : 0;
+    auto column_cost = [&](int shift) {
+        const int tiles = (d->nz + shift + 62) / 63;
+        int lo, hi;
+        free_range(tiles, 63, 1, d->nz, a->ef.face[2], a->hf.face[2], &lo, &hi, shift);
+        return 10 * (hi - lo) + 17 * (tiles - (hi - lo));
+    };
+    const int aligned = zend > 0 ? (63 - zend % 63) % 63 : 0;
+    a->zshift = column_cost(aligned) < column_cost(0) ? aligned : 0;
     a->ztiles = (d->nz + a->zshift + 62) / 63;
     const int chunks = (d->nx + chunk - 1) / chunk;
     const long long blocks = (long long)chunks * a->ytiles * a->ztiles;
@@ -621,26 +637,87 @@ int layout_march(const tfdtd_march* d, March<real>* a) {
     return TFDTD_OK;
 }
 
+// Two side streams per device for the concurrent variant launches, with the events that fork them
+// from and join them back to the caller's stream.  Created on first use outside stream capture and
+// kept for the life of the process; every stream has the default priority.
+struct Side {
+    hipStream_t stream[2];
+    hipEvent_t fork, join[2];
+};
+
+Side* side_streams(hipStream_t caller) {
+    static std::mutex lock;
+    static Side* sides[64] = {};
+    int device = -1;
+    if (hipGetDevice(&device) != hipSuccess || device < 0 || device >= 64) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> guard(lock);
+    if (sides[device]) return sides[device];
+    hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(caller, &capture) != hipSuccess || capture != hipStreamCaptureStatusNone) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    Side* side = new Side;
+    bool ok = hipEventCreateWithFlags(&side->fork, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; k < 2 && ok; ++k)
+        ok = hipStreamCreateWithFlags(&side->stream[k], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&side->join[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        delete side;     // a failed creation leaves the launches serial
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return sides[device] = side;
+}
+
+// The three variants write disjoint cells and read only time-n buffers, so they may run at once:
+// the z-face and boundary tiles (latency-bound) overlap the interior tiles (bandwidth-bound).
+// Without side streams (first use under capture) they run one after another on the caller's stream.
+// launch(v, s) starts variant v (0 interior, 1 z faces, 2 boundary) on stream s.
+template <typename Launch>
+int launch_variants(const int blocks[3], hipStream_t stream, Launch&& launch) {
+    int status = TFDTD_OK;
+    auto run = [&](int v, hipStream_t s) {
+        if (blocks[v] > 0 && !status) status = launch(v, s);
+    };
+    auto check = [&](hipError_t e, const char* what) {
+        if (!status && e != hipSuccess) status = tfdtd_fail(TFDTD_EHIP, "march: %s: %s", what, hipGetErrorString(e));
+    };
+    Side* side = kConcurrentVariants && (blocks[1] > 0 || blocks[2] > 0) ? side_streams(stream) : nullptr;
+    if (!side) {
+        for (int v = 0; v < 3; ++v) run(v, stream);
+        return status;
+    }
+    check(hipEventRecord(side->fork, stream), "fork");
+    for (int k = 0; k < 2; ++k) check(hipStreamWaitEvent(side->stream[k], side->fork, 0), "fork wait");
+    run(2, side->stream[0]);
+    run(1, side->stream[1]);
+    run(0, stream);
+    for (int k = 0; k < 2; ++k) {
+        check(hipEventRecord(side->join[k], side->stream[k]), "join");
+        check(hipStreamWaitEvent(stream, side->join[k], 0), "join wait");
+    }
+    return status;
+}
+
 template <typename real, int RW>
 int launch_tile(const tfdtd_march* d, hipStream_t stream) {
     March<real> a;
     int status = layout_march<real, RW>(d, &a);
     if (status) return status;
-    auto launch = [&](auto kernel, int count, const char* what) {
-        if (count <= 0 || status) return;
-        hipLaunchKernelGGL(kernel, dim3((unsigned)count), dim3(256), 0, stream, a);
-        status = tfdtd_check_launch(what);
-    };
-    if (d->eps_stride == 3) {
-        launch(tile_kernel<real, RW, true, 0>, a.blocks[0], "march interior");
-        launch(tile_kernel<real, RW, true, 4>, a.blocks[1], "march z faces");
-        launch(tile_kernel<real, RW, true, 7>, a.blocks[2], "march boundary");
-    } else {
-        launch(tile_kernel<real, RW, false, 0>, a.blocks[0], "march interior");
-        launch(tile_kernel<real, RW, false, 4>, a.blocks[1], "march z faces");
-        launch(tile_kernel<real, RW, false, 7>, a.blocks[2], "march boundary");
-    }
-    return status;
+    const bool eps3 = d->eps_stride == 3;
+    return launch_variants(a.blocks, stream, [&](int v, hipStream_t s) {
+        auto go = [&](auto kernel, const char* what) {
+            hipLaunchKernelGGL(kernel, dim3((unsigned)a.blocks[v]), dim3(256), 0, s, a);
+            return tfdtd_check_launch(what);
+        };
+        if (v == 0) return eps3 ? go(tile_kernel<real, RW, true, 0>, "march interior") : go(tile_kernel<real, RW, false, 0>, "march interior");
+        if (v == 1) return eps3 ? go(tile_kernel<real, RW, true, 4>, "march z faces") : go(tile_kernel<real, RW, false, 4>, "march z faces");
+        return eps3 ? go(tile_kernel<real, RW, true, 7>, "march boundary") : go(tile_kernel<real, RW, false, 7>, "march boundary");
+    });
 }
 
 template <typename real, int RW>
@@ -666,16 +743,16 @@ int pack_marches(const tfdtd_march* d, int count, void* host_table, size_t bytes
 template <typename real, int RW, bool EPS3>
 int launch_batch_tiles(const void* table, const int32_t* prefix, int count, const int32_t* totals, hipStream_t stream) {
     const March<real>* t = static_cast<const March<real>*>(table);
-    int status = TFDTD_OK;
-    auto launch = [&](auto kernel, int v, const char* what) {
-        if (totals[v] <= 0 || status) return;
-        hipLaunchKernelGGL(kernel, dim3((unsigned)totals[v]), dim3(256), 0, stream, t, prefix + v * (count + 1), count);
-        status = tfdtd_check_launch(what);
-    };
-    launch(batch_tile_kernel<real, RW, EPS3, 0>, 0, "batch march interior");
-    launch(batch_tile_kernel<real, RW, EPS3, 4>, 1, "batch march z faces");
-    launch(batch_tile_kernel<real, RW, EPS3, 7>, 2, "batch march boundary");
-    return status;
+    const int blocks[3] = {totals[0], totals[1], totals[2]};
+    return launch_variants(blocks, stream, [&](int v, hipStream_t s) {
+        auto go = [&](auto kernel, const char* what) {
+            hipLaunchKernelGGL(kernel, dim3((unsigned)blocks[v]), dim3(256), 0, s, t, prefix + v * (count + 1), count);
+            return tfdtd_check_launch(what);
+        };
+        if (v == 0) return go(batch_tile_kernel<real, RW, EPS3, 0>, "batch march interior");
+        if (v == 1) return go(batch_tile_kernel<real, RW, EPS3, 4>, "batch march z faces");
+        return go(batch_tile_kernel<real, RW, EPS3, 7>, "batch march boundary");
+    });
 }
 
 template <typename real>

@@ -44,6 +44,8 @@ class StepGraphs:
         with torch.cuda.stream(stream):
             for _ in range(min(warmup, steps)):
                 step()
+            if pair is not None and steps >= 2:
+                pair()      # the pair's first launches happen outside capture too
             for value in states:
                 value.zero_()
         torch.cuda.current_stream().wait_stream(stream)

@@ -211,9 +211,12 @@ def slab_transmission(densities, reference_flux, **kwargs):
     return slab_flux(densities, **kwargs)/reference_flux
 
 
-def train(net, densities, targets, *, epochs=20, batch=128, lr=3e-4, weight_decay=1e-2, seed=0, log=None, cosine=False):
+def train(net, densities, targets, *, epochs=20, batch=128, lr=3e-4, weight_decay=1e-2, seed=0, log=None, cosine=False,
+          mirror=False):
     """AdamW on mean-squared error; densities (n, 64, 64, 8), targets (n, outputs), both on the device.
-    `cosine`: decay the learning rate to zero over the epochs."""
+    `cosine`: decay the learning rate to zero over the epochs.  `mirror`: flip each batch along x
+    and/or y at random -- the centred slab under a normally incident Ex plane source transmits the
+    same flux for a design and its x or y mirror image."""
     gen = torch.Generator(device='cpu').manual_seed(seed)
     opt = torch.optim.AdamW(net.parameters(), lr=lr, weight_decay=weight_decay)
     schedule = torch.optim.lr_scheduler.CosineAnnealingLR(opt, epochs) if cosine else None
@@ -224,8 +227,15 @@ def train(net, densities, targets, *, epochs=20, batch=128, lr=3e-4, weight_deca
         total = 0.
         for i in range(0, n, batch):
             idx = order[i:i+batch].to(densities.device)
+            x = densities[idx]
+            if mirror:
+                flips = torch.randint(0, 4, (1,), generator=gen).item()
+                if flips & 1:
+                    x = x.flip(1)
+                if flips & 2:
+                    x = x.flip(2)
             opt.zero_grad(set_to_none=True)
-            loss = torch.nn.functional.mse_loss(net(densities[idx]), targets[idx])
+            loss = torch.nn.functional.mse_loss(net(x), targets[idx])
             loss.backward()
             opt.step()
             total += loss.item()*len(idx)
@@ -240,6 +250,19 @@ def train(net, densities, targets, *, epochs=20, batch=128, lr=3e-4, weight_deca
 @torch.no_grad()
 def predict(net, densities, batch=512):
     return torch.cat([net(densities[i:i+batch]) for i in range(0, densities.shape[0], batch)])
+
+
+def tilt_figure_of_merit(transmission, wavelengths_um, edge=1.55):
+    """Mean transmission below `edge` minus mean above: an edge-filter figure of merit per design."""
+    t = torch.as_tensor(transmission)
+    w = torch.as_tensor(wavelengths_um, dtype=t.dtype)
+    return t[..., w < edge].mean(-1)-t[..., w >= edge].mean(-1)
+
+
+def match_figure_of_merit(transmission, target):
+    """Minus the RMS difference from a target spectrum: 0 is a perfect match."""
+    t = torch.as_tensor(transmission)
+    return -((t-torch.as_tensor(target, dtype=t.dtype))**2).mean(-1).sqrt()
 
 
 def band_figure_of_merit(transmission, wavelengths_um, band=(1.50, 1.60)):
@@ -353,10 +376,28 @@ def generate(start, count, *, seed=2026, steps=2000, frequencies=32, cohort=64, 
     return flux/reference, reference
 
 
+def study_objectives(data, held, wavelengths, band=(1.50, 1.60)):
+    """The design campaigns of the study: (name, spectra -> FOM).  Band-pass (in-band minus out-of-band
+    mean transmission), edge tilt (short minus long half), and matches to the spectra of two held-out
+    designs (reachable targets that the surrogate never saw)."""
+    flat = np.asarray(data[:held])
+    level = flat.mean(1)
+    picks = [int(np.argmin(np.abs(level-np.quantile(level, q)))) for q in (.25, .75)]
+    out = [('bandpass', lambda t: band_figure_of_merit(t.cpu(), wavelengths, band)),
+           ('tilt', lambda t: tilt_figure_of_merit(t.cpu(), wavelengths))]
+    for k in picks:
+        target = torch.as_tensor(flat[k], dtype=torch.float32)
+        out.append((f'match-{k}', lambda t, target=target: match_figure_of_merit(t.cpu().float(), target)))
+    return out
+
+
 def design_study(data, *, seed=2026, epochs=150, batch=128, lr=1e-3, candidates=4096, verify=32, iterations=6,
-                 baseline_iterations=48, refine_epochs=4, steps=2000, frequencies=32, band=(1.50, 1.60), log=print):
-    """Train the surrogate on `data` (transmission [n, 32] of the first n seeded designs), then run the
-    screened and the unscreened design loops with the FDTD solve as the judge."""
+                 baseline_iterations=48, refine_epochs=4, steps=2000, frequencies=32, band=(1.50, 1.60), objectives=None,
+                 log=print):
+    """Train the surrogate on `data` (transmission [n, 32] of the first n seeded designs), then run one
+    screened and one unscreened design campaign per objective with the FDTD solve as the judge.
+    The surrogate is trained once; each campaign starts from that network."""
+    import copy
     import time
     from .workloads import slab_densities
     device = torch.device('cuda')
@@ -367,15 +408,12 @@ def design_study(data, *, seed=2026, epochs=150, batch=128, lr=1e-3, candidates=
     mean, std = targets[held:].mean(0), targets[held:].std(0)
     surrogate = Surrogate(SurrogateNet(seed=seed).to(device), mean, std)
     started = time.perf_counter()
-    history = surrogate.fit(densities[held:], targets[held:], epochs=epochs, batch=batch, lr=lr, weight_decay=1e-4,
-                            seed=seed, cosine=True, log=lambda e, v: log(f'train: epoch {e} mse {v:.5f}'))
+    history = surrogate.fit(densities[held:], targets[held:], epochs=epochs, batch=batch, lr=lr, weight_decay=1e-3,
+                            seed=seed, cosine=True, mirror=True, log=lambda e, v: log(f'train: epoch {e} mse {v:.5f}'))
     train_seconds = time.perf_counter()-started
     predicted = surrogate(densities[:held])
     error = float(((predicted-targets[:held])**2).mean().sqrt())
     wavelengths = slab_wavelengths(frequencies)
-    fom = lambda t: band_figure_of_merit(t.cpu(), wavelengths, band)
-    heldout_fom = fom(targets[:held]).numpy()
-    heldout_fom_error = float(np.sqrt(np.mean((fom(predicted).numpy()-heldout_fom)**2)))
     reference = slab_flux(np.zeros((1, *densities.shape[1:]), np.float32), steps=steps, frequencies=frequencies)[0]
 
     def solve(pool):
@@ -383,19 +421,29 @@ def design_study(data, *, seed=2026, epochs=150, batch=128, lr=1e-3, candidates=
                                dtype=torch.float32)
     gen = torch.Generator(device='cpu').manual_seed(seed)
     sample = torch.randperm(n-held, generator=gen)[:1024].to(device)+held
-    screened, errors, best = screen(surrogate, solve, fom, iterations=iterations, candidates=candidates, verify=verify,
-                                    seed=seed+1, replay=(densities[sample], targets[sample]), refine_epochs=refine_epochs,
-                                    log=log)
-    plain, plain_best = unscreened(solve, fom, iterations=baseline_iterations, per_iteration=verify, seed=seed+1)
-    target = screened[-1][1]
-    reached = next((solves for solves, value in plain if value >= target), None)
-    return dict(samples=n, parameters=surrogate.net.parameter_count(), epochs=epochs, train_seconds=train_seconds,
-                final_train_mse=history[-1], heldout_rmse=error, heldout_fom_std=float(heldout_fom.std()),
-                heldout_fom_rmse=heldout_fom_error, refine_epochs=refine_epochs, screened=screened, unscreened=plain,
-                screened_solves=screened[-1][0], screened_best=target,
-                unscreened_solves_to_match=reached, unscreened_best=plain[-1][1],
-                solves_ratio=(reached/screened[-1][0]) if reached else None,
-                surrogate_fom_rmse=float(np.sqrt(np.mean(errors**2))), band_um=list(band))
+    campaigns = {}
+    for name, fom in objectives or study_objectives(data, held, wavelengths, band):
+        truth, guess = fom(targets[:held]).numpy(), fom(predicted).numpy()
+        r2 = float(1-np.mean((guess-truth)**2)/np.var(truth))
+        log(f'campaign {name}: held-out FOM R^2 {r2:.3f}')
+        local = Surrogate(copy.deepcopy(surrogate.net), mean, std)
+        screened, errors, _ = screen(local, solve, fom, iterations=iterations, candidates=candidates, verify=verify,
+                                     seed=seed+1, replay=(densities[sample], targets[sample]), refine_epochs=refine_epochs,
+                                     log=log)
+        plain, _ = unscreened(solve, fom, iterations=baseline_iterations, per_iteration=verify, seed=seed+1)
+        target = screened[-1][1]
+        reached = next((solves for solves, value in plain if value >= target), None)
+        campaigns[name] = dict(heldout_fom_r2=r2, heldout_fom_std=float(truth.std()), screened=screened, unscreened=plain,
+                               screened_solves=screened[-1][0], screened_best=target, unscreened_solves_to_match=reached,
+                               unscreened_best=plain[-1][1],
+                               solves_ratio=(reached/screened[-1][0]) if reached else None,
+                               solves_ratio_at_least=None if reached else plain[-1][0]/screened[-1][0],
+                               surrogate_fom_rmse=float(np.sqrt(np.mean(errors**2))))
+        log(f'campaign {name}: screened best {target:.4f} in {screened[-1][0]} solves; unscreened matched at '
+            f'{reached} (best {plain[-1][1]:.4f} in {plain[-1][0]})')
+    return dict(samples=n, training_solves=n, parameters=surrogate.net.parameter_count(), epochs=epochs,
+                train_seconds=train_seconds, final_train_mse=history[-1], heldout_rmse=error,
+                refine_epochs=refine_epochs, campaigns=campaigns)
 
 
 def _main(argv=None):
@@ -430,7 +478,10 @@ def _main(argv=None):
                           refine_epochs=args.refine_epochs, steps=args.steps, log=log)
     with open(args.out, 'w') as f:
         json.dump(result, f, indent=1, default=float)
-    print(json.dumps({k: v for k, v in result.items() if k not in ('screened', 'unscreened')}, default=float))
+    summary = {k: v for k, v in result.items() if k != 'campaigns'}
+    summary['campaigns'] = {name: {k: v for k, v in c.items() if k not in ('screened', 'unscreened')}
+                            for name, c in result['campaigns'].items()}
+    print(json.dumps(summary, default=float))
     return 0
 
 
