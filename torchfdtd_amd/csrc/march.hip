@@ -84,25 +84,6 @@ __device__ __forceinline__ real cpml(real d, const Family<real>& F, const Hit& h
     return d * ik + m;
 }
 
-// Per-lane z-slab membership packed in one int: -1 none, else face << 20 | row.
-template <typename real>
-__device__ __forceinline__ int zcode(const Family<real>& F, int z) {
-    const Hit h = find_hit(F, 2, z);
-    return h.face < 0 ? -1 : (h.face << 20) | h.q;
-}
-
-template <typename real>
-__device__ __forceinline__ Hit zexpand(const Family<real>& F, int code) {
-    if (code < 0) return Hit{-1, 0, 1, 0u, 0u, 0u};
-    const int face = code >> 20, q = code & 0xfffff;
-    // Select values, not addresses: a runtime-chosen reference into the descriptor would force
-    // a private copy of it.
-    const Face& f0 = F.face[2][0];
-    const Face& f1 = F.face[2][1];
-    return Hit{face, q, face ? f1.hi - f1.lo : f0.hi - f0.lo, face ? f1.psi[0] : f0.psi[0], face ? f1.psi[1] : f0.psi[1],
-               face ? f1.coef : f0.coef};
-}
-
 template <typename real>
 struct ZCoef {         // per-lane z-slab coefficients (shared by the two z terms of a family)
     real b, c, ik;
@@ -395,10 +376,14 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
     // Row slots: k < RW the wavefront's rows; k == RW the halo row y0+TY, computed by wavefront 0 only.
     auto row = [&](int k) { return k < RW ? y0 + wave * RW + k : y0 + TY; };
     auto computed = [&](int k) { return zin && row(k) < ny && (k < RW || wave == 0); };
-    const int ez_code = zin ? zcode(a.ef, z) : -1;
-    const int hz_code = zin ? zcode(a.hf, z) : -1;
-    const ZCoef<real> ez_coef = lane_coefficients(a.ef, zexpand(a.ef, ez_code));
-    const ZCoef<real> hz_coef = lane_coefficients(a.hf, zexpand(a.hf, hz_code));
+    // The lane's z-slab membership and coefficients, once for the whole march (z is fixed per lane).
+    // Read inside the loop, the descriptor fields would be reloaded every row: the loop's stores may
+    // alias the descriptor table as far as the compiler knows.
+    const Hit none{-1, 0, 1, 0u, 0u, 0u};
+    const Hit ez_hit = (CPML & 4) && zin ? find_hit(a.ef, 2, z) : none;
+    const Hit hz_hit = (CPML & 4) && zin ? find_hit(a.hf, 2, z) : none;
+    const ZCoef<real> ez_coef = lane_coefficients(a.ef, ez_hit);
+    const ZCoef<real> hz_coef = lane_coefficients(a.hf, hz_hit);
     // Element tid of an sH row: component tid % 3 of column z0 - 1 + tid / 3 (clamped into the grid).
     const unsigned row_offset = 3u * (unsigned)min(max(z0 - 1 + tid / 3, 0), nz - 1) + (unsigned)(tid % 3);
 
@@ -452,7 +437,7 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
                     const real* pzm = p - 3;
                     const bool owned = zown && k < RW && x < xb;
                     const Hit ey_hit = find_hit(a.ef, 1, y);
-                    en[k] = e_cell<real, CPML>(a, x, y, z, owned, ex_hit, ey_hit, zexpand(a.ef, ez_code), ez_coef, hn[k], Hc[k],
+                    en[k] = e_cell<real, CPML>(a, x, y, z, owned, ex_hit, ey_hit, ez_hit, ez_coef, hn[k], Hc[k],
                                                V3<real>{pym[0], pym[1], pym[2]}, pzm[0], pzm[1], eo[k], inv[k]);
                     add_sources(a.se, a.nsrc_e, a.ebox_lo, a.ebox_hi, n, a.source_samples, x, y, z, en[k].x, en[k].y, en[k].z);
                     if (owned) store3(e_out, 3u * ((unsigned)x * plane + (unsigned)y * nz + (unsigned)z), en[k]);
@@ -470,7 +455,7 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
                     const real* pyn = p + 3 * EC;
                     const real* pzn = p + 3;
                     const Hit hy_hit = find_hit(a.hf, 1, y);
-                    V3<real> out = h_cell<real, CPML>(a, xh, y, z, true, hcoef, hx_hit, hy_hit, zexpand(a.hf, hz_code), hz_coef, e,
+                    V3<real> out = h_cell<real, CPML>(a, xh, y, z, true, hcoef, hx_hit, hy_hit, hz_hit, hz_coef, e,
                                                       has_e ? en[k] : e, V3<real>{pyn[0], pyn[1], pyn[2]}, pzn[0], pzn[1], Hc[k]);
                     add_sources(a.sh, a.nsrc_h, a.hbox_lo, a.hbox_hi, n, a.source_samples, xh, y, z, out.x, out.y, out.z);
                     store3(h_out, 3u * ((unsigned)xh * plane + (unsigned)y * nz + (unsigned)z), out);
@@ -497,8 +482,14 @@ constexpr bool kConcurrentVariants = true;
 constexpr bool kConcurrentVariants = false;
 #endif
 
-#define TFDTD_TILE_BOUNDS \
-    __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ? (RW <= 2 ? 3 : 2) : CPML == 4 ? (RW <= 2 ? 4 : 3) : (RW <= 2 ? 6 : 4), 8)))
+// Occupancy targets (waves per SIMD), measured at 512^3: FP32 tiles of 8 rows run the interior at 6
+// and the z-face and boundary tiles at 5; FP64 and 16-row tiles hold twice the values per lane.
+#define TFDTD_TILE_BOUNDS                                                                          \
+    __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(                                      \
+        CPML == 0 ? (RW <= 2 ? 6 : 4)                                                                \
+        : sizeof(real) == 4 ? (RW <= 2 ? 5 : CPML == 7 ? 2 : 3)                                      \
+                            : (RW <= 2 ? 3 : 2),                                                     \
+        8)))
 
 template <typename real, int RW, bool EPS3, int CPML>
 __global__ TFDTD_TILE_BOUNDS void tile_kernel(const March<real> a) {
