@@ -33,6 +33,12 @@
 
 namespace {
 
+#ifndef TFDTD_STAGE_REGISTERS
+constexpr bool kStageDMA = true;     // H planes staged into LDS by direct loads, double-buffered
+#else
+constexpr bool kStageDMA = false;
+#endif
+
 template <typename real>
 struct MSource {
     int comp;
@@ -314,7 +320,11 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
     constexpr int HR = TY + 2, HC = TZ + 2;    // sH: rows y0-1 .. y0+TY, columns z0-1 .. z0+63
     constexpr int EC = TZ + 1;                 // sE: rows y0 .. y0+TY, columns z0 .. z0+63
     constexpr int NC = RW + 1;                 // row slots per lane: RW owned + the halo row (wavefront 0)
-    __shared__ real sH[HR * HC * 3];
+    constexpr int SH = HR * HC * 3;            // reals of one staged H plane
+    // Two staging buffers as two LDS objects: the compiler then knows that reading one does not wait
+    // for the direct loads into the other.
+    __shared__ real sH0[SH];
+    __shared__ real sH1[kStageDMA ? SH : 1];
     __shared__ real sE[(TY + 1) * EC * 3];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     int ty, tz, chunk;
@@ -386,6 +396,27 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
     const ZCoef<real> hz_coef = lane_coefficients(a.hf, hz_hit);
     // Element tid of an sH row: component tid % 3 of column z0 - 1 + tid / 3 (clamped into the grid).
     const unsigned row_offset = 3u * (unsigned)min(max(z0 - 1 + tid / 3, 0), nz - 1) + (unsigned)(tid % 3);
+    // Staging of H(x) rows y0-1 .. y0+TY straight into LDS buffer x & 1 (no registers), in 4-byte words:
+    // word e of the flattened plane belongs to element e / WPE, which is row el / (3 HC) and column
+    // element el % (3 HC), clamped into the grid like the register path.  Wavefront w issues segments
+    // w, w + 4, ... of 64 consecutive words; a lane whose word lies past the plane stays inactive.
+    constexpr int WPE = (int)(sizeof(real) / 4), SHW = SH * WPE;
+    auto stage = [&](int x, real* into) {
+        unsigned* buffer = reinterpret_cast<unsigned*>(into);
+        const unsigned base = (unsigned)x * plane;
+        for (int seg = wave; seg * 64 < SHW; seg += 4) {
+            const int e = seg * 64 + lane;
+            if (e < SHW) {
+                const int el = e / WPE, part = e - el * WPE;
+                const int r = el / (3 * HC), c = el - r * (3 * HC);
+                const int y = min(max(y0 - 1 + r, 0), ny - 1);
+                const unsigned off = 3u * (unsigned)min(max(z0 - 1 + c / 3, 0), nz - 1) + (unsigned)(c % 3);
+                const unsigned* word = reinterpret_cast<const unsigned*>(h_in + 3u * (base + (unsigned)y * nz) + off) + part;
+                __builtin_amdgcn_global_load_lds(global(word), (__attribute__((address_space(3))) void*)(buffer + seg * 64),
+                                                 4, 0, 0);
+            }
+        }
+    };
 
     V3<real> Hc[NC];   // H(x-1) of each slot
 #pragma unroll
@@ -393,7 +424,8 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
         Hc[k] = zero3<real>();
         if (xa > 0) Hc[k] = load3(h_in, 3u * ((unsigned)(xa - 1) * plane + (unsigned)min(row(k), ny - 1) * nz + zc));
     }
-    for (int x = xa; x <= xb; ++x) {
+    // One plane x: sHx holds H(x) (staged by the previous plane, or before the march), sHn receives H(x+1).
+    auto march_plane = [&](const int x, real* const sHx, real* const sHn) {
         const bool has_e = x < nx;
         const bool has_h = x > xa;
         V3<real> eo[NC], inv[NC];
@@ -401,11 +433,11 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
             const unsigned base = (unsigned)x * plane;
             // Row r of sH is 3*HC contiguous reals in memory (clamped at the grid edges): lane t < 3*HC
             // copies element t of every row.
-            if (tid < 3 * HC) {
+            if (!kStageDMA && tid < 3 * HC) {
 #pragma unroll
                 for (int r = 0; r < HR; ++r) {
                     const int y = min(max(y0 - 1 + r, 0), ny - 1);
-                    sH[r * 3 * HC + tid] = global(h_in)[3u * (base + (unsigned)y * nz) + row_offset];
+                    sHx[r * 3 * HC + tid] = global(h_in)[3u * (base + (unsigned)y * nz) + row_offset];
                 }
             }
 #pragma unroll
@@ -420,7 +452,10 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
                 }
             }
         }
-        __syncthreads();
+        __syncthreads();   // also completes the staging of H(x) into sHx
+        // The next plane's H rows go to the other buffer while this plane computes; it was last read
+        // before the barrier above.
+        if (kStageDMA && x + 1 <= xb && x + 1 < nx) stage(x + 1, sHn);
         V3<real> en[NC], hn[NC];
         const Hit ex_hit = find_hit(a.ef, 0, x);
         const Hit hx_hit = find_hit(a.hf, 0, x - 1);
@@ -430,7 +465,7 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
                 en[k] = zero3<real>();
                 const int y = row(k);
                 const int r = y - y0 + 1;
-                const real* p = sH + 3 * (r * HC + lane + 1);
+                const real* p = sHx + 3 * (r * HC + lane + 1);
                 hn[k] = V3<real>{p[0], p[1], p[2]};
                 if (computed(k)) {
                     const real* pym = p - 3 * HC;
@@ -462,7 +497,15 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
                 }
             }
         }
-        __syncthreads();
+        if (kStageDMA) {
+            // Every wavefront's reads of sE (and of sHx) are complete before sE is overwritten.  A plain
+            // barrier: a __syncthreads here would also wait for the staging of the next plane.
+            __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0): this wavefront's LDS reads have returned
+            __builtin_amdgcn_s_barrier();
+            __builtin_amdgcn_sched_barrier(0);
+        } else {
+            __syncthreads();
+        }
         if (has_e) {
 #pragma unroll
             for (int k = 0; k < NC; ++k) {
@@ -473,6 +516,15 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
                 Hc[k] = hn[k];
             }
         }
+    };
+    if constexpr (kStageDMA) {
+        if (xa < nx) stage(xa, sH0);
+        for (int x = xa; x <= xb; x += 2) {
+            march_plane(x, sH0, sH1);
+            if (x + 1 <= xb) march_plane(x + 1, sH1, sH0);
+        }
+    } else {
+        for (int x = xa; x <= xb; ++x) march_plane(x, sH0, sH0);
     }
 }
 
@@ -482,12 +534,13 @@ constexpr bool kConcurrentVariants = true;
 constexpr bool kConcurrentVariants = false;
 #endif
 
-// Occupancy targets (waves per SIMD), measured at 512^3: FP32 tiles of 8 rows run the interior at 6
-// and the z-face and boundary tiles at 5; FP64 and 16-row tiles hold twice the values per lane.
+// Occupancy targets (waves per SIMD), measured at 512^3 with the H planes staged by direct loads:
+// FP32 tiles of 8 rows ask 8 for the interior (it compiles to 73 VGPRs, i.e. 6) and 6 for the z-face
+// and boundary tiles (80 VGPRs, a few spilled); FP64 and 16-row tiles hold twice the values per lane.
 #define TFDTD_TILE_BOUNDS                                                                          \
     __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(                                      \
-        CPML == 0 ? (RW <= 2 ? 6 : 4)                                                                \
-        : sizeof(real) == 4 ? (RW <= 2 ? 5 : CPML == 7 ? 2 : 3)                                      \
+        CPML == 0 ? (RW <= 2 ? 8 : 4)                                                                \
+        : sizeof(real) == 4 ? (RW <= 2 ? 6 : CPML == 7 ? 2 : 3)                                      \
                             : (RW <= 2 ? 3 : 2),                                                     \
         8)))
 
