@@ -116,7 +116,7 @@ class Launch:
     """Kernel configuration of a cohort: 'split' (E and H launches) or 'march' (one-pass tiles)."""
     kernel: str = 'march'
     tile_y: int = 8
-    chunk_x: int = 8
+    chunk_x: int = 6
 
     def __post_init__(self):
         if self.kernel not in ('split', 'march'):
@@ -130,8 +130,8 @@ class Launch:
         return dict(kernel=self.kernel, tile_y=self.tile_y, chunk_x=self.chunk_x)
 
 
-DEFAULT_CANDIDATES = (Launch('split'), Launch('march', 8, 4), Launch('march', 8, 8), Launch('march', 8, 16),
-                      Launch('march', 8, 0), Launch('march', 16, 8), Launch('march', 16, 16))
+DEFAULT_CANDIDATES = (Launch('split'), Launch('march', 8, 4), Launch('march', 8, 6), Launch('march', 8, 8),
+                      Launch('march', 8, 16), Launch('march', 8, 0), Launch('march', 16, 6), Launch('march', 16, 16))
 
 
 def _cases(cases):

@@ -298,25 +298,27 @@ __device__ __forceinline__ V3<real> h_cell(const March<real>& a, int x, int y, i
     return V3<real>{h.x - coefficient * cx, h.y - coefficient * cy, h.z - coefficient * cz};
 }
 
-// One workgroup (4 wavefronts, 256 lanes) marches a tile of TY x 63 (y, z) cells, TY = 4*RW, over
-// a chunk of x planes.  Wavefront w owns rows y0 + w*RW .. y0 + w*RW + RW-1 (a wave-uniform y); lane l
-// covers column z0 + l, lanes 0..62 own their cells and lane 63 recomputes E of the next tile's first
-// column.  Per plane x:
-//   A. the workgroup stages H(x) on rows y0-1 .. y0+TY and columns z0-1 .. z0+63 in LDS (sH), and
-//      every lane loads E(x) and 1/eps of its cells into registers;
-//   B. barrier;
-//   C. E(x) of the tile's rows (and, in wavefront 0, of the halo row y0+TY) from sH and the carried
-//      H(x-1);
+// One workgroup (W = 4 or 8 wavefronts) marches a tile of TY x 63 (y, z) cells, TY = 2W, over a chunk
+// of x planes.  Wavefront w owns rows y0 + 2w, y0 + 2w + 1 (a wave-uniform y); lane l covers column
+// z0 + l, lanes 0..62 own their cells and lane 63 recomputes E of the next tile's first column.
+// Per plane x:
+//   A. every lane loads E(x) and 1/eps of its cells into registers; H(x) on rows y0-1 .. y0+TY and
+//      columns z0-1 .. z0+63 is already on its way into an LDS buffer (sH0 / sH1, alternating);
+//   B. barrier (H(x) staged);  then the direct loads of H(x+1) into the other buffer start;
+//   C. E(x) of the tile's rows (and, in wavefront 0, of the halo row y0+TY) from the staged H(x) and
+//      the carried H(x-1);
 //   D. H(x-1) of the owned cells from E(x-1) in LDS (sE), E(x) in registers and the carried H(x-1);
-//   E. barrier;  F. E(x) into sE, H(x) becomes the carried H(x-1).
+//   E. barrier (LDS reads done; the staging of H(x+1) continues);  F. E(x) into sE, H(x) becomes the
+//      carried H(x-1).
 // Cells outside the grid read clamped in-grid addresses and are never computed or stored.  z tiles
 // start at tz*63 - zshift, chosen so a tile edge meets the end of the low z slab.
 // CPML: bit 0 x, bit 1 y, bit 2 z slabs compiled in.  Variant 0 runs the tiles free of every slab,
 // variant 4 the tiles that meet only z slabs, variant 7 the rest.
 // w is the workgroup's index among the G workgroups of its variant and grid.
-template <typename real, int RW, bool EPS3, int CPML>
+template <typename real, int W, bool EPS3, int CPML>
 __device__ __forceinline__ void tile_body(const March<real>& a, const int w, const int G) {
-    constexpr int TY = 4 * RW, TZ = 63;
+    constexpr int RW = 2;                      // rows per wavefront
+    constexpr int TY = W * RW, TZ = 63;
     constexpr int HR = TY + 2, HC = TZ + 2;    // sH: rows y0-1 .. y0+TY, columns z0-1 .. z0+63
     constexpr int EC = TZ + 1;                 // sE: rows y0 .. y0+TY, columns z0 .. z0+63
     constexpr int NC = RW + 1;                 // row slots per lane: RW owned + the halo row (wavefront 0)
@@ -404,7 +406,7 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
     auto stage = [&](int x, real* into) {
         unsigned* buffer = reinterpret_cast<unsigned*>(into);
         const unsigned base = (unsigned)x * plane;
-        for (int seg = wave; seg * 64 < SHW; seg += 4) {
+        for (int seg = wave; seg * 64 < SHW; seg += W) {
             const int e = seg * 64 + lane;
             if (e < SHW) {
                 const int el = e / WPE, part = e - el * WPE;
@@ -538,20 +540,18 @@ constexpr bool kConcurrentVariants = false;
 // FP32 tiles of 8 rows ask 8 for the interior (it compiles to 73 VGPRs, i.e. 6) and 6 for the z-face
 // and boundary tiles (80 VGPRs, a few spilled); FP64 and 16-row tiles hold twice the values per lane.
 #define TFDTD_TILE_BOUNDS                                                                          \
-    __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(                                      \
-        CPML == 0 ? (RW <= 2 ? 8 : 4)                                                                \
-        : sizeof(real) == 4 ? (RW <= 2 ? 6 : CPML == 7 ? 2 : 3)                                      \
-                            : (RW <= 2 ? 3 : 2),                                                     \
+    __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(                                   \
+        CPML == 0 ? 8 : sizeof(real) == 4 ? 6 : 3,                                                   \
         8)))
 
-template <typename real, int RW, bool EPS3, int CPML>
+template <typename real, int W, bool EPS3, int CPML>
 __global__ TFDTD_TILE_BOUNDS void tile_kernel(const March<real> a) {
-    tile_body<real, RW, EPS3, CPML>(a, blockIdx.x, gridDim.x);
+    tile_body<real, W, EPS3, CPML>(a, blockIdx.x, gridDim.x);
 }
 
 // Many grids in one launch per variant.  Grid g owns workgroups [prefix[g], prefix[g+1]), a multiple
 // of 8 so that its tiles keep the XCD assignment of a launch of their own; the padding exits at once.
-template <typename real, int RW, bool EPS3, int CPML>
+template <typename real, int W, bool EPS3, int CPML>
 __global__ TFDTD_TILE_BOUNDS void batch_tile_kernel(const March<real>* __restrict__ table, const int* __restrict__ prefix,
                                                     int grids) {
     const int b = blockIdx.x;
@@ -565,7 +565,7 @@ __global__ TFDTD_TILE_BOUNDS void batch_tile_kernel(const March<real>* __restric
     const int count = a.blocks[CPML == 0 ? 0 : CPML == 4 ? 1 : 2];
     const int w = b - prefix[lo];
     if (w >= count) return;
-    tile_body<real, RW, EPS3, CPML>(a, w, count);
+    tile_body<real, W, EPS3, CPML>(a, w, count);
 }
 
 #undef TFDTD_TILE_BOUNDS
@@ -643,11 +643,11 @@ inline void free_range(int tiles, int size, int e_extra, int n, const Face* ef, 
 }
 
 // Tile layout of one grid: chunking, z shift, the slab-free boxes and the workgroups of each variant.
-template <typename real, int RW>
+template <typename real, int W>
 int layout_march(const tfdtd_march* d, March<real>* a) {
     const int status = make_march(d, a);
     if (status) return status;
-    constexpr int TY = 4 * RW;
+    constexpr int TY = 2 * W;
     const int chunk = d->chunk_x > 0 ? d->chunk_x : d->nx;
     a->chunk = chunk;
     a->ytiles = (d->ny + TY - 1) / TY;
@@ -747,24 +747,24 @@ int launch_variants(const int blocks[3], hipStream_t stream, Launch&& launch) {
     return status;
 }
 
-template <typename real, int RW>
+template <typename real, int W>
 int launch_tile(const tfdtd_march* d, hipStream_t stream) {
     March<real> a;
-    int status = layout_march<real, RW>(d, &a);
+    int status = layout_march<real, W>(d, &a);
     if (status) return status;
     const bool eps3 = d->eps_stride == 3;
     return launch_variants(a.blocks, stream, [&](int v, hipStream_t s) {
         auto go = [&](auto kernel, const char* what) {
-            hipLaunchKernelGGL(kernel, dim3((unsigned)a.blocks[v]), dim3(256), 0, s, a);
+            hipLaunchKernelGGL(kernel, dim3((unsigned)a.blocks[v]), dim3(64 * W), 0, s, a);
             return tfdtd_check_launch(what);
         };
-        if (v == 0) return eps3 ? go(tile_kernel<real, RW, true, 0>, "march interior") : go(tile_kernel<real, RW, false, 0>, "march interior");
-        if (v == 1) return eps3 ? go(tile_kernel<real, RW, true, 4>, "march z faces") : go(tile_kernel<real, RW, false, 4>, "march z faces");
-        return eps3 ? go(tile_kernel<real, RW, true, 7>, "march boundary") : go(tile_kernel<real, RW, false, 7>, "march boundary");
+        if (v == 0) return eps3 ? go(tile_kernel<real, W, true, 0>, "march interior") : go(tile_kernel<real, W, false, 0>, "march interior");
+        if (v == 1) return eps3 ? go(tile_kernel<real, W, true, 4>, "march z faces") : go(tile_kernel<real, W, false, 4>, "march z faces");
+        return eps3 ? go(tile_kernel<real, W, true, 7>, "march boundary") : go(tile_kernel<real, W, false, 7>, "march boundary");
     });
 }
 
-template <typename real, int RW>
+template <typename real, int W>
 int pack_marches(const tfdtd_march* d, int count, void* host_table, size_t bytes, int32_t* prefix) {
     if (!host_table || bytes < sizeof(March<real>) * (size_t)count) return tfdtd_fail(TFDTD_EINVAL, "batch march: table buffer too small");
     March<real>* table = static_cast<March<real>*>(host_table);
@@ -772,7 +772,7 @@ int pack_marches(const tfdtd_march* d, int count, void* host_table, size_t bytes
     for (int k = 0; k < count; ++k) {
         if (d[k].dtype != d[0].dtype || d[k].tile_y != d[0].tile_y || d[k].eps_stride != d[0].eps_stride)
             return tfdtd_fail(TFDTD_EINVAL, "batch march: descriptor %d differs in dtype, tile_y or eps_stride", k);
-        const int status = layout_march<real, RW>(d + k, table + k);
+        const int status = layout_march<real, W>(d + k, table + k);
         if (status) return status;
         for (int v = 0; v < 3; ++v) {
             prefix[v * (count + 1) + k] = (int32_t)total[v];
@@ -784,35 +784,35 @@ int pack_marches(const tfdtd_march* d, int count, void* host_table, size_t bytes
     return TFDTD_OK;
 }
 
-template <typename real, int RW, bool EPS3>
+template <typename real, int W, bool EPS3>
 int launch_batch_tiles(const void* table, const int32_t* prefix, int count, const int32_t* totals, hipStream_t stream) {
     const March<real>* t = static_cast<const March<real>*>(table);
     const int blocks[3] = {totals[0], totals[1], totals[2]};
     return launch_variants(blocks, stream, [&](int v, hipStream_t s) {
         auto go = [&](auto kernel, const char* what) {
-            hipLaunchKernelGGL(kernel, dim3((unsigned)blocks[v]), dim3(256), 0, s, t, prefix + v * (count + 1), count);
+            hipLaunchKernelGGL(kernel, dim3((unsigned)blocks[v]), dim3(64 * W), 0, s, t, prefix + v * (count + 1), count);
             return tfdtd_check_launch(what);
         };
-        if (v == 0) return go(batch_tile_kernel<real, RW, EPS3, 0>, "batch march interior");
-        if (v == 1) return go(batch_tile_kernel<real, RW, EPS3, 4>, "batch march z faces");
-        return go(batch_tile_kernel<real, RW, EPS3, 7>, "batch march boundary");
+        if (v == 0) return go(batch_tile_kernel<real, W, EPS3, 0>, "batch march interior");
+        if (v == 1) return go(batch_tile_kernel<real, W, EPS3, 4>, "batch march z faces");
+        return go(batch_tile_kernel<real, W, EPS3, 7>, "batch march boundary");
     });
 }
 
 template <typename real>
 int batch_march(const void* table, const int32_t* prefix, int count, int tile_y, int eps_stride, const int32_t* totals,
                 hipStream_t stream) {
-    if (tile_y == 8) return eps_stride == 3 ? launch_batch_tiles<real, 2, true>(table, prefix, count, totals, stream)
-                                            : launch_batch_tiles<real, 2, false>(table, prefix, count, totals, stream);
-    return eps_stride == 3 ? launch_batch_tiles<real, 4, true>(table, prefix, count, totals, stream)
-                           : launch_batch_tiles<real, 4, false>(table, prefix, count, totals, stream);
+    if (tile_y == 8) return eps_stride == 3 ? launch_batch_tiles<real, 4, true>(table, prefix, count, totals, stream)
+                                            : launch_batch_tiles<real, 4, false>(table, prefix, count, totals, stream);
+    return eps_stride == 3 ? launch_batch_tiles<real, 8, true>(table, prefix, count, totals, stream)
+                           : launch_batch_tiles<real, 8, false>(table, prefix, count, totals, stream);
 }
 
 template <typename real>
 int dispatch_march(const tfdtd_march* d, hipStream_t stream) {
     switch (d->tile_y) {
-        case 8: return launch_tile<real, 2>(d, stream);
-        case 16: return launch_tile<real, 4>(d, stream);
+        case 8: return launch_tile<real, 4>(d, stream);    // 4 wavefronts x 2 rows
+        case 16: return launch_tile<real, 8>(d, stream);   // 8 wavefronts x 2 rows
         default: return tfdtd_fail(TFDTD_EINVAL, "march: tile_y must be 8 or 16");
     }
 }
@@ -839,11 +839,11 @@ extern "C" int tfdtd_batch_march_pack(const tfdtd_march* steps, int32_t count, v
     const int ty = steps[0].tile_y;
     if (ty != 8 && ty != 16) return tfdtd_fail(TFDTD_EINVAL, "march: tile_y must be 8 or 16");
     if (steps[0].dtype == TFDTD_F32)
-        return ty == 8 ? pack_marches<float, 2>(steps, count, host_table, bytes, block_prefix)
-                       : pack_marches<float, 4>(steps, count, host_table, bytes, block_prefix);
+        return ty == 8 ? pack_marches<float, 4>(steps, count, host_table, bytes, block_prefix)
+                       : pack_marches<float, 8>(steps, count, host_table, bytes, block_prefix);
     if (steps[0].dtype == TFDTD_F64)
-        return ty == 8 ? pack_marches<double, 2>(steps, count, host_table, bytes, block_prefix)
-                       : pack_marches<double, 4>(steps, count, host_table, bytes, block_prefix);
+        return ty == 8 ? pack_marches<double, 4>(steps, count, host_table, bytes, block_prefix)
+                       : pack_marches<double, 8>(steps, count, host_table, bytes, block_prefix);
     return tfdtd_fail(TFDTD_EINVAL, "march: dtype must be float32 or float64");
 }
 

@@ -138,7 +138,7 @@ def pack_table(count_bytes, pack, descriptors, prefix_rows, device):
 class Stepper:
     """Device state of one resident run and its step (the reference's closure, solver.py:776-797)."""
 
-    def __init__(self, project, steps=None, kernel=None, one_pass=None, tile_y=8, chunk_x=8, counter=None, epsilon=None):
+    def __init__(self, project, steps=None, kernel=None, one_pass=None, tile_y=8, chunk_x=6, counter=None, epsilon=None):
         p, r = project, project.region
         self.project = p
         self.steps = r.steps if steps is None else steps
