@@ -317,7 +317,14 @@ def screen(surrogate, solve, fom, *, iterations=6, candidates=4096, verify=32, s
         pool = propose(rng, candidates, best)
         x = torch.as_tensor(pool, device=device)
         predicted = fom(surrogate(x)).cpu().numpy()
-        top = np.argsort(-predicted)[:verify]
+        if best is None:
+            top = np.argsort(-predicted)[:verify]
+        else:
+            # Half the solves go to the best-predicted mutations of the best design (odd pool entries),
+            # half to the best-predicted fresh designs, so the local search always advances.
+            half = verify//2
+            picks = [parity+2*np.argsort(-predicted[parity::2])[:n] for parity, n in ((1, half), (0, verify-half))]
+            top = np.concatenate(picks)
         spectra = solve(pool[top])
         judged = fom(spectra).cpu().numpy()
         solves += verify
@@ -378,11 +385,11 @@ def generate(start, count, *, seed=2026, steps=2000, frequencies=32, cohort=64, 
 
 def study_objectives(data, held, wavelengths, band=(1.50, 1.60)):
     """The design campaigns of the study: (name, spectra -> FOM).  Band-pass (in-band minus out-of-band
-    mean transmission), edge tilt (short minus long half), and matches to the spectra of two held-out
+    mean transmission), edge tilt (short minus long half), and matches to the spectra of four held-out
     designs (reachable targets that the surrogate never saw)."""
     flat = np.asarray(data[:held])
     level = flat.mean(1)
-    picks = [int(np.argmin(np.abs(level-np.quantile(level, q)))) for q in (.25, .75)]
+    picks = [int(np.argmin(np.abs(level-np.quantile(level, q)))) for q in (.25, .5, .75, .9)]
     out = [('bandpass', lambda t: band_figure_of_merit(t.cpu(), wavelengths, band)),
            ('tilt', lambda t: tilt_figure_of_merit(t.cpu(), wavelengths))]
     for k in picks:
@@ -433,7 +440,8 @@ def design_study(data, *, seed=2026, epochs=150, batch=128, lr=1e-3, candidates=
         plain, _ = unscreened(solve, fom, iterations=baseline_iterations, per_iteration=verify, seed=seed+1)
         target = screened[-1][1]
         reached = next((solves for solves, value in plain if value >= target), None)
-        campaigns[name] = dict(heldout_fom_r2=r2, heldout_fom_std=float(truth.std()), screened=screened, unscreened=plain,
+        checkpoints = [(solves, value, next((s for s, v in plain if v >= value), None)) for solves, value in screened]
+        campaigns[name] = dict(checkpoints=checkpoints,heldout_fom_r2=r2, heldout_fom_std=float(truth.std()), screened=screened, unscreened=plain,
                                screened_solves=screened[-1][0], screened_best=target, unscreened_solves_to_match=reached,
                                unscreened_best=plain[-1][1],
                                solves_ratio=(reached/screened[-1][0]) if reached else None,
