@@ -502,9 +502,13 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
         if (kStageDMA) {
             // Every wavefront's reads of sE (and of sHx) are complete before sE is overwritten.  A plain
             // barrier: a __syncthreads here would also wait for the staging of the next plane.
-            __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0): this wavefront's LDS reads have returned
+            // The clobbers keep the compiler from moving LDS accesses across the barrier (the barrier
+            // builtin itself is not a memory operation); lgkmcnt(0): this wavefront's LDS reads have returned.
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_s_waitcnt(0xc07f);
             __builtin_amdgcn_s_barrier();
             __builtin_amdgcn_sched_barrier(0);
+            asm volatile("" ::: "memory");
         } else {
             __syncthreads();
         }
