@@ -68,15 +68,20 @@ def test_random_fields_and_every_cpml_state_bitwise(gpu, precision, layout):
 
 
 @pytest.mark.parametrize('precision', ['float32', 'float64'])
-@pytest.mark.parametrize('layout,tile_y,chunk_x,nz', [('pml3d', 8, 7, 131), ('pml3d', 16, 64, 200), ('pec', 8, 5, 131),
-                                                      ('pml3d', 16, 0, 131), ('pml3d', 8, 9, 200), ('pec', 16, 6, 200)])
-def test_one_pass_step_matches_oracle_bitwise(gpu, precision, layout, tile_y, chunk_x, nz):
+@pytest.mark.parametrize('layout,tile_y,chunk_x,nz,components', [
+    ('pml3d', 8, 7, 131, 3), ('pml3d', 16, 64, 200, 3), ('pec', 8, 5, 131, 3), ('pml3d', 16, 0, 131, 3),
+    ('pml3d', 8, 9, 200, 3), ('pec', 16, 6, 200, 3), ('pml3d', 8, 6, 200, 1), ('pec', 16, 4, 200, 1)])
+def test_one_pass_step_matches_oracle_bitwise(gpu, precision, layout, tile_y, chunk_x, nz, components):
     """The one-pass E+H kernel (ping-pong buffers, LDS tiles with halo recompute, x chunks, CPML-free
-    interior and CPML boundary launches) equals the reference update bit for bit."""
+    interior and CPML boundary launches) equals the reference update bit for bit, with 1/eps per
+    Yee component or per cell."""
     from torchfdtd_amd import Project
     from torchfdtd_amd.solver import Stepper
     region = region_for(layout, precision)
     region.size = (3.2, 7.4, nz/10)     # 32 x 74 x nz cells: partial y and z tiles
+    if components == 1:
+        region.material_sampling = 'cell'
+
     project = Project(region=region)
     run = Stepper(project, steps=40, one_pass=True, tile_y=tile_y, chunk_x=chunk_x)
     g = run.grid
@@ -85,7 +90,8 @@ def test_one_pass_step_matches_oracle_bitwise(gpu, precision, layout, tile_y, ch
     rng = np.random.default_rng(7)
     E = (rng.normal(size=(*scene.shape, 3))*.02).astype(real)
     H = (rng.normal(size=(*scene.shape, 3))*.02).astype(real)
-    inv = rng.uniform(.25, 1, (*scene.shape, 3)).astype(real)
+    inv = rng.uniform(.25, 1, (*scene.shape, components)).astype(real)
+    assert tuple(run.grid.inverse_permittivity.shape) == inv.shape
     ref = OracleGrid(scene, inv.copy())
     ref.E[:], ref.H[:] = E, H
     g.E.copy_(torch.as_tensor(E))
