@@ -26,8 +26,6 @@ typedef unsigned short bf16_bits;
 
 constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8;   // LDS rows padded by 16 bytes
 
-__device__ __forceinline__ float bf16_to_float(bf16_bits v) { return __uint_as_float((unsigned)v << 16); }
-
 // Round to nearest even (finite inputs), as torch's float -> bfloat16 conversion.
 __device__ __forceinline__ bf16_bits float_to_bf16(float f) {
     const unsigned u = __float_as_uint(f);
