@@ -218,7 +218,8 @@ def main():
     torch.cuda.synchronize()
     width = args.graph_width
     graph = StepGraphs(run.step, run.states(), region.steps, width, warmup=1,
-                       pair=run.step_pair if run.one_pass else None)
+                       pair=run.step_pair if run.one_pass else None,
+                       prepare=run.refresh_materials if run.one_pass else None)
     print(f'bench: graphs captured (one-pass kernel: {run.one_pass})', file=sys.stderr, flush=True)
 
     def advance(steps):

@@ -151,8 +151,14 @@ typedef struct tfdtd_march {
     tfdtd_source srcs_h[TFDTD_MAX_SOURCES];
     int64_t source_samples;
     const int64_t* counter;
-    int32_t tile_y;                 /* rows per wavefront: 4, 8 or 16 */
+    int32_t tile_y;                 /* rows per tile: 8 (4 wavefronts) or 16 (8 wavefronts) */
     int32_t chunk_x;                /* planes per workgroup; 0 = the whole x axis */
+    /* Optional per-cell material index, eps_stride 1 only: when non-null, 1/eps of cell i is
+     * material_table[material[i]] (`materials` entries, 1 to 256, of dtype) and the step does not
+     * read inverse_permittivity.  The caller keeps index and table equal to the permittivity. */
+    const uint8_t* material;
+    const void* material_table;
+    int32_t materials;
 } tfdtd_march;
 
 /* torchfdtd/reversible_cuda_fused.py:259-325 (FusedEHMarching) generalised to the

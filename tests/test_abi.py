@@ -61,6 +61,7 @@ int main(void) {
   printf("tfdtd_march_family %zu\n", sizeof(tfdtd_march_family));
   F(tfdtd_march, courant); F(tfdtd_march, e); F(tfdtd_march, h); F(tfdtd_march, nsrc_e); F(tfdtd_march, srcs_h);
   F(tfdtd_march, source_samples); F(tfdtd_march, counter); F(tfdtd_march, chunk_x);
+  F(tfdtd_march, material); F(tfdtd_march, material_table); F(tfdtd_march, materials);
   F(tfdtd_march_family, face); F(tfdtd_march_face, psi); F(tfdtd_march_face, coef);
   printf("tfdtd_norm_grid %zu\n", sizeof(tfdtd_norm_grid));
   F(tfdtd_norm_grid, eps_stride); F(tfdtd_norm_grid, widths); F(tfdtd_norm_grid, psi); F(tfdtd_norm_grid, psi_count);

@@ -70,15 +70,18 @@ def test_random_fields_and_every_cpml_state_bitwise(gpu, precision, layout):
 @pytest.mark.parametrize('precision', ['float32', 'float64'])
 @pytest.mark.parametrize('layout,tile_y,chunk_x,nz,components', [
     ('pml3d', 8, 7, 131, 3), ('pml3d', 16, 64, 200, 3), ('pec', 8, 5, 131, 3), ('pml3d', 16, 0, 131, 3),
-    ('pml3d', 8, 9, 200, 3), ('pec', 16, 6, 200, 3), ('pml3d', 8, 6, 200, 1), ('pec', 16, 4, 200, 1)])
+    ('pml3d', 8, 9, 200, 3), ('pec', 16, 6, 200, 3), ('pml3d', 8, 6, 200, 'index'), ('pec', 16, 4, 200, 1)])
 def test_one_pass_step_matches_oracle_bitwise(gpu, precision, layout, tile_y, chunk_x, nz, components):
     """The one-pass E+H kernel (ping-pong buffers, LDS tiles with halo recompute, x chunks, CPML-free
     interior and CPML boundary launches) equals the reference update bit for bit, with 1/eps per
-    Yee component or per cell."""
+    Yee component, per cell, or per cell through the one-byte material index (seven values)."""
     from torchfdtd_amd import Project
     from torchfdtd_amd.solver import Stepper
     region = region_for(layout, precision)
     region.size = (3.2, 7.4, nz/10)     # 32 x 74 x nz cells: partial y and z tiles
+    indexed = components == 'index'
+    if indexed:
+        components = 1
     if components == 1:
         region.material_sampling = 'cell'
 
@@ -91,6 +94,8 @@ def test_one_pass_step_matches_oracle_bitwise(gpu, precision, layout, tile_y, ch
     E = (rng.normal(size=(*scene.shape, 3))*.02).astype(real)
     H = (rng.normal(size=(*scene.shape, 3))*.02).astype(real)
     inv = rng.uniform(.25, 1, (*scene.shape, components)).astype(real)
+    if indexed:
+        inv = rng.choice(np.unique(inv)[:7], size=inv.shape)
     assert tuple(run.grid.inverse_permittivity.shape) == inv.shape
     ref = OracleGrid(scene, inv.copy())
     ref.E[:], ref.H[:] = E, H
@@ -106,6 +111,7 @@ def test_one_pass_step_matches_oracle_bitwise(gpu, precision, layout, tile_y, ch
     ref.update_E()
     ref.update_H()
     torch.cuda.synchronize()
+    assert run.materials.count == (7 if indexed else 0)
     np.testing.assert_array_equal(g.E.cpu().numpy(), ref.E)
     np.testing.assert_array_equal(g.H.cpu().numpy(), ref.H)
     for got, expected in zip(g.memory_states, ref.memory_states):
@@ -269,3 +275,42 @@ def test_blocked_plane_flush_mid_window(gpu):
     ref = simulate.run(project.model_dump())['planes'][0]
     expected = ref.value[..., [names.index(c) for c in plane.monitor.record_fields]]
     np.testing.assert_allclose(plane.result()['fields']/scale, expected/scale, rtol=1e-12, atol=1e-12)
+
+
+def test_material_index_follows_permittivity_changes_between_graph_replays(gpu):
+    """The one-pass step reads 1/eps through a one-byte material index; changing the permittivity in
+    place between graph replays (and to more values than the index holds, and back) must give the
+    reference update on the new values, with the same captured graphs."""
+    from torchfdtd_amd import Project
+    from torchfdtd_amd.graphs import StepGraphs
+    from torchfdtd_amd.solver import Stepper
+    region = region_for('pml3d', 'float32')
+    region.size = (2.4, 2.6, 13.0)
+    region.material_sampling = 'cell'
+    project = Project(region=region)
+    run = Stepper(project, steps=40, one_pass=True)
+    g = run.grid
+    scene = Scene(project.model_dump())
+    rng = np.random.default_rng(11)
+    E = (rng.normal(size=(*scene.shape, 3))*.02).astype(np.float32)
+    H = (rng.normal(size=(*scene.shape, 3))*.02).astype(np.float32)
+    few = rng.choice(np.float32([1, .5, .25, 1/12]), size=(*scene.shape, 1)).astype(np.float32)
+    many = rng.uniform(.25, 1, (*scene.shape, 1)).astype(np.float32)
+    g.inverse_permittivity.copy_(torch.as_tensor(few))
+    graphs = StepGraphs(run.step, run.states(), 40, 4, pair=run.step_pair, prepare=run.refresh_materials)
+    ref = OracleGrid(scene, few.copy())
+    g.E.copy_(torch.as_tensor(E))
+    g.H.copy_(torch.as_tensor(H))
+    ref.E[:], ref.H[:] = E, H
+    for inv, count in ((few, 4), (many, 0), (few[::-1].copy(), 4)):
+        g.inverse_permittivity.copy_(torch.as_tensor(inv))
+        ref.inverse_permittivity = inv.copy()
+        graphs.replay(4)
+        for _ in range(4):
+            ref.update_E()
+            ref.update_H()
+        torch.cuda.synchronize()
+        assert run.materials.count == count
+        np.testing.assert_array_equal(g.E.cpu().numpy(), ref.E)
+        np.testing.assert_array_equal(g.H.cpu().numpy(), ref.H)
+

@@ -95,7 +95,8 @@ class March(ctypes.Structure):
                 ('e', MarchFamily), ('h', MarchFamily),
                 ('nsrc_e', i32), ('srcs_e', SourceBox*MAX_SOURCES),
                 ('nsrc_h', i32), ('srcs_h', SourceBox*MAX_SOURCES),
-                ('source_samples', i64), ('counter', vp), ('tile_y', i32), ('chunk_x', i32)]
+                ('source_samples', i64), ('counter', vp), ('tile_y', i32), ('chunk_x', i32),
+                ('material', vp), ('material_table', vp), ('materials', i32)]
 
 
 class NormGrid(ctypes.Structure):

@@ -151,7 +151,8 @@ class _Engine:
         width = min(options.cuda_graph_steps, self.steps)
         self.width = width
         self.forward_graphs = StepGraphs(self._step, run.states()+[self.record], self.steps, width,
-                                         pair=self._pair if run.one_pass else None)
+                                         pair=self._pair if run.one_pass else None,
+                                         prepare=run.refresh_materials if run.one_pass else None)
         self.report = dict(adjoint='discrete Yee/CPML (HIP transpose)', steps=self.steps, design_box=[list(b) for b in self.box],
                            checkpoint_interval=self.interval, record_bytes=self.record.numel()*item,
                            forward_kernel='one-pass' if run.one_pass else 'split', observers=len(observers))

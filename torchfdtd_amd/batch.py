@@ -264,8 +264,23 @@ class Cohort:
             run.grid.set_permittivity(np.asarray(eps))
             if run.grid.inverse_permittivity is not layout:
                 raise ValueError("a reset keeps each grid's permittivity layout (per-cell or per-component)")
+        self.refresh_materials()
         for value in self.states():
             value.zero_()
+
+    def refresh_materials(self):
+        """Carry permittivity changes of any grid into its material index and into the cohort's
+        one-pass tables (rewritten in place: captured graphs stay valid).  Called before every
+        eager one-pass step and graph replay; a version check when nothing changed."""
+        if not self.one_pass:
+            return False
+        if not any([run.refresh_materials() for run in self.runs]):
+            return False
+        for k, (table, prefix, totals) in enumerate(self.march):
+            fresh, _, _ = _pack(lambda n: self.lib.tfdtd_batch_march_bytes(n, self.dtype), self.lib.tfdtd_batch_march_pack,
+                                [run.march[k] for run in self.runs], 3, self.counter.device)
+            table.copy_(fresh)
+        return True
 
     def states(self):
         states = [self.counter, self.traces]
@@ -292,6 +307,7 @@ class Cohort:
 
     def step_pair(self):
         """Two steps with the one-pass kernels: grid buffers -> second set -> grid buffers."""
+        self.refresh_materials()
         stream = N.stream_handle()
         lib, count, launch = self.lib, len(self.runs), self.launch
         eps_stride = self.grids[0].inverse_permittivity.shape[-1]
@@ -312,7 +328,8 @@ class Cohort:
                 self.step()
 
     def graphs(self, width=1):
-        return StepGraphs(self.step, self.states(), self.steps, width, pair=self.step_pair if self.one_pass else None)
+        return StepGraphs(self.step, self.states(), self.steps, width, pair=self.step_pair if self.one_pass else None,
+                          prepare=self.refresh_materials if self.one_pass else None)
 
     def traces_of(self, i, completed):
         return self.traces[:completed, int(self.trace_offsets[i]):int(self.trace_offsets[i+1])].cpu().numpy().copy()

@@ -57,6 +57,9 @@ struct March {
     real* h_out;
     const real* inv_eps;
     int eps_stride;
+    const unsigned char* material;     // optional per-cell index into material_table (eps_stride 1)
+    const real* material_table;
+    int materials;
     const real* inv_mu;
     real courant;
     int pec_mask;
@@ -328,6 +331,7 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
     __shared__ real sH0[SH];
     __shared__ real sH1[kStageDMA ? SH : 1];
     __shared__ real sE[(TY + 1) * EC * 3];
+    __shared__ real sMat[EPS3 ? 1 : 256];      // 1/eps of each material when the grid is indexed
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     int ty, tz, chunk;
     if (CPML != 7) {
@@ -378,6 +382,9 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
     real* __restrict__ e_out = a.e_out;
     real* __restrict__ h_out = a.h_out;
     const real* __restrict__ inv_eps = a.inv_eps;
+    const unsigned char* __restrict__ material = EPS3 ? nullptr : a.material;
+    if (!EPS3 && material)
+        for (int i = tid; i < a.materials; i += 64 * W) sMat[i] = global(a.material_table)[i];   // seen after the first barrier
     const real hcoef = a.courant * global(a.inv_mu)[0];
     const unsigned plane = (unsigned)ny * nz;
 
@@ -449,7 +456,7 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
                 if (EPS3) {
                     inv[k] = load3(inv_eps, 3u * cell);
                 } else {
-                    const real v = global(inv_eps)[cell];
+                    const real v = material ? sMat[global(material)[cell]] : global(inv_eps)[cell];
                     inv[k] = V3<real>{v, v, v};
                 }
             }
@@ -587,6 +594,13 @@ int make_march(const tfdtd_march* d, March<real>* a) {
     a->e_in = static_cast<const real*>(d->e_in); a->h_in = static_cast<const real*>(d->h_in);
     a->e_out = static_cast<real*>(d->e_out); a->h_out = static_cast<real*>(d->h_out);
     a->inv_eps = static_cast<const real*>(d->inverse_permittivity); a->eps_stride = d->eps_stride;
+    if (d->material) {
+        if (d->eps_stride != 1 || d->materials < 1 || d->materials > 256 || !d->material_table)
+            return tfdtd_fail(TFDTD_EINVAL, "march: a material index needs eps_stride 1, 1 to 256 materials and a table");
+        a->material = d->material;
+        a->material_table = static_cast<const real*>(d->material_table);
+        a->materials = d->materials;
+    }
     a->inv_mu = static_cast<const real*>(d->inverse_permeability);
     a->courant = (real)d->courant;
     a->pec_mask = d->pec_upper_mask & 7;

@@ -33,10 +33,12 @@ class StepGraphs:
     """Capture `step` for 1 and `width` steps on a side stream; `states` are zeroed after warm-up and capture.
 
     With `pair` (two steps that return the state to its own buffers) the `width` graph
-    is built from pairs, plus one `step` when `width` is odd.
+    is built from pairs, plus one `step` when `width` is odd.  `prepare`, when given, runs
+    before every replay (e.g. Stepper.refresh_materials, which follows permittivity changes).
     """
 
-    def __init__(self, step, states, steps, width=1, warmup=3, pair=None):
+    def __init__(self, step, states, steps, width=1, warmup=3, pair=None, prepare=None):
+        self.prepare = prepare
         self.width = min(validate_graph_steps(width), steps)
         states = list(states)
         stream = torch.cuda.Stream()
@@ -70,4 +72,6 @@ class StepGraphs:
         torch.cuda.synchronize()
 
     def replay(self, width=1):
+        if self.prepare is not None:
+            self.prepare()
         self.graphs[width].replay()

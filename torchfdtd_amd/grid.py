@@ -184,8 +184,58 @@ def march_family(grid, forward, psi_in, psi_out):
     return f
 
 
-def march_descriptor(grid, e_in, h_in, e_out, h_out, psi_e_in, psi_e_out, sources, counter, tile_y=8, chunk_x=8):
-    """tfdtd_march: one whole step reading (e_in, h_in, psi_e_in) and writing (e_out, h_out, psi_e_out)."""
+class MaterialIndex:
+    """The grid's per-cell 1/eps as a one-byte material index and a table of at most 256 values, for
+    the one-pass step (which then reads a quarter of the bytes).  Only grids with one value per
+    cell and at most 256 distinct values are indexed; the table holds the exact values, so the
+    step's arithmetic is unchanged.  `refresh` rebuilds index and table in place (their device
+    addresses never change) when inverse_permittivity was modified or replaced since the last
+    build, as torch's version counter of the tensor tells."""
+
+    MAX = 256
+
+    def __init__(self, grid):
+        self.grid = grid
+        cells = grid.Nx*grid.Ny*grid.Nz
+        self.index = torch.zeros(cells, dtype=torch.uint8, device=grid.device)
+        self.table = torch.zeros(self.MAX, dtype=grid.dtype, device=grid.device)
+        self.count = 0
+        self.seen = None
+        self.refresh()
+
+    def _key(self):
+        inv = self.grid.inverse_permittivity
+        return id(inv), inv._version
+
+    def stale(self):
+        return self._key() != self.seen
+
+    def refresh(self):
+        """Rebuild from inverse_permittivity if it changed; returns True when it did."""
+        if not self.stale():
+            return False
+        inv = self.grid.inverse_permittivity
+        self.count = 0
+        if inv.shape[-1] == 1:
+            values, inverse = torch.unique(inv.reshape(-1), return_inverse=True)
+            if values.numel() <= self.MAX:
+                self.table[:values.numel()] = values
+                self.index.copy_(inverse.to(torch.uint8))
+                self.count = values.numel()
+        self.seen = self._key()
+        return True
+
+    def describe(self, d):
+        """Point tfdtd_march `d` at the index (or at the plain 1/eps when the grid is not indexed)."""
+        d.material = _ptr(self.index) if self.count else None
+        d.material_table = _ptr(self.table) if self.count else None
+        d.materials = self.count
+
+
+def march_descriptor(grid, e_in, h_in, e_out, h_out, psi_e_in, psi_e_out, sources, counter, tile_y=8, chunk_x=8,
+                     materials=None):
+    """tfdtd_march: one whole step reading (e_in, h_in, psi_e_in) and writing (e_out, h_out, psi_e_out);
+    `materials` (a MaterialIndex) lets the step read 1/eps through the one-byte index."""
     d = N.March()
     d.dtype = N.F64 if grid.dtype == torch.float64 else N.F32
     d.nx, d.ny, d.nz = grid.Nx, grid.Ny, grid.Nz
@@ -218,6 +268,8 @@ def march_descriptor(grid, e_in, h_in, e_out, h_out, psi_e_in, psi_e_out, source
     d.source_samples = samples
     d.counter = _ptr(counter)
     d.tile_y, d.chunk_x = tile_y, chunk_x
+    if materials is not None:
+        materials.describe(d)
     return d
 
 

@@ -22,7 +22,7 @@ import torch
 
 from . import _native as N
 from .graphs import StepGraphs, observation_schedule, validate_graph_steps
-from .grid import (YeeGrid, configure_cuda_kernel, half_step_descriptor, march_descriptor, march_supported,
+from .grid import (MaterialIndex, YeeGrid, configure_cuda_kernel, half_step_descriptor, march_descriptor, march_supported,
                    resolve_cuda_kernel)
 from .models import Project
 from .monitors import FrequencyPlane, apodization_window, point_spectrum
@@ -170,6 +170,8 @@ class Stepper:
         if self.one_pass and not supported:
             raise ValueError('the one-pass kernel needs a 3D grid without periodic axes and at most 8 sources per family')
         self.tile_y, self.chunk_x = tile_y, chunk_x
+        # One-byte material index of 1/eps for the one-pass step (grids with few distinct values).
+        self.materials = MaterialIndex(g) if self.one_pass else None
         self._describe()
 
     def set_trace_points(self, points):
@@ -209,14 +211,16 @@ class Stepper:
         self.trace_table = self._trace_table(g.E, g.H)
         self.observe = self._observe(self.trace_table)
         if self.one_pass:
+            if self.materials is not None:
+                self.materials.refresh()     # the permittivity may have been replaced (a widened layout)
             # Second buffer set of the one-pass step: time n is read from one set, n+1 written to the other.
             self.E_b, self.H_b = torch.empty_like(g.E), torch.empty_like(g.H)
             self.psi_e_b = torch.zeros_like(g.psi_buffers[False])
             psi_a = g.psi_buffers[False]
             self.march = [march_descriptor(g, g.E, g.H, self.E_b, self.H_b, psi_a, self.psi_e_b, self.sources, self.counter,
-                                           self.tile_y, self.chunk_x),
+                                           self.tile_y, self.chunk_x, self.materials),
                           march_descriptor(g, self.E_b, self.H_b, g.E, g.H, self.psi_e_b, psi_a, self.sources, self.counter,
-                                           self.tile_y, self.chunk_x)]
+                                           self.tile_y, self.chunk_x, self.materials)]
             # The step reads its descriptor from device memory (a batch of one grid): the kernel then
             # loads tile parameters on demand instead of holding the whole descriptor in registers.
             self.march_tables = []
@@ -232,8 +236,31 @@ class Stepper:
         extra = [self.E_b, self.H_b, self.psi_e_b] if self.one_pass else []
         return [self.grid.E, self.grid.H, self.counter, self.traces, *self.grid.memory_states, *extra]
 
+    def refresh_materials(self):
+        """Carry a change of the grid's 1/eps into the one-pass step's material index: rebuilds the
+        index and rewrites the device descriptors in place (captured graphs stay valid).  A version
+        check when nothing changed; called before every eager one-pass step and graph replay."""
+        m = self.materials
+        if m is None or not m.stale():
+            return False
+        inv = self.grid.inverse_permittivity
+        if any(d.inverse_permittivity != inv.data_ptr() or d.eps_stride != inv.shape[-1] for d in self.march):
+            raise RuntimeError('the inverse permittivity tensor was replaced after the step was built; '
+                               'build a new Stepper (in-place changes are followed)')
+        if torch.cuda.is_current_stream_capturing():
+            raise RuntimeError('the permittivity changed while the step was being captured')
+        m.refresh()
+        dtype = self.march[0].dtype
+        for d, (table, prefix, totals) in zip(self.march, self.march_tables):
+            m.describe(d)
+            fresh, _, _ = pack_table(lambda n: self.lib.tfdtd_batch_march_bytes(n, dtype), self.lib.tfdtd_batch_march_pack,
+                                     [d], 3, self.grid.device)
+            table.copy_(fresh)
+        return True
+
     def march_step(self, k, stream):
         """One one-pass step from buffer set k (0: grid buffers -> second set, 1: back)."""
+        self.refresh_materials()
         table, prefix, totals = self.march_tables[k]
         d = self.march[k]
         N.check(self.lib.tfdtd_batch_march(table.data_ptr(), prefix.data_ptr(), 1, d.dtype, self.tile_y, d.eps_stride,
@@ -309,7 +336,8 @@ class Simulation:
         control = r.run_control
         diagnostics = StateDiagnostics(g) if control.auto_shutoff or control.divergence_check else None
         decision = DecayDecision(control, source_end_time(p), g.time_step)
-        graph = (StepGraphs(run.step, run.states(), r.steps, cuda_graph_steps, pair=run.step_pair if run.one_pass else None)
+        graph = (StepGraphs(run.step, run.states(), r.steps, cuda_graph_steps, pair=run.step_pair if run.one_pass else None,
+                            prepare=run.refresh_materials if run.one_pass else None)
                  if cuda_graph else None)
         view = DisplaySlice(p)
         plane_index = view.index
