@@ -331,7 +331,11 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
     __shared__ real sH0[SH];
     __shared__ real sH1[kStageDMA ? SH : 1];
     __shared__ real sE[(TY + 1) * EC * 3];
-    __shared__ real sMat[EPS3 ? 1 : 256];      // 1/eps of each material when the grid is indexed
+    // The interior tiles (bandwidth-bound) read 1/eps through the grid's material index when it has
+    // one; the z-face and boundary tiles (latency-bound) read the plain array, which the index
+    // mirrors -- the dependent table lookup costs them more than the bytes it saves.
+    constexpr bool INDEXED = !EPS3 && CPML == 0;
+    __shared__ real sMat[INDEXED ? 256 : 1];   // 1/eps of each material
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     int ty, tz, chunk;
     if (CPML != 7) {
@@ -382,8 +386,8 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
     real* __restrict__ e_out = a.e_out;
     real* __restrict__ h_out = a.h_out;
     const real* __restrict__ inv_eps = a.inv_eps;
-    const unsigned char* __restrict__ material = EPS3 ? nullptr : a.material;
-    if (!EPS3 && material)
+    const unsigned char* __restrict__ material = INDEXED ? a.material : nullptr;
+    if (INDEXED && material)
         for (int i = tid; i < a.materials; i += 64 * W) sMat[i] = global(a.material_table)[i];   // seen after the first barrier
     const real hcoef = a.courant * global(a.inv_mu)[0];
     const unsigned plane = (unsigned)ny * nz;
