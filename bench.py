@@ -79,6 +79,24 @@ def counted_traffic(n):
     return None
 
 
+def stream_copy_rate(torch, gib=4, reps=10):
+    """GB/s (bytes read + written) of a plain device-to-device copy of `gib` GiB of float32 on this GPU:
+    the practical HBM ceiling the step's achieved bandwidth is also compared with."""
+    src = torch.ones(int(gib*2**30)//4, device='cuda')
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    rate = 2*src.numel()*4*reps/(e0.elapsed_time(e1)/1e3)/1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return rate
+
+
 def batched_line(steps, count, loop_grids, launch='tune', width=50):
     """The uneven batch: 64 grids of 64x64x32 .. 192x160x64 cells, 2000 steps, autotuned launch,
     against the per-grid loop of Simulation.run (both with graphs of `width` steps)."""
@@ -263,6 +281,7 @@ def main():
         e1.record()
         torch.cuda.synchronize()
         kernel_ms[name] = e0.elapsed_time(e1)/reps
+    copy_gbs = stream_copy_rate(torch)
     step_bytes, psi_per_cell = algorithmic_bytes(region)
     if run.one_pass:
         # The step is three concurrent tile launches (fork/join on the launch stream): its duration is
@@ -275,7 +294,8 @@ def main():
     achieved = step_bytes/(per_step_ms/1e3)/1e9
     roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
                     frac=round(achieved/HBM_PEAK_GBS, 4), traffic=counted_traffic(args.n) if run.one_pass else None,
-                    kernel=kernel, kernel_ms=kernel_ms,
+                    kernel=kernel, kernel_ms=kernel_ms, stream_copy=round(copy_gbs, 1),
+                    frac_of_copy=round(achieved/copy_gbs, 4),
                     bytes_per_cell_update=round(step_bytes/cells, 2), psi_per_cell=round(psi_per_cell, 4))
     baseline = None
     if not args.no_cpu_baseline:
