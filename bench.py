@@ -67,14 +67,16 @@ def cpu_baseline(seconds_budget, mask):
                        f'{elapsed:.1f} s')
 
 
-def counted_traffic(n):
-    """HBM bytes per one-pass step of this cube from the committed counter summary (FETCH_SIZE and
-    WRITE_SIZE per dispatch, scaled by a plain copy's counts), or None for another size."""
+def counted_traffic(n, two_step=False):
+    """HBM bytes per step of this cube (the two-step pass: half of one launch) from the committed
+    counter summary (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a plain copy's counts), or None
+    for another size."""
     path = os.path.join(ROOT, 'profiles', 'march_512cube_hbm_counters.md')
     if n != 512 or not os.path.exists(path):
         return None
+    key = 'two_step_traffic_bytes_per_step:' if two_step else 'traffic_bytes_per_step:'
     for line in open(path):
-        if line.startswith('traffic_bytes_per_step:'):
+        if line.startswith(key):
             return int(float(line.split(':')[1]))
     return None
 
@@ -195,6 +197,7 @@ def main():
     ap.add_argument('--batch-steps', type=int, default=2000)
     ap.add_argument('--batch-launch', choices=('tune', 'split', 'march'), default='tune')
     ap.add_argument('--no-cube', action='store_true', help='skip the 512^3 cube (profiling the other parts alone)')
+    ap.add_argument('--one-step', action='store_true', help='time the one-step kernel (one step per pass over HBM)')
     ap.add_argument('--loop-grids', type=int, default=64)
     ap.add_argument('--copy-gib', type=float, default=0,
                     help='only copy a buffer of this many GiB three times (the calibration of HBM byte counters)')
@@ -230,15 +233,17 @@ def main():
     region = project.region
     print(f'bench: building {args.n}^3 cube, {region.pml_cells}-cell CPML', file=sys.stderr, flush=True)
     mask = cube_noise_mask(args.n)
-    run = Stepper(project)
+    run = Stepper(project, two_step=None if not args.one_step else False)
     g = run.grid
     g.inverse_permittivity.copy_(cube_inverse_permittivity(mask)[..., None])
     torch.cuda.synchronize()
     width = args.graph_width
     graph = StepGraphs(run.step, run.states(), region.steps, width, warmup=1,
                        pair=run.step_pair if run.one_pass else None,
-                       prepare=run.refresh_materials if run.one_pass else None)
-    print(f'bench: graphs captured (one-pass kernel: {run.one_pass})', file=sys.stderr, flush=True)
+                       prepare=run.refresh_materials if run.one_pass else None,
+                       quad=run.quad if run.two_step else None)
+    print(f'bench: graphs captured (one-pass kernel: {run.one_pass}, two steps per pass: {run.two_step})',
+          file=sys.stderr, flush=True)
 
     def advance(steps):
         while steps >= width:
@@ -269,18 +274,20 @@ def main():
     launches = [('yee_half_step_E', lambda: lib.tfdtd_yee_half_step(ctypes.byref(run.e_step), stream)),
                 ('yee_half_step_H', lambda: lib.tfdtd_yee_half_step(ctypes.byref(run.h_step), stream))]
     if run.one_pass:
-        launches.append(('march', None))
+        launches.append(('march', lambda k: run.march_step(k % 2, stream)))
+    if run.two_step:
+        launches.append(('march2_per_step', lambda k: run.march2_step(k % 2, stream)))
     for name, launch in launches:
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         for k in range(reps):
-            if launch is None:
-                run.march_step(k % 2, stream)
+            if name.startswith('march'):
+                launch(k)
             else:
                 N.check(launch(), name)
         e1.record()
         torch.cuda.synchronize()
-        kernel_ms[name] = e0.elapsed_time(e1)/reps
+        kernel_ms[name] = e0.elapsed_time(e1)/reps/(2 if name == 'march2_per_step' else 1)
     copy_gbs = stream_copy_rate(torch)
     step_bytes, psi_per_cell = algorithmic_bytes(region)
     if run.one_pass:
@@ -289,11 +296,15 @@ def main():
         per_step_ms = seconds*1e3/args.steps
         kernel = ('one-pass Yee step: interior, z-face and boundary tile launches running concurrently; '
                   'duration = per-step span of the timed region')
+        if run.two_step:
+            kernel = ('two-step Yee pass (two steps per pass over HBM): interior, z-face and boundary tile launches '
+                      'running concurrently; duration = per-step span of the timed region')
     else:
         per_step_ms, kernel = kernel_ms['yee_half_step_E']+kernel_ms['yee_half_step_H'], 'yee_half_step E+H'
     achieved = step_bytes/(per_step_ms/1e3)/1e9
     roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
-                    frac=round(achieved/HBM_PEAK_GBS, 4), traffic=counted_traffic(args.n) if run.one_pass else None,
+                    frac=round(achieved/HBM_PEAK_GBS, 4),
+                    traffic=counted_traffic(args.n, run.two_step) if run.one_pass else None,
                     kernel=kernel, kernel_ms=kernel_ms, stream_copy=round(copy_gbs, 1),
                     frac_of_copy=round(achieved/copy_gbs, 4),
                     bytes_per_cell_update=round(step_bytes/cells, 2), psi_per_cell=round(psi_per_cell, 4))

@@ -175,6 +175,18 @@ typedef struct tfdtd_march {
  * tfdtd_batch_march. */
 int tfdtd_yee_step_march(const tfdtd_march* step, void* stream);
 
+/* Two whole Yee steps, n -> n+2, in one pass over HBM (float32): the step closure of
+ * torchfdtd/solver.py:785-797 twice without the observers, i.e. tfdtd_yee_step_march from
+ * the *_in set into the *_out set and back, with the intermediate step kept on chip.  The
+ * result -- E, H and both CPML memories -- equals two calls of tfdtd_yee_step_march bit for bit.
+ * Time n is read from the *_in buffers and n+2 written to the *_out buffers; unlike the one-step
+ * call, the H family's CPML memory is double-buffered too (h.psi_out must differ from h.psi_in
+ * when the grid has H slabs), because a tile recomputes step n+1 on cells its neighbours own.
+ * Sources are added at both steps (samples n and n+1 of the device counter, which the call reads
+ * and does not advance).  chunk_x is the planes per workgroup (about 20 suits a 512^3 grid);
+ * tile_y is ignored (tiles are 9 rows x 61 columns).  Concurrency as tfdtd_yee_step_march. */
+int tfdtd_yee_step_march2(const tfdtd_march* step, void* stream);
+
 /* torchfdtd/cuda_kernels.py:349-365 (FusedYeeCUDA.update_E / update_H), with the
  * family's sources of torchfdtd/solver.py:778-783 folded in when nsrc > 0. */
 int tfdtd_yee_half_step(const tfdtd_half_step* step, void* stream);

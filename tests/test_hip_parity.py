@@ -118,6 +118,87 @@ def test_one_pass_step_matches_oracle_bitwise(gpu, precision, layout, tile_y, ch
         np.testing.assert_array_equal(got.cpu().numpy(), expected)
 
 
+@pytest.mark.parametrize('layout,chunk_x2,nz,components', [
+    ('pml3d', 7, 131, 3), ('pml3d', 20, 200, 'index'), ('pec', 5, 131, 1), ('pml3d', 0, 131, 1), ('pec', 20, 200, 3),
+    ('pml3d', 3, 64, 'index')])
+def test_two_step_pass_matches_oracle_bitwise(gpu, layout, chunk_x2, nz, components):
+    """The two-step pass (step n+1 recomputed on widened tiles and kept on chip, E, H and both CPML
+    memories double-buffered) equals the reference update bit for bit, starting from random fields
+    and random CPML memory, over five passes of four steps, one pair and one single step."""
+    from torchfdtd_amd import Project
+    from torchfdtd_amd.solver import Stepper
+    region = region_for(layout, 'float32')
+    region.size = (3.2, 7.4, nz/10)     # 32 x 74 x nz cells: partial y and z tiles
+    indexed = components == 'index'
+    if indexed:
+        components = 1
+    if components == 1:
+        region.material_sampling = 'cell'
+    project = Project(region=region)
+    run = Stepper(project, steps=40, one_pass=True, two_step=True, chunk_x2=chunk_x2)
+    g = run.grid
+    scene = Scene(project.model_dump())
+    real = scene.real
+    rng = np.random.default_rng(11)
+    E = (rng.normal(size=(*scene.shape, 3))*.02).astype(real)
+    H = (rng.normal(size=(*scene.shape, 3))*.02).astype(real)
+    inv = rng.uniform(.25, 1, (*scene.shape, components)).astype(real)
+    if indexed:
+        inv = rng.choice(np.unique(inv)[:5], size=inv.shape)
+    ref = OracleGrid(scene, inv.copy())
+    ref.E[:], ref.H[:] = E, H
+    g.E.copy_(torch.as_tensor(E))
+    g.H.copy_(torch.as_tensor(H))
+    g.inverse_permittivity.copy_(torch.as_tensor(inv))
+    assert len(g.memory_states) == len(ref.memory_states) > 0
+    for got, expected in zip(g.memory_states, ref.memory_states):
+        expected[...] = (rng.normal(size=expected.shape)*.01).astype(real)
+        got.copy_(torch.as_tensor(expected))
+    for _ in range(5):
+        run.quad()
+        for _ in range(4):
+            ref.update_E()
+            ref.update_H()
+    run.step_pair()     # the one-step kernel continues from the two-step pass's state
+    run.step()
+    for _ in range(3):
+        ref.update_E()
+        ref.update_H()
+    torch.cuda.synchronize()
+    assert run.materials.count == (5 if indexed else 0)
+    assert int(run.counter.item()) == 23
+    np.testing.assert_array_equal(g.E.cpu().numpy(), ref.E)
+    np.testing.assert_array_equal(g.H.cpu().numpy(), ref.H)
+    for got, expected in zip(g.memory_states, ref.memory_states):
+        np.testing.assert_array_equal(got.cpu().numpy(), expected)
+
+
+def test_two_step_pass_with_sources_equals_one_step_pairs_under_graphs(gpu):
+    """The cube workload at 96^3 (16-cell CPML, two-material 1/eps, Ez dipole): graph replays of the
+    two-step pass (quads, then a pair and a single step) equal the one-step pairs bit for bit,
+    source included."""
+    from torchfdtd_amd.graphs import StepGraphs
+    from torchfdtd_amd.solver import Stepper
+    from torchfdtd_amd.workloads import cube, cube_inverse_permittivity, cube_noise_mask
+    mask = cube_noise_mask(96)
+    out = []
+    for two_step in (False, True):
+        run = Stepper(cube(n=96, steps=60), two_step=two_step)
+        assert run.two_step == two_step
+        run.grid.inverse_permittivity.copy_(cube_inverse_permittivity(mask)[..., None])
+        graphs = StepGraphs(run.step, run.states(), 60, 23, warmup=1, pair=run.step_pair,
+                            prepare=run.refresh_materials, quad=run.quad if two_step else None)
+        for _ in range(2):
+            graphs.replay(23)
+        graphs.replay(1)
+        torch.cuda.synchronize()
+        assert int(run.counter.item()) == 47
+        out.append([t.clone() for t in (run.grid.E, run.grid.H, *run.grid.memory_states)])
+    assert float(out[0][0].abs().max()) > 0
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
+
+
 def test_subnormal_fields_are_preserved(gpu):
     from torchfdtd_amd import Project, Region
     from torchfdtd_amd.grid import HipYee, YeeGrid

@@ -121,7 +121,7 @@ class _Engine:
         r = project.region
         self.options = options
         self.steps = r.steps
-        self.run = run = Stepper(project, kernel='hip', epsilon=epsilon)
+        self.run = run = Stepper(project, kernel='hip', epsilon=epsilon, two_step=False)
         run.set_trace_points(observers)
         g = self.grid = run.grid
         self.lib = N.library()

@@ -180,7 +180,7 @@ class Cohort:
         self.counter = torch.zeros(1, device=device, dtype=torch.long)
         one_pass = launch.kernel == 'march'
         self.runs = [Stepper(p, steps=self.steps, kernel='hip', one_pass=None if one_pass else False, tile_y=launch.tile_y,
-                             chunk_x=launch.chunk_x, counter=self.counter, epsilon=eps)
+                             chunk_x=launch.chunk_x, counter=self.counter, epsilon=eps, two_step=False)
                      for p, eps in zip(self.projects, epsilons)]
         if one_pass and not all(run.one_pass for run in self.runs):
             raise ValueError('The one-pass kernels need 3D grids without periodic axes and at most 8 sources per family.')

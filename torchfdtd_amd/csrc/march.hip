@@ -301,6 +301,50 @@ __device__ __forceinline__ V3<real> h_cell(const March<real>& a, int x, int y, i
     return V3<real>{h.x - coefficient * cx, h.y - coefficient * cy, h.z - coefficient * cz};
 }
 
+// Tile (y tile, z tile, x chunk) of workgroup w among the G workgroups of variant CPML of a grid.
+// CPML 0 and 4 enumerate the box of x chunks [box_lo[0], box_hi[0]) and y tiles [box_lo[1], box_hi[1])
+// with the z tiles free of z slabs (0) or the others (4); each XCD (workgroup w runs on XCD w % 8) takes a
+// contiguous range, so neighbouring tiles share L2.  CPML 7 enumerates the complement of the box as four
+// blocks, round robin over XCDs.
+template <typename real, int CPML>
+__device__ __forceinline__ void tile_coords(const March<real>& a, const int w, const int G, int& ty, int& tz, int& chunk) {
+    if (CPML != 7) {
+        const int full = (G / 8) * 8;
+        int t = w < full ? (w % 8) * (G / 8) + (w / 8) : w;
+        const int my = a.box_hi[1] - a.box_lo[1];
+        const int zlo = a.zbox_lo, zhi = a.zbox_hi;
+        const int mz = CPML == 0 ? zhi - zlo : a.ztiles - (zhi - zlo);
+        ty = a.box_lo[1] + t % my;
+        t /= my;
+        const int k = t % mz;
+        tz = CPML == 0 ? zlo + k : (k < zlo ? k : zhi + (k - zlo));
+        chunk = a.box_lo[0] + t / mz;
+    } else {
+        int b = w;
+        const int per_chunk = a.ytiles * a.ztiles, inner = a.box_hi[0] - a.box_lo[0];
+        const int nA = a.box_lo[0] * per_chunk, nB = (a.nchunks - a.box_hi[0]) * per_chunk;
+        const int nC = inner * a.box_lo[1] * a.ztiles;
+        if (b < nA + nB) {
+            chunk = b < nA ? b / per_chunk : a.box_hi[0] + (b - nA) / per_chunk;
+            const int r = b < nA ? b % per_chunk : (b - nA) % per_chunk;
+            ty = r % a.ytiles;
+            tz = r / a.ytiles;
+        } else if ((b -= nA + nB) < nC) {
+            ty = b % a.box_lo[1];
+            b /= a.box_lo[1];
+            tz = b % a.ztiles;
+            chunk = a.box_lo[0] + b / a.ztiles;
+        } else {
+            b -= nC;
+            const int high = a.ytiles - a.box_hi[1];
+            ty = a.box_hi[1] + b % high;
+            b /= high;
+            tz = b % a.ztiles;
+            chunk = a.box_lo[0] + b / a.ztiles;
+        }
+    }
+}
+
 // One workgroup (W = 4 or 8 wavefronts) marches a tile of TY x 63 (y, z) cells, TY = 2W, over a chunk
 // of x planes.  Wavefront w owns rows y0 + 2w, y0 + 2w + 1 (a wave-uniform y); lane l covers column
 // z0 + l, lanes 0..62 own their cells and lane 63 recomputes E of the next tile's first column.
@@ -338,45 +382,7 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
     __shared__ real sMat[INDEXED ? 256 : 1];   // 1/eps of each material
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     int ty, tz, chunk;
-    if (CPML != 7) {
-        // Lean tiles: the box of x chunks [lo0, hi0) and y tiles [lo1, hi1), every z tile.  Each XCD
-        // (workgroup w runs on XCD w % 8) takes a contiguous range, so neighbouring tiles share L2.
-        const int full = (G / 8) * 8;
-        int t = w < full ? (w % 8) * (G / 8) + (w / 8) : w;
-        // CPML == 0: z tiles [zlo, zhi) free of z slabs; CPML == 4: the other z tiles.
-        const int my = a.box_hi[1] - a.box_lo[1];
-        const int zlo = a.zbox_lo, zhi = a.zbox_hi;
-        const int mz = CPML == 0 ? zhi - zlo : a.ztiles - (zhi - zlo);
-        ty = a.box_lo[1] + t % my;
-        t /= my;
-        const int k = t % mz;
-        tz = CPML == 0 ? zlo + k : (k < zlo ? k : zhi + (k - zlo));
-        chunk = a.box_lo[0] + t / mz;
-    } else {
-        // Full tiles: the complement of the box, enumerated as four blocks, round robin over XCDs.
-        int b = w;
-        const int per_chunk = a.ytiles * a.ztiles, inner = a.box_hi[0] - a.box_lo[0];
-        const int nA = a.box_lo[0] * per_chunk, nB = (a.nchunks - a.box_hi[0]) * per_chunk;
-        const int nC = inner * a.box_lo[1] * a.ztiles;
-        if (b < nA + nB) {
-            chunk = b < nA ? b / per_chunk : a.box_hi[0] + (b - nA) / per_chunk;
-            const int r = b < nA ? b % per_chunk : (b - nA) % per_chunk;
-            ty = r % a.ytiles;
-            tz = r / a.ytiles;
-        } else if ((b -= nA + nB) < nC) {
-            ty = b % a.box_lo[1];
-            b /= a.box_lo[1];
-            tz = b % a.ztiles;
-            chunk = a.box_lo[0] + b / a.ztiles;
-        } else {
-            b -= nC;
-            const int high = a.ytiles - a.box_hi[1];
-            ty = a.box_hi[1] + b % high;
-            b /= high;
-            tz = b % a.ztiles;
-            chunk = a.box_lo[0] + b / a.ztiles;
-        }
-    }
+    tile_coords<real, CPML>(a, w, G, ty, tz, chunk);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int y0 = ty * TY, z0 = tz * TZ - a.zshift;
     const int xa = chunk * a.chunk, xb = min(nx, xa + a.chunk);
@@ -542,6 +548,379 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
         }
     } else {
         for (int x = xa; x <= xb; ++x) march_plane(x, sH0, sH0);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------
+// Two steps per pass: time n -> n+2 with the intermediate step n+1 kept on chip.
+//
+// CPML memory of the two-step march: stage 1 (step n+1) reads psi at time n from memory and keeps psi
+// at n+1 in a register slot of its cell; stage 2 (step n+2) continues from that slot and stores psi at
+// n+2 (owned cells only).  Both families read psi_in and write psi_out, which must differ.
+template <typename real, int STAGE>
+__device__ __forceinline__ real cpml2(real d, const Family<real>& F, const Hit& hit, int k, unsigned p, bool write, real& slot) {
+    const unsigned o = (k ? hit.psi1 : hit.psi0) + p;
+    const auto coef = global(F.coef);
+    const real b = coef[hit.coef + hit.q];
+    const real c = coef[hit.coef + hit.w + hit.q];
+    const real ik = coef[hit.coef + 2 * hit.w + hit.q];
+    const real m = (STAGE == 1 ? global(F.psi_in)[o] : slot) * b + c * d;
+    if (STAGE == 1) slot = m;
+    else if (write) global(F.psi_out)[o] = m;
+    return d * ik + m;
+}
+
+template <typename real, int STAGE>
+__device__ __forceinline__ real cpml_lane2(real d, const Family<real>& F, const Hit& hit, const ZCoef<real>& zc, int k,
+                                           unsigned p, bool write, real& slot) {
+    const unsigned o = (k ? hit.psi1 : hit.psi0) + p;
+    const real m = (STAGE == 1 ? global(F.psi_in)[o] : slot) * zc.b + zc.c * d;
+    if (STAGE == 1) slot = m;
+    else if (write) global(F.psi_out)[o] = m;
+    return d * zc.ik + m;
+}
+
+// e_cell with the CPML memory of term t in slot[t].
+template <typename real, int CPML, int STAGE>
+__device__ __forceinline__ V3<real> e_cell2(const March<real>& a, int x, int y, int z, bool write, const Hit& hx, const Hit& hy,
+                                            const Hit& hz, const ZCoef<real>& zc, V3<real> h, V3<real> hxm, V3<real> hym,
+                                            real hx_zm, real hy_zm, V3<real> e, V3<real> inv, real (&slot)[6]) {
+    const int ny = a.ny, nz = a.nz;
+    real cx = real(0), cy = real(0), cz = real(0);
+    const unsigned px = ((unsigned)hx.q * ny + y) * nz + z;
+    const unsigned py = ((unsigned)x * hy.w + hy.q) * nz + z;
+    const unsigned pz = ((unsigned)x * ny + y) * hz.w + hz.q;
+    {   // t0: d/dx Hz -> Ey, -1
+        real d = real(0);
+        if (x > 0) {
+            d = h.z - hxm.z;
+            if ((CPML & 1) && hx.face >= 0) d = cpml2<real, STAGE>(d, a.ef, hx, 0, px, write, slot[0]);
+        }
+        cy = cy - d;
+    }
+    {   // t1: d/dx Hy -> Ez, +1
+        real d = real(0);
+        if (x > 0) {
+            d = h.y - hxm.y;
+            if ((CPML & 1) && hx.face >= 0) d = cpml2<real, STAGE>(d, a.ef, hx, 1, px, write, slot[1]);
+        }
+        cz = cz + d;
+    }
+    {   // t2: d/dy Hz -> Ex, +1
+        real d = real(0);
+        if (y > 0) {
+            d = h.z - hym.z;
+            if ((CPML & 2) && hy.face >= 0) d = cpml2<real, STAGE>(d, a.ef, hy, 0, py, write, slot[2]);
+        }
+        cx = cx + d;
+    }
+    {   // t3: d/dy Hx -> Ez, -1
+        real d = real(0);
+        if (y > 0) {
+            d = h.x - hym.x;
+            if ((CPML & 2) && hy.face >= 0) d = cpml2<real, STAGE>(d, a.ef, hy, 1, py, write, slot[3]);
+        }
+        cz = cz - d;
+    }
+    {   // t4: d/dz Hy -> Ex, -1
+        real d = real(0);
+        if (z > 0) {
+            d = h.y - hy_zm;
+            if ((CPML & 4) && hz.face >= 0) d = cpml_lane2<real, STAGE>(d, a.ef, hz, zc, 0, pz, write, slot[4]);
+        }
+        cx = cx - d;
+    }
+    {   // t5: d/dz Hx -> Ey, +1
+        real d = real(0);
+        if (z > 0) {
+            d = h.x - hx_zm;
+            if ((CPML & 4) && hz.face >= 0) d = cpml_lane2<real, STAGE>(d, a.ef, hz, zc, 1, pz, write, slot[5]);
+        }
+        cy = cy + d;
+    }
+    return V3<real>{e.x + (a.courant * inv.x) * cx, e.y + (a.courant * inv.y) * cy, e.z + (a.courant * inv.z) * cz};
+}
+
+// h_cell with the CPML memory of term t in slot[t].
+template <typename real, int CPML, int STAGE>
+__device__ __forceinline__ V3<real> h_cell2(const March<real>& a, int x, int y, int z, bool write, real coefficient, const Hit& hx,
+                                            const Hit& hy, const Hit& hz, const ZCoef<real>& zc, V3<real> e, V3<real> exn,
+                                            V3<real> eyn, real ex_zp, real ey_zp, V3<real> h, real (&slot)[6]) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    real cx = real(0), cy = real(0), cz = real(0);
+    const unsigned px = ((unsigned)hx.q * ny + y) * nz + z;
+    const unsigned py = ((unsigned)x * hy.w + hy.q) * nz + z;
+    const unsigned pz = ((unsigned)x * ny + y) * hz.w + hz.q;
+    {   // t0: d/dx Ez -> Hy, -1
+        real d = real(0);
+        if (x < nx - 1) {
+            d = exn.z - e.z;
+            if ((CPML & 1) && hx.face >= 0) d = cpml2<real, STAGE>(d, a.hf, hx, 0, px, write, slot[0]);
+        } else if (a.pec_mask & 1) {
+            d = -a.pec_factor[0] * e.z;
+        }
+        cy = cy - d;
+    }
+    {   // t1: d/dx Ey -> Hz, +1
+        real d = real(0);
+        if (x < nx - 1) {
+            d = exn.y - e.y;
+            if ((CPML & 1) && hx.face >= 0) d = cpml2<real, STAGE>(d, a.hf, hx, 1, px, write, slot[1]);
+        } else if (a.pec_mask & 1) {
+            d = -a.pec_factor[0] * e.y;
+        }
+        cz = cz + d;
+    }
+    {   // t2: d/dy Ez -> Hx, +1
+        real d = real(0);
+        if (y < ny - 1) {
+            d = eyn.z - e.z;
+            if ((CPML & 2) && hy.face >= 0) d = cpml2<real, STAGE>(d, a.hf, hy, 0, py, write, slot[2]);
+        } else if (a.pec_mask & 2) {
+            d = -a.pec_factor[1] * e.z;
+        }
+        cx = cx + d;
+    }
+    {   // t3: d/dy Ex -> Hz, -1
+        real d = real(0);
+        if (y < ny - 1) {
+            d = eyn.x - e.x;
+            if ((CPML & 2) && hy.face >= 0) d = cpml2<real, STAGE>(d, a.hf, hy, 1, py, write, slot[3]);
+        } else if (a.pec_mask & 2) {
+            d = -a.pec_factor[1] * e.x;
+        }
+        cz = cz - d;
+    }
+    {   // t4: d/dz Ey -> Hx, -1
+        real d = real(0);
+        if (z < nz - 1) {
+            d = ey_zp - e.y;
+            if ((CPML & 4) && hz.face >= 0) d = cpml_lane2<real, STAGE>(d, a.hf, hz, zc, 0, pz, write, slot[4]);
+        } else if (a.pec_mask & 4) {
+            d = -a.pec_factor[2] * e.y;
+        }
+        cx = cx - d;
+    }
+    {   // t5: d/dz Ex -> Hy, +1
+        real d = real(0);
+        if (z < nz - 1) {
+            d = ex_zp - e.x;
+            if ((CPML & 4) && hz.face >= 0) d = cpml_lane2<real, STAGE>(d, a.hf, hz, zc, 1, pz, write, slot[5]);
+        } else if (a.pec_mask & 4) {
+            d = -a.pec_factor[2] * e.x;
+        }
+        cy = cy + d;
+    }
+    return V3<real>{h.x - coefficient * cx, h.y - coefficient * cy, h.z - coefficient * cz};
+}
+
+constexpr int kTZ2 = 61;   // owned z columns of a two-step tile
+constexpr int kTY2 = 9;    // owned y rows of a two-step tile
+
+// One workgroup of 4 wavefronts advances a tile of 9 rows x 61 columns (y, z) over a chunk of x planes
+// by two steps.  Lane l covers column z0 - 1 + l; wavefront w covers the widened rows y0 - 1 + 3w + k
+// (k < 3).  Step n+1 is computed on the widened tile, step n+2 on the owned cells:
+//   stage 1   E1 = E(n+1) on rows y0-1 .. y0+10, lanes 0..63       (needs H(n) rows y0-2 .., staged)
+//   stage 1b  H1 = H(n+1) on rows y0-1 .. y0+9,  lanes 0..62       (needs E1 of the row and column above)
+//   stage 2   E2 = E(n+2) on rows y0 .. y0+9,    lanes 1..62       (needs H1 of the row and column below)
+//   stage 2b  H2 = H(n+2) on rows y0 .. y0+8,    lanes 1..61 = the owned cells
+// Along x the stages trail each other by a plane: iteration x computes E1(x), H1(x-1), E2(x-1) and
+// H2(x-2), so a chunk [xa, xb) runs x = xa-1 .. xb+1.  H(n) planes are staged in LDS by direct loads one
+// plane ahead; E1(x-1), H1(x-1) and E2(x-2) are exchanged between wavefronts through LDS, and every
+// value a cell needs from its own previous plane (H(n), E1, H1, E2, 1/eps, CPML memory) is carried in
+// registers.  Cells outside the grid are never computed or stored; their addresses are clamped.
+template <typename real, bool EPS3, int CPML>
+__device__ __forceinline__ void tile2_body(const March<real>& a, const int w, const int G) {
+    constexpr int W = 4, RW = 3, NR = W * RW;
+    constexpr int SR = NR + 1, SC = kTZ2 + 4, SH = SR * SC * 3;   // staged H(n): rows y0-2 .. y0+10, columns z0-2 .. z0+62
+    constexpr int PL = NR * 64 * 3 + 64 * 3 + 3;                 // one widened plane, padded past its last row
+    __shared__ real sA[SH];
+    __shared__ real sB[SH];
+    __shared__ real sE1[PL];   // E1(x-1)
+    __shared__ real sH1[PL];   // H1(x-1)
+    __shared__ real sE2[PL];   // E2(x-2)
+    constexpr bool INDEXED = !EPS3 && CPML == 0;
+    __shared__ real sMat[INDEXED ? 256 : 1];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    int ty, tz, chunk;
+    tile_coords<real, CPML>(a, w, G, ty, tz, chunk);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int y0 = ty * kTY2, z0 = tz * kTZ2 - a.zshift;
+    const int xa = chunk * a.chunk, xb = min(nx, xa + a.chunk);
+    const long long n = *global(a.counter);
+    const real* __restrict__ e_in = a.e_in;
+    const real* __restrict__ h_in = a.h_in;
+    real* __restrict__ e_out = a.e_out;
+    real* __restrict__ h_out = a.h_out;
+    const real* __restrict__ inv_eps = a.inv_eps;
+    const unsigned char* __restrict__ material = INDEXED ? a.material : nullptr;
+    if (INDEXED && material)
+        for (int i = tid; i < a.materials; i += 64 * W) sMat[i] = global(a.material_table)[i];   // seen after the first barrier
+    const real hcoef = a.courant * global(a.inv_mu)[0];
+    const unsigned plane = (unsigned)ny * nz;
+
+    const int z = z0 - 1 + lane;
+    const bool zin = z >= 0 && z < nz;
+    const unsigned zc = (unsigned)min(max(z, 0), nz - 1);
+    auto rowof = [&](int k) { return wave * RW + k; };
+    auto yof = [&](int k) { return y0 - 1 + wave * RW + k; };
+    const Hit none{-1, 0, 1, 0u, 0u, 0u};
+    const Hit ez_hit = (CPML & 4) && zin ? find_hit(a.ef, 2, z) : none;
+    const Hit hz_hit = (CPML & 4) && zin ? find_hit(a.hf, 2, z) : none;
+    const ZCoef<real> ez_coef = lane_coefficients(a.ef, ez_hit);
+    const ZCoef<real> hz_coef = lane_coefficients(a.hf, hz_hit);
+
+    // Staging of H(x) rows y0-2 .. y0+10, columns z0-2 .. z0+62 straight into LDS, in 4-byte words
+    // (clamped into the grid).  Wavefront w issues segments w, w + 4, ... of 64 consecutive words.
+    auto stage = [&](int x, real* into) {
+        unsigned* buffer = reinterpret_cast<unsigned*>(into);
+        const unsigned base = (unsigned)x * plane;
+        for (int seg = wave; seg * 64 < SH; seg += W) {
+            const int e = seg * 64 + lane;
+            if (e < SH) {
+                const int r = e / (3 * SC), c = e - r * (3 * SC);
+                const int y = min(max(y0 - 2 + r, 0), ny - 1);
+                const unsigned off = 3u * (unsigned)min(max(z0 - 2 + c / 3, 0), nz - 1) + (unsigned)(c % 3);
+                const unsigned* word = reinterpret_cast<const unsigned*>(h_in + 3u * (base + (unsigned)y * nz) + off);
+                __builtin_amdgcn_global_load_lds(global(word), (__attribute__((address_space(3))) void*)(buffer + seg * 64),
+                                                 4, 0, 0);
+            }
+        }
+    };
+    auto lds3 = [](const real* p) { return V3<real>{p[0], p[1], p[2]}; };
+    auto put3 = [](real* p, V3<real> v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; };
+    // A plain barrier after this wavefront's LDS accesses have completed: it does not wait for the
+    // direct loads of the next H plane.  The clobbers keep LDS accesses on their side of it.
+    auto lds_barrier = []() {
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_waitcnt(0xc07f);
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_sched_barrier(0);
+        asm volatile("" ::: "memory");
+    };
+
+    const int xs = max(xa - 1, 0), xe = xb + 1;
+    V3<real> Hn_c[RW], H1_c[RW], E1_c[RW], E2_c[RW], inv_c[RW];
+    real pe0[RW][6], pe1[RW][6], ph0[RW][6], ph1[RW][6];   // CPML memory at n+1: E1(x-1) / E1(x), H1(x-2) / H1(x-1)
+#pragma unroll
+    for (int k = 0; k < RW; ++k) {
+        Hn_c[k] = zero3<real>(); H1_c[k] = zero3<real>(); E1_c[k] = zero3<real>(); E2_c[k] = zero3<real>();
+        inv_c[k] = zero3<real>();
+#pragma unroll
+        for (int t = 0; t < 6; ++t) pe0[k][t] = pe1[k][t] = ph0[k][t] = ph1[k][t] = real(0);
+        if (xs > 0) Hn_c[k] = load3(h_in, 3u * ((unsigned)(xs - 1) * plane + (unsigned)min(max(yof(k), 0), ny - 1) * nz + zc));
+    }
+    auto march_plane = [&](const int x, real* const sHx, real* const sHn) {
+        const bool pE1 = x < nx;
+        const bool pH1 = x >= xa && x >= 1 && x - 1 < nx;
+        const bool pE2 = x - 1 >= xa && x - 1 <= xb && x - 1 < nx;
+        const bool pH2 = x - 2 >= xa && x - 2 < xb;
+        V3<real> eo[RW], inv[RW];
+        if (pE1) {
+            const unsigned base = (unsigned)x * plane;
+#pragma unroll
+            for (int k = 0; k < RW; ++k) {
+                const unsigned cell = base + (unsigned)min(max(yof(k), 0), ny - 1) * nz + zc;
+                eo[k] = load3(e_in, 3u * cell);
+                if (EPS3) {
+                    inv[k] = load3(inv_eps, 3u * cell);
+                } else {
+                    const real v = material ? sMat[global(material)[cell]] : global(inv_eps)[cell];
+                    inv[k] = V3<real>{v, v, v};
+                }
+            }
+        }
+        __syncthreads();   // H(n) of plane x staged; E1(x-1) and E2(x-2) written to LDS
+        if (x + 1 <= xe && x + 1 < nx) stage(x + 1, sHn);
+        V3<real> e1[RW], hn[RW], h1[RW], e2[RW];
+        // Stage 1: E1(x).
+        const Hit ex1 = find_hit(a.ef, 0, x);
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            const int r = rowof(k), y = yof(k);
+            const real* p = sHx + 3 * ((r + 1) * SC + lane + 1);
+            hn[k] = lds3(p);
+            e1[k] = zero3<real>();
+            if (pE1 && zin && y >= 0 && y < ny) {
+                const real* pym = p - 3 * SC;
+                const real* pzm = p - 3;
+                const Hit ey = find_hit(a.ef, 1, y);
+                e1[k] = e_cell2<real, CPML, 1>(a, x, y, z, false, ex1, ey, ez_hit, ez_coef, hn[k], Hn_c[k], lds3(pym), pzm[0],
+                                              pzm[1], eo[k], inv[k], pe1[k]);
+                add_sources(a.se, a.nsrc_e, a.ebox_lo, a.ebox_hi, n, a.source_samples, x, y, z, e1[k].x, e1[k].y, e1[k].z);
+            }
+        }
+        // Stage 1b: H1(x-1) from E1(x-1) (LDS, with the row and column above) and E1(x).
+        const Hit hx1 = find_hit(a.hf, 0, x - 1);
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            const int r = rowof(k), y = yof(k);
+            h1[k] = zero3<real>();
+            if (pH1 && r < NR - 1 && lane < 63 && zin && y >= 0 && y < ny) {
+                const real* p = sE1 + 3 * (r * 64 + lane);
+                const real* pzn = p + 3;
+                const Hit hy = find_hit(a.hf, 1, y);
+                h1[k] = h_cell2<real, CPML, 1>(a, x - 1, y, z, false, hcoef, hx1, hy, hz_hit, hz_coef, E1_c[k], pE1 ? e1[k] : E1_c[k],
+                                              lds3(p + 3 * 64), pzn[0], pzn[1], Hn_c[k], ph1[k]);
+                add_sources(a.sh, a.nsrc_h, a.hbox_lo, a.hbox_hi, n, a.source_samples, x - 1, y, z, h1[k].x, h1[k].y, h1[k].z);
+            }
+            put3(sH1 + 3 * (r * 64 + lane), h1[k]);
+        }
+        lds_barrier();   // H1(x-1) of every row in LDS
+        // Stage 2: E2(x-1); the owned cells (rows y0 .. y0+8, lanes 1..61, planes < xb) are stored.
+        const Hit ex2 = find_hit(a.ef, 0, x - 1);
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            const int r = rowof(k), y = yof(k);
+            e2[k] = zero3<real>();
+            if (pE2 && r >= 1 && r < NR - 1 && lane >= 1 && lane < 63 && zin && y >= 0 && y < ny) {
+                const real* p = sH1 + 3 * (r * 64 + lane);
+                const real* pzm = p - 3;
+                const bool owned = r < NR - 2 && lane < 62 && x - 1 < xb;
+                const Hit ey = find_hit(a.ef, 1, y);
+                e2[k] = e_cell2<real, CPML, 2>(a, x - 1, y, z, owned, ex2, ey, ez_hit, ez_coef, h1[k], H1_c[k], lds3(p - 3 * 64),
+                                              pzm[0], pzm[1], E1_c[k], inv_c[k], pe0[k]);
+                add_sources(a.se, a.nsrc_e, a.ebox_lo, a.ebox_hi, n + 1, a.source_samples, x - 1, y, z, e2[k].x, e2[k].y, e2[k].z);
+                if (owned) store3(e_out, 3u * ((unsigned)(x - 1) * plane + (unsigned)y * nz + (unsigned)z), e2[k]);
+            }
+        }
+        // Stage 2b: H2(x-2) from E2(x-2) (LDS, with the row and column above) and E2(x-1).
+        const Hit hx2 = find_hit(a.hf, 0, x - 2);
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            const int r = rowof(k), y = yof(k);
+            if (pH2 && r >= 1 && r < NR - 2 && lane >= 1 && lane < 62 && zin && y >= 0 && y < ny) {
+                const real* p = sE2 + 3 * (r * 64 + lane);
+                const real* pzn = p + 3;
+                const Hit hy = find_hit(a.hf, 1, y);
+                V3<real> h2 = h_cell2<real, CPML, 2>(a, x - 2, y, z, true, hcoef, hx2, hy, hz_hit, hz_coef, E2_c[k],
+                                                     pE2 ? e2[k] : E2_c[k], lds3(p + 3 * 64), pzn[0], pzn[1], H1_c[k], ph0[k]);
+                add_sources(a.sh, a.nsrc_h, a.hbox_lo, a.hbox_hi, n + 1, a.source_samples, x - 2, y, z, h2.x, h2.y, h2.z);
+                store3(h_out, 3u * ((unsigned)(x - 2) * plane + (unsigned)y * nz + (unsigned)z), h2);
+            }
+        }
+        lds_barrier();   // every read of sE1, sH1 and sE2 for this plane is done
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            const int r = rowof(k);
+            put3(sE1 + 3 * (r * 64 + lane), e1[k]);
+            put3(sE2 + 3 * (r * 64 + lane), e2[k]);
+            Hn_c[k] = hn[k];
+            H1_c[k] = h1[k];
+            E1_c[k] = e1[k];
+            E2_c[k] = e2[k];
+            inv_c[k] = pE1 ? inv[k] : zero3<real>();
+#pragma unroll
+            for (int t = 0; t < 6; ++t) {
+                pe0[k][t] = pe1[k][t];
+                ph0[k][t] = ph1[k][t];
+            }
+        }
+    };
+    if (xs < nx) stage(xs, sA);
+    for (int x = xs; x <= xe; x += 2) {
+        march_plane(x, sA, sB);
+        if (x + 1 <= xe) march_plane(x + 1, sB, sA);
     }
 }
 
@@ -786,6 +1165,99 @@ int launch_tile(const tfdtd_march* d, hipStream_t stream) {
     });
 }
 
+// ---------------------------------------------------------------------------------------------------
+// Two-step launches (float32).
+
+// Occupancy: the five LDS planes (51 KB) allow three workgroups per CU; the boundary tiles carry the
+// CPML memory of both stages of every axis in registers and may drop to two.
+template <bool EPS3, int CPML>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ? 2 : 3, 8))) void tile2_kernel(const March<float> a) {
+    tile2_body<float, EPS3, CPML>(a, blockIdx.x, gridDim.x);
+}
+
+// Tiles [lo, hi) along one axis whose cells of both stages avoid every CPML row of that axis: a tile
+// of `size` cells starting at s computes E on [s-1, s+size+2) and H on [s-1, s+size+1).
+inline void free_range2(int tiles, int size, int n, const Face* ef, const Face* hf, int* lo, int* hi, int shift = 0) {
+    auto clear = [&](int k) {
+        const int s0 = k * size - shift;
+        const int a0 = std::max(0, s0 - 1), e1 = std::min(n, s0 + size + 2), h1 = std::min(n, s0 + size + 1);
+        for (int f = 0; f < 2; ++f) {
+            if (ef[f].hi > ef[f].lo && ef[f].lo < e1 && a0 < ef[f].hi) return false;
+            if (hf[f].hi > hf[f].lo && hf[f].lo < h1 && a0 < hf[f].hi) return false;
+        }
+        return true;
+    };
+    int best_lo = 0, best_hi = 0;
+    for (int k = 0; k < tiles;) {
+        if (!clear(k)) { ++k; continue; }
+        int j = k;
+        while (j < tiles && clear(j)) ++j;
+        if (j - k > best_hi - best_lo) { best_lo = k; best_hi = j; }
+        k = j;
+    }
+    *lo = best_lo;
+    *hi = best_hi;
+}
+
+// Two-step tile layout: 9 x 61 tiles, chunks of chunk_x planes, the z shift and the slab-free boxes.
+int layout_march2(const tfdtd_march* d, March<float>* a) {
+    tfdtd_march in_place = *d;                  // make_march checks the one-step (in-place H memory) form
+    in_place.h.psi_out = const_cast<void*>(d->h.psi_in);
+    int status = make_march(&in_place, a);
+    if (status) return status;
+    bool h_slabs = false;
+    for (int ax = 0; ax < 3; ++ax)
+        for (int f = 0; f < 2; ++f) h_slabs |= a->hf.face[ax][f].hi > a->hf.face[ax][f].lo;
+    if (h_slabs && (!d->h.psi_out || d->h.psi_out == d->h.psi_in))
+        return tfdtd_fail(TFDTD_EINVAL, "march2: the H-family CPML memory needs separate input and output buffers");
+    a->hf.psi_out = static_cast<float*>(h_slabs ? d->h.psi_out : const_cast<void*>(d->h.psi_in));
+    const int chunk = d->chunk_x > 0 ? d->chunk_x : d->nx;
+    a->chunk = chunk;
+    a->ytiles = (d->ny + kTY2 - 1) / kTY2;
+    // Shift the z tiles so one tile's widened columns start just past the low z slabs, when that lowers
+    // the cost of a z column (slab-free tiles weigh 10, tiles with z slab columns 17).
+    const int zend = std::max(a->ef.face[2][0].hi, a->hf.face[2][0].hi);
+    auto column_cost = [&](int shift) {
+        const int tiles = (d->nz + shift + kTZ2 - 1) / kTZ2;
+        int lo, hi;
+        free_range2(tiles, kTZ2, d->nz, a->ef.face[2], a->hf.face[2], &lo, &hi, shift);
+        return 10 * (hi - lo) + 17 * (tiles - (hi - lo));
+    };
+    const int aligned = zend > 0 ? (kTZ2 - (zend + 1) % kTZ2) % kTZ2 : 0;
+    a->zshift = column_cost(aligned) < column_cost(0) ? aligned : 0;
+    a->ztiles = (d->nz + a->zshift + kTZ2 - 1) / kTZ2;
+    const int chunks = (d->nx + chunk - 1) / chunk;
+    const long long blocks = (long long)chunks * a->ytiles * a->ztiles;
+    if (blocks > 0x7fffffffLL) return tfdtd_fail(TFDTD_EINVAL, "march2: too many workgroups");
+    free_range2(chunks, chunk, d->nx, a->ef.face[0], a->hf.face[0], &a->box_lo[0], &a->box_hi[0]);
+    free_range2(a->ytiles, kTY2, d->ny, a->ef.face[1], a->hf.face[1], &a->box_lo[1], &a->box_hi[1]);
+    a->nchunks = chunks;
+    free_range2(a->ztiles, kTZ2, d->nz, a->ef.face[2], a->hf.face[2], &a->zbox_lo, &a->zbox_hi, a->zshift);
+    const long long xy = (long long)(a->box_hi[0] - a->box_lo[0]) * (a->box_hi[1] - a->box_lo[1]);
+    const long long pure = xy * (a->zbox_hi - a->zbox_lo), inner = xy * a->ztiles;
+    if (inner == 0) { a->box_lo[0] = a->box_hi[0] = 0; a->box_lo[1] = a->box_hi[1] = 0; }
+    a->blocks[0] = (int)pure;
+    a->blocks[1] = (int)(inner - pure);
+    a->blocks[2] = (int)(blocks - inner);
+    return TFDTD_OK;
+}
+
+int launch_tile2(const tfdtd_march* d, hipStream_t stream) {
+    March<float> a;
+    int status = layout_march2(d, &a);
+    if (status) return status;
+    const bool eps3 = d->eps_stride == 3;
+    return launch_variants(a.blocks, stream, [&](int v, hipStream_t s) {
+        auto go = [&](auto kernel, const char* what) {
+            hipLaunchKernelGGL(kernel, dim3((unsigned)a.blocks[v]), dim3(256), 0, s, a);
+            return tfdtd_check_launch(what);
+        };
+        if (v == 0) return eps3 ? go(tile2_kernel<true, 0>, "march2 interior") : go(tile2_kernel<false, 0>, "march2 interior");
+        if (v == 1) return eps3 ? go(tile2_kernel<true, 4>, "march2 z faces") : go(tile2_kernel<false, 4>, "march2 z faces");
+        return eps3 ? go(tile2_kernel<true, 7>, "march2 boundary") : go(tile2_kernel<false, 7>, "march2 boundary");
+    });
+}
+
 template <typename real, int W>
 int pack_marches(const tfdtd_march* d, int count, void* host_table, size_t bytes, int32_t* prefix) {
     if (!host_table || bytes < sizeof(March<real>) * (size_t)count) return tfdtd_fail(TFDTD_EINVAL, "batch march: table buffer too small");
@@ -847,6 +1319,12 @@ extern "C" int tfdtd_yee_step_march(const tfdtd_march* d, void* stream) {
     if (d->dtype == TFDTD_F32) return dispatch_march<float>(d, s);
     if (d->dtype == TFDTD_F64) return dispatch_march<double>(d, s);
     return tfdtd_fail(TFDTD_EINVAL, "march: dtype must be float32 or float64");
+}
+
+extern "C" int tfdtd_yee_step_march2(const tfdtd_march* d, void* stream) {
+    if (!d) return tfdtd_fail(TFDTD_EINVAL, "null march descriptor");
+    if (d->dtype != TFDTD_F32) return tfdtd_fail(TFDTD_EINVAL, "march2: float32 only");
+    return launch_tile2(d, static_cast<hipStream_t>(stream));
 }
 
 extern "C" size_t tfdtd_batch_march_bytes(int32_t count, int32_t dtype) {

@@ -33,11 +33,12 @@ class StepGraphs:
     """Capture `step` for 1 and `width` steps on a side stream; `states` are zeroed after warm-up and capture.
 
     With `pair` (two steps that return the state to its own buffers) the `width` graph
-    is built from pairs, plus one `step` when `width` is odd.  `prepare`, when given, runs
+    is built from pairs, plus one `step` when `width` is odd; with `quad` as well (four steps
+    that return the state to its own buffers) from quads first.  `prepare`, when given, runs
     before every replay (e.g. Stepper.refresh_materials, which follows permittivity changes).
     """
 
-    def __init__(self, step, states, steps, width=1, warmup=3, pair=None, prepare=None):
+    def __init__(self, step, states, steps, width=1, warmup=3, pair=None, prepare=None, quad=None):
         self.prepare = prepare
         self.width = min(validate_graph_steps(width), steps)
         states = list(states)
@@ -48,6 +49,8 @@ class StepGraphs:
                 step()
             if pair is not None and steps >= 2:
                 pair()      # the pair's first launches happen outside capture too
+            if quad is not None and steps >= 4:
+                quad()
             for value in states:
                 value.zero_()
         torch.cuda.current_stream().wait_stream(stream)
@@ -58,7 +61,12 @@ class StepGraphs:
             with torch.cuda.stream(stream):
                 graph.capture_begin()
                 if pair is not None and count > 1:
-                    for _ in range(count // 2):
+                    rest = count
+                    if quad is not None:
+                        for _ in range(rest // 4):
+                            quad()
+                        rest %= 4
+                    for _ in range(rest // 2):
                         pair()
                     if count % 2:
                         step()

@@ -233,9 +233,10 @@ class MaterialIndex:
 
 
 def march_descriptor(grid, e_in, h_in, e_out, h_out, psi_e_in, psi_e_out, sources, counter, tile_y=8, chunk_x=8,
-                     materials=None):
+                     materials=None, psi_h=None):
     """tfdtd_march: one whole step reading (e_in, h_in, psi_e_in) and writing (e_out, h_out, psi_e_out);
-    `materials` (a MaterialIndex) lets the step read 1/eps through the one-byte index."""
+    `materials` (a MaterialIndex) lets the step read 1/eps through the one-byte index.  The H-family
+    CPML memory updates in place unless `psi_h` = (input, output) is given (the two-step march)."""
     d = N.March()
     d.dtype = N.F64 if grid.dtype == torch.float64 else N.F32
     d.nx, d.ny, d.nz = grid.Nx, grid.Ny, grid.Nz
@@ -248,7 +249,8 @@ def march_descriptor(grid, e_in, h_in, e_out, h_out, psi_e_in, psi_e_out, source
     for a, f in grid.pec_upper.items():
         d.pec_upper_factor[a] = f
     d.e = march_family(grid, False, psi_e_in, psi_e_out)
-    d.h = march_family(grid, True, grid.psi_buffers[True], grid.psi_buffers[True])
+    psi_h_in, psi_h_out = psi_h if psi_h is not None else (grid.psi_buffers[True],)*2
+    d.h = march_family(grid, True, psi_h_in, psi_h_out)
     samples = 0
     for family, count_name, table_name in (('E', 'nsrc_e', 'srcs_e'), ('H', 'nsrc_h', 'srcs_h')):
         entries = sources.get(family, ())

@@ -138,7 +138,8 @@ def pack_table(count_bytes, pack, descriptors, prefix_rows, device):
 class Stepper:
     """Device state of one resident run and its step (the reference's closure, solver.py:776-797)."""
 
-    def __init__(self, project, steps=None, kernel=None, one_pass=None, tile_y=8, chunk_x=6, counter=None, epsilon=None):
+    def __init__(self, project, steps=None, kernel=None, one_pass=None, tile_y=8, chunk_x=6, counter=None, epsilon=None,
+                 two_step=None, chunk_x2=20):
         p, r = project, project.region
         self.project = p
         self.steps = r.steps if steps is None else steps
@@ -169,15 +170,24 @@ class Stepper:
         self.one_pass = supported if one_pass is None else bool(one_pass)
         if self.one_pass and not supported:
             raise ValueError('the one-pass kernel needs a 3D grid without periodic axes and at most 8 sources per family')
-        self.tile_y, self.chunk_x = tile_y, chunk_x
+        self.tile_y, self.chunk_x, self.chunk_x2 = tile_y, chunk_x, chunk_x2
+        # Two steps per pass (float32, one-pass grids without plane monitors or traces, whose
+        # intermediate steps then never need to reach memory).
+        self.two_step = self._two_step_allowed() if two_step is None else bool(two_step)
+        if self.two_step and not self._two_step_allowed():
+            raise ValueError('the two-step kernel needs a float32 one-pass grid without plane monitors or point traces')
         # One-byte material index of 1/eps for the one-pass step (grids with few distinct values).
         self.materials = MaterialIndex(g) if self.one_pass else None
         self._describe()
+
+    def _two_step_allowed(self):
+        return (self.one_pass and self.grid.dtype == torch.float32 and not self.planes and not self.trace_points)
 
     def set_trace_points(self, points):
         """Replace the recorded traces by `points`: (magnetic, flat element index) pairs."""
         self.trace_points = [(bool(m), int(flat)) for m, flat in points]
         self.traces = torch.zeros((self.steps, len(self.trace_points)), device=self.grid.device, dtype=self.grid.dtype)
+        self.two_step = self.two_step and self._two_step_allowed()   # traces need every step in memory
         self._describe()
 
     def _trace_table(self, E, H):
@@ -228,12 +238,24 @@ class Stepper:
                 table, prefix, host = pack_table(lambda n: self.lib.tfdtd_batch_march_bytes(n, dtype),
                                                  self.lib.tfdtd_batch_march_pack, [d], 3, g.device)
                 self.march_tables.append((table, prefix, (ctypes.c_int32*3)(*host[:, 1].tolist())))
+            if self.two_step:
+                # The two-step pass double-buffers the H-family CPML memory as well.
+                self.psi_h_b = torch.zeros_like(g.psi_buffers[True])
+                psi_h = g.psi_buffers[True]
+                self.march2 = [march_descriptor(g, g.E, g.H, self.E_b, self.H_b, psi_a, self.psi_e_b, self.sources,
+                                                self.counter, self.tile_y, self.chunk_x2, self.materials,
+                                                psi_h=(psi_h, self.psi_h_b)),
+                               march_descriptor(g, self.E_b, self.H_b, g.E, g.H, self.psi_e_b, psi_a, self.sources,
+                                                self.counter, self.tile_y, self.chunk_x2, self.materials,
+                                                psi_h=(self.psi_h_b, psi_h))]
             self.plane_table_b = self._planes(self.E_b, self.H_b)
             self.trace_table_b = self._trace_table(self.E_b, self.H_b)
             self.observe_b = self._observe(self.trace_table_b)
 
     def states(self):
         extra = [self.E_b, self.H_b, self.psi_e_b] if self.one_pass else []
+        if self.two_step:
+            extra.append(self.psi_h_b)
         return [self.grid.E, self.grid.H, self.counter, self.traces, *self.grid.memory_states, *extra]
 
     def refresh_materials(self):
@@ -256,6 +278,8 @@ class Stepper:
             fresh, _, _ = pack_table(lambda n: self.lib.tfdtd_batch_march_bytes(n, dtype), self.lib.tfdtd_batch_march_pack,
                                      [d], 3, self.grid.device)
             table.copy_(fresh)
+        for d in getattr(self, 'march2', ()) if self.two_step else ():
+            m.describe(d)
         return True
 
     def march_step(self, k, stream):
@@ -265,6 +289,20 @@ class Stepper:
         d = self.march[k]
         N.check(self.lib.tfdtd_batch_march(table.data_ptr(), prefix.data_ptr(), 1, d.dtype, self.tile_y, d.eps_stride,
                                            totals, stream), 'tfdtd_batch_march')
+
+    def march2_step(self, k, stream):
+        """Two steps in one pass from buffer set k (0: grid buffers -> second set, 1: back); the
+        counter is not advanced."""
+        self.refresh_materials()
+        N.check(self.lib.tfdtd_yee_step_march2(ctypes.byref(self.march2[k]), stream), 'tfdtd_yee_step_march2')
+
+    def quad(self):
+        """Four steps with the two-step kernel: grid buffers -> second set -> grid buffers."""
+        stream = N.stream_handle(self.grid.device)
+        for k in (0, 1):
+            self.march2_step(k, stream)
+            for _ in range(2):     # no traces: the observe launch only advances the counter
+                N.check(self.lib.tfdtd_observe_step(ctypes.byref(self.observe), stream), 'tfdtd_observe_step')
 
     def step_pair(self):
         """Two steps with the one-pass kernel: grid buffers -> second set -> grid buffers."""
@@ -277,7 +315,12 @@ class Stepper:
             N.check(lib.tfdtd_observe_step(ctypes.byref(observe), stream), 'tfdtd_observe_step')
 
     def advance(self, count):
-        """`count` steps, eagerly: pairs through the one-pass kernel where available, else single steps."""
+        """`count` steps, eagerly: fours through the two-step kernel and pairs through the one-pass kernel
+        where available, else single steps."""
+        if self.two_step:
+            for _ in range(count // 4):
+                self.quad()
+            count %= 4
         if self.one_pass:
             for _ in range(count // 2):
                 self.step_pair()
@@ -337,7 +380,8 @@ class Simulation:
         diagnostics = StateDiagnostics(g) if control.auto_shutoff or control.divergence_check else None
         decision = DecayDecision(control, source_end_time(p), g.time_step)
         graph = (StepGraphs(run.step, run.states(), r.steps, cuda_graph_steps, pair=run.step_pair if run.one_pass else None,
-                            prepare=run.refresh_materials if run.one_pass else None)
+                            prepare=run.refresh_materials if run.one_pass else None,
+                            quad=run.quad if run.two_step else None)
                  if cuda_graph else None)
         view = DisplaySlice(p)
         plane_index = view.index
