@@ -176,15 +176,17 @@ typedef struct tfdtd_march {
 int tfdtd_yee_step_march(const tfdtd_march* step, void* stream);
 
 /* Two whole Yee steps, n -> n+2, in one pass over HBM (float32): the step closure of
- * torchfdtd/solver.py:785-797 twice without the observers, i.e. tfdtd_yee_step_march from
- * the *_in set into the *_out set and back, with the intermediate step kept on chip.  The
- * result -- E, H and both CPML memories -- equals two calls of tfdtd_yee_step_march bit for bit.
+ * torchfdtd/solver.py:785-797 twice without the observers, with the intermediate step kept
+ * on chip.  The result -- E, H and both CPML memories -- equals two calls of
+ * tfdtd_yee_step_march (*_in -> *_out -> *_in) bit for bit, but lands in the *_out set.
  * Time n is read from the *_in buffers and n+2 written to the *_out buffers; unlike the one-step
  * call, the H family's CPML memory is double-buffered too (h.psi_out must differ from h.psi_in
  * when the grid has H slabs), because a tile recomputes step n+1 on cells its neighbours own.
  * Sources are added at both steps (samples n and n+1 of the device counter, which the call reads
- * and does not advance).  chunk_x is the planes per workgroup (about 20 suits a 512^3 grid);
- * tile_y is ignored (tiles are 9 rows x 61 columns).  Concurrency as tfdtd_yee_step_march. */
+ * and does not advance).  Tiles are 9 rows x 61 columns; chunk_x is the planes per workgroup
+ * (about 20 suits a 512^3 grid) and tile_y the wavefronts of an interior workgroup, 4, 6 or 12
+ * (0 = 12), which split the tile's 12 widened rows (z-face and boundary tiles use 12).
+ * Concurrency as tfdtd_yee_step_march. */
 int tfdtd_yee_step_march2(const tfdtd_march* step, void* stream);
 
 /* torchfdtd/cuda_kernels.py:349-365 (FusedYeeCUDA.update_E / update_H), with the

@@ -20,6 +20,9 @@
 // CPML slab membership is uniform per plane along x, uniform per row along y and
 // fixed per lane along z; the z-slab coefficients are loaded once per lane.
 //
+// The two-step pass (tile2_body, further down) advances two steps per pass over HBM: each tile
+// recomputes step n+1 on a widened region and keeps it on chip.
+//
 // Every product and sum rounds in the reference's order (torchfdtd/boundaries.py:363-438),
 // so the result equals the split half-step kernels and the NumPy path bit for bit.
 #include <hip/hip_runtime.h>
@@ -393,8 +396,11 @@ __device__ __forceinline__ void tile_body(const March<real>& a, const int w, con
     real* __restrict__ h_out = a.h_out;
     const real* __restrict__ inv_eps = a.inv_eps;
     const unsigned char* __restrict__ material = INDEXED ? a.material : nullptr;
-    if (INDEXED && material)
-        for (int i = tid; i < a.materials; i += 64 * W) sMat[i] = global(a.material_table)[i];   // seen after the first barrier
+    if (INDEXED) {
+        if (material)
+            for (int i = tid; i < a.materials; i += 64 * W) sMat[i] = global(a.material_table)[i];
+        __syncthreads();   // the first plane's loads read the table
+    }
     const real hcoef = a.courant * global(a.inv_mu)[0];
     const unsigned plane = (unsigned)ny * nz;
 
@@ -717,9 +723,9 @@ __device__ __forceinline__ V3<real> h_cell2(const March<real>& a, int x, int y, 
 constexpr int kTZ2 = 61;   // owned z columns of a two-step tile
 constexpr int kTY2 = 9;    // owned y rows of a two-step tile
 
-// One workgroup of 4 wavefronts advances a tile of 9 rows x 61 columns (y, z) over a chunk of x planes
-// by two steps.  Lane l covers column z0 - 1 + l; wavefront w covers the widened rows y0 - 1 + 3w + k
-// (k < 3).  Step n+1 is computed on the widened tile, step n+2 on the owned cells:
+// One workgroup of W = 4, 6 or 12 wavefronts advances a tile of 9 rows x 61 columns (y, z) over a chunk
+// of x planes by two steps.  Lane l covers column z0 - 1 + l; wavefront w covers the widened rows
+// y0 - 1 + RW w + k (k < RW = 12 / W).  Step n+1 is computed on the widened tile, step n+2 on the owned cells:
 //   stage 1   E1 = E(n+1) on rows y0-1 .. y0+10, lanes 0..63       (needs H(n) rows y0-2 .., staged)
 //   stage 1b  H1 = H(n+1) on rows y0-1 .. y0+9,  lanes 0..62       (needs E1 of the row and column above)
 //   stage 2   E2 = E(n+2) on rows y0 .. y0+9,    lanes 1..62       (needs H1 of the row and column below)
@@ -729,9 +735,10 @@ constexpr int kTY2 = 9;    // owned y rows of a two-step tile
 // plane ahead; E1(x-1), H1(x-1) and E2(x-2) are exchanged between wavefronts through LDS, and every
 // value a cell needs from its own previous plane (H(n), E1, H1, E2, 1/eps, CPML memory) is carried in
 // registers.  Cells outside the grid are never computed or stored; their addresses are clamped.
-template <typename real, bool EPS3, int CPML>
+template <typename real, bool EPS3, int CPML, int W>
 __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, const int G) {
-    constexpr int W = 4, RW = 3, NR = W * RW;
+    constexpr int NR = 12, RW = NR / W;   // widened rows, rows per wavefront
+    static_assert(RW * W == NR, "the wavefronts split the 12 widened rows evenly");
     constexpr int SR = NR + 1, SC = kTZ2 + 4, SH = SR * SC * 3;   // staged H(n): rows y0-2 .. y0+10, columns z0-2 .. z0+62
     constexpr int PL = NR * 64 * 3 + 64 * 3 + 3;                 // one widened plane, padded past its last row
     __shared__ real sA[SH];
@@ -754,8 +761,11 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
     real* __restrict__ h_out = a.h_out;
     const real* __restrict__ inv_eps = a.inv_eps;
     const unsigned char* __restrict__ material = INDEXED ? a.material : nullptr;
-    if (INDEXED && material)
-        for (int i = tid; i < a.materials; i += 64 * W) sMat[i] = global(a.material_table)[i];   // seen after the first barrier
+    if (INDEXED) {
+        if (material)
+            for (int i = tid; i < a.materials; i += 64 * W) sMat[i] = global(a.material_table)[i];
+        __syncthreads();   // the first plane's loads read the table
+    }
     const real hcoef = a.courant * global(a.inv_mu)[0];
     const unsigned plane = (unsigned)ny * nz;
 
@@ -810,28 +820,47 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
         for (int t = 0; t < 6; ++t) pe0[k][t] = pe1[k][t] = ph0[k][t] = ph1[k][t] = real(0);
         if (xs > 0) Hn_c[k] = load3(h_in, 3u * ((unsigned)(xs - 1) * plane + (unsigned)min(max(yof(k), 0), ny - 1) * nz + zc));
     }
+    // E(n) and 1/eps (or its material index) of a plane's cells.  The lean variants load them one plane
+    // ahead, right after the barrier, so that the loads overlap a plane of work instead of the barrier
+    // that follows them; the boundary tiles (register-bound) load them just before that barrier.
+    constexpr bool PREFETCH = CPML != 7;
+    V3<real> eo_n[RW], inv3_n[RW];
+    real inv1_n[RW];
+    unsigned idx_n[RW];
+    auto load_plane = [&](int x) {
+        const unsigned base = (unsigned)x * plane;
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            const unsigned cell = base + (unsigned)min(max(yof(k), 0), ny - 1) * nz + zc;
+            eo_n[k] = load3(e_in, 3u * cell);
+            if (EPS3) inv3_n[k] = load3(inv_eps, 3u * cell);
+            else if (material) idx_n[k] = global(material)[cell];
+            else inv1_n[k] = global(inv_eps)[cell];
+        }
+    };
+    if (PREFETCH && xs < nx) load_plane(xs);
     auto march_plane = [&](const int x, real* const sHx, real* const sHn) {
         const bool pE1 = x < nx;
         const bool pH1 = x >= xa && x >= 1 && x - 1 < nx;
         const bool pE2 = x - 1 >= xa && x - 1 <= xb && x - 1 < nx;
         const bool pH2 = x - 2 >= xa && x - 2 < xb;
+        if (!PREFETCH && pE1) load_plane(x);
+        __syncthreads();   // H(n) of plane x staged; E1(x-1) and E2(x-2) written to LDS
+        if (x + 1 <= xe && x + 1 < nx) stage(x + 1, sHn);
         V3<real> eo[RW], inv[RW];
         if (pE1) {
-            const unsigned base = (unsigned)x * plane;
 #pragma unroll
             for (int k = 0; k < RW; ++k) {
-                const unsigned cell = base + (unsigned)min(max(yof(k), 0), ny - 1) * nz + zc;
-                eo[k] = load3(e_in, 3u * cell);
+                eo[k] = eo_n[k];
                 if (EPS3) {
-                    inv[k] = load3(inv_eps, 3u * cell);
+                    inv[k] = inv3_n[k];
                 } else {
-                    const real v = material ? sMat[global(material)[cell]] : global(inv_eps)[cell];
+                    const real v = material ? sMat[idx_n[k]] : inv1_n[k];
                     inv[k] = V3<real>{v, v, v};
                 }
             }
         }
-        __syncthreads();   // H(n) of plane x staged; E1(x-1) and E2(x-2) written to LDS
-        if (x + 1 <= xe && x + 1 < nx) stage(x + 1, sHn);
+        if (PREFETCH && x + 1 <= xe && x + 1 < nx) load_plane(x + 1);
         V3<real> e1[RW], hn[RW], h1[RW], e2[RW];
         // Stage 1: E1(x).
         const Hit ex1 = find_hit(a.ef, 0, x);
@@ -1168,11 +1197,12 @@ int launch_tile(const tfdtd_march* d, hipStream_t stream) {
 // ---------------------------------------------------------------------------------------------------
 // Two-step launches (float32).
 
-// Occupancy: the five LDS planes (51 KB) allow three workgroups per CU; the boundary tiles carry the
-// CPML memory of both stages of every axis in registers and may drop to two.
-template <bool EPS3, int CPML>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CPML == 7 ? 2 : 3, 8))) void tile2_kernel(const March<float> a) {
-    tile2_body<float, EPS3, CPML>(a, blockIdx.x, gridDim.x);
+// The five LDS planes (51 KB) allow three workgroups per CU.  More wavefronts per workgroup hold fewer
+// rows each, hence fewer registers: the boundary tiles carry the CPML memory of both stages of every
+// axis in registers.
+template <bool EPS3, int CPML, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 ? 6 : 1, 8))) void tile2_kernel(const March<float> a) {
+    tile2_body<float, EPS3, CPML, W>(a, blockIdx.x, gridDim.x);
 }
 
 // Tiles [lo, hi) along one axis whose cells of both stages avoid every CPML row of that axis: a tile
@@ -1242,20 +1272,31 @@ int layout_march2(const tfdtd_march* d, March<float>* a) {
     return TFDTD_OK;
 }
 
-int launch_tile2(const tfdtd_march* d, hipStream_t stream) {
+// W wavefronts per interior tile; the z-face and boundary tiles (more registers per row) use 12.
+template <int W>
+int launch_tile2(const March<float>& a, bool eps3, hipStream_t stream) {
+    return launch_variants(a.blocks, stream, [&](int v, hipStream_t s) {
+        auto go = [&](auto kernel, int waves, const char* what) {
+            hipLaunchKernelGGL(kernel, dim3((unsigned)a.blocks[v]), dim3(64 * waves), 0, s, a);
+            return tfdtd_check_launch(what);
+        };
+        if (v == 0) return eps3 ? go(tile2_kernel<true, 0, W>, W, "march2 interior") : go(tile2_kernel<false, 0, W>, W, "march2 interior");
+        if (v == 1) return eps3 ? go(tile2_kernel<true, 4, 12>, 12, "march2 z faces") : go(tile2_kernel<false, 4, 12>, 12, "march2 z faces");
+        return eps3 ? go(tile2_kernel<true, 7, 12>, 12, "march2 boundary") : go(tile2_kernel<false, 7, 12>, 12, "march2 boundary");
+    });
+}
+
+int dispatch_tile2(const tfdtd_march* d, hipStream_t stream) {
     March<float> a;
     int status = layout_march2(d, &a);
     if (status) return status;
     const bool eps3 = d->eps_stride == 3;
-    return launch_variants(a.blocks, stream, [&](int v, hipStream_t s) {
-        auto go = [&](auto kernel, const char* what) {
-            hipLaunchKernelGGL(kernel, dim3((unsigned)a.blocks[v]), dim3(256), 0, s, a);
-            return tfdtd_check_launch(what);
-        };
-        if (v == 0) return eps3 ? go(tile2_kernel<true, 0>, "march2 interior") : go(tile2_kernel<false, 0>, "march2 interior");
-        if (v == 1) return eps3 ? go(tile2_kernel<true, 4>, "march2 z faces") : go(tile2_kernel<false, 4>, "march2 z faces");
-        return eps3 ? go(tile2_kernel<true, 7>, "march2 boundary") : go(tile2_kernel<false, 7>, "march2 boundary");
-    });
+    switch (d->tile_y) {
+        case 4: return launch_tile2<4>(a, eps3, stream);
+        case 6: return launch_tile2<6>(a, eps3, stream);
+        case 0: case 12: return launch_tile2<12>(a, eps3, stream);
+        default: return tfdtd_fail(TFDTD_EINVAL, "march2: tile_y (wavefronts of an interior workgroup) must be 4, 6, 12 or 0");
+    }
 }
 
 template <typename real, int W>
@@ -1324,7 +1365,7 @@ extern "C" int tfdtd_yee_step_march(const tfdtd_march* d, void* stream) {
 extern "C" int tfdtd_yee_step_march2(const tfdtd_march* d, void* stream) {
     if (!d) return tfdtd_fail(TFDTD_EINVAL, "null march descriptor");
     if (d->dtype != TFDTD_F32) return tfdtd_fail(TFDTD_EINVAL, "march2: float32 only");
-    return launch_tile2(d, static_cast<hipStream_t>(stream));
+    return dispatch_tile2(d, static_cast<hipStream_t>(stream));
 }
 
 extern "C" size_t tfdtd_batch_march_bytes(int32_t count, int32_t dtype) {

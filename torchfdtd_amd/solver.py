@@ -139,7 +139,7 @@ class Stepper:
     """Device state of one resident run and its step (the reference's closure, solver.py:776-797)."""
 
     def __init__(self, project, steps=None, kernel=None, one_pass=None, tile_y=8, chunk_x=6, counter=None, epsilon=None,
-                 two_step=None, chunk_x2=20):
+                 two_step=None, chunk_x2=20, waves2=12):
         p, r = project, project.region
         self.project = p
         self.steps = r.steps if steps is None else steps
@@ -170,7 +170,8 @@ class Stepper:
         self.one_pass = supported if one_pass is None else bool(one_pass)
         if self.one_pass and not supported:
             raise ValueError('the one-pass kernel needs a 3D grid without periodic axes and at most 8 sources per family')
-        self.tile_y, self.chunk_x, self.chunk_x2 = tile_y, chunk_x, chunk_x2
+        self.tile_y, self.chunk_x = tile_y, chunk_x
+        self.chunk_x2, self.waves2 = chunk_x2, waves2      # two-step pass: planes per chunk, wavefronts per tile
         # Two steps per pass (float32, one-pass grids without plane monitors or traces, whose
         # intermediate steps then never need to reach memory).
         self.two_step = self._two_step_allowed() if two_step is None else bool(two_step)
@@ -243,10 +244,10 @@ class Stepper:
                 self.psi_h_b = torch.zeros_like(g.psi_buffers[True])
                 psi_h = g.psi_buffers[True]
                 self.march2 = [march_descriptor(g, g.E, g.H, self.E_b, self.H_b, psi_a, self.psi_e_b, self.sources,
-                                                self.counter, self.tile_y, self.chunk_x2, self.materials,
+                                                self.counter, self.waves2, self.chunk_x2, self.materials,
                                                 psi_h=(psi_h, self.psi_h_b)),
                                march_descriptor(g, self.E_b, self.H_b, g.E, g.H, self.psi_e_b, psi_a, self.sources,
-                                                self.counter, self.tile_y, self.chunk_x2, self.materials,
+                                                self.counter, self.waves2, self.chunk_x2, self.materials,
                                                 psi_h=(self.psi_h_b, psi_h))]
             self.plane_table_b = self._planes(self.E_b, self.H_b)
             self.trace_table_b = self._trace_table(self.E_b, self.H_b)
