@@ -189,6 +189,17 @@ int tfdtd_yee_step_march(const tfdtd_march* step, void* stream);
  * Concurrency as tfdtd_yee_step_march. */
 int tfdtd_yee_step_march2(const tfdtd_march* step, void* stream);
 
+/* The two-step pass of one or many grids from a device table, as tfdtd_batch_march_pack /
+ * tfdtd_batch_march for the one-step kernel: tfdtd_batch_march2_pack validates `count` float32
+ * descriptors of one tile_y and eps_stride (the layout of tfdtd_yee_step_march2), writes the table
+ * (tfdtd_batch_march_bytes(count, TFDTD_F32) bytes) and block_prefix[3][count+1]; the caller copies
+ * both to device memory once and launches tfdtd_batch_march2 for every pass with totals[v] =
+ * block_prefix[v][count].  The tile parameters are then read from device memory instead of being
+ * held in kernel-argument registers. */
+int tfdtd_batch_march2_pack(const tfdtd_march* steps, int32_t count, void* host_table, size_t bytes, int32_t* block_prefix);
+int tfdtd_batch_march2(const void* device_table, const int32_t* device_prefix, int32_t count, int32_t tile_y,
+                       int32_t eps_stride, const int32_t* totals, void* stream);
+
 /* torchfdtd/cuda_kernels.py:349-365 (FusedYeeCUDA.update_E / update_H), with the
  * family's sources of torchfdtd/solver.py:778-783 folded in when nsrc > 0. */
 int tfdtd_yee_half_step(const tfdtd_half_step* step, void* stream);

@@ -199,6 +199,37 @@ def test_two_step_pass_with_sources_equals_one_step_pairs_under_graphs(gpu):
         assert torch.equal(a, b)
 
 
+def test_two_step_entry_points_agree_bitwise(gpu):
+    """tfdtd_yee_step_march2 (descriptor in kernel arguments) and tfdtd_batch_march2 (device table)
+    produce the same pass from the same state: fields and both CPML memories."""
+    import ctypes
+    from torchfdtd_amd import Project
+    from torchfdtd_amd import _native as N
+    from torchfdtd_amd.solver import Stepper
+    region = region_for('pml3d', 'float32')
+    region.size = (4.4, 7.4, 13.1)
+    run = Stepper(Project(region=region), steps=40, one_pass=True, two_step=True)
+    g = run.grid
+    gen = torch.Generator(device=g.device).manual_seed(5)
+    E0 = torch.randn(g.E.shape, device=g.device, generator=gen)*.02
+    H0 = torch.randn(g.H.shape, device=g.device, generator=gen)*.02
+    stream = N.stream_handle(g.device)
+    outs = []
+    for launch in (lambda: run.march2_step(0, stream),
+                   lambda: N.check(run.lib.tfdtd_yee_step_march2(ctypes.byref(run.march2[0]), stream), 'march2')):
+        g.E.copy_(E0)
+        g.H.copy_(H0)
+        for t in g.memory_states:
+            t.fill_(.01)
+        run.E_b.zero_(), run.H_b.zero_(), run.psi_e_b.zero_(), run.psi_h_b.zero_()
+        launch()
+        torch.cuda.synchronize()
+        outs.append([t.clone() for t in (run.E_b, run.H_b, run.psi_e_b, run.psi_h_b)])
+    assert float(outs[0][3].abs().max()) > 0
+    for a, b in zip(*outs):
+        assert torch.equal(a, b)
+
+
 def test_subnormal_fields_are_preserved(gpu):
     from torchfdtd_amd import Project, Region
     from torchfdtd_amd.grid import HipYee, YeeGrid

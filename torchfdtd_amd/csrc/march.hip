@@ -1205,6 +1205,25 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
     tile2_body<float, EPS3, CPML, W>(a, blockIdx.x, gridDim.x);
 }
 
+// Many grids (or one) from a device table, as batch_tile_kernel: the tile parameters are loaded on
+// demand instead of occupying the kernel-argument registers.
+template <bool EPS3, int CPML, int W>
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 ? 6 : 1, 8))) void batch_tile2_kernel(
+    const March<float>* __restrict__ table, const int* __restrict__ prefix, int grids) {
+    const int b = blockIdx.x;
+    int lo = 0, hi = grids;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    const March<float>& a = table[lo];
+    const int count = a.blocks[CPML == 0 ? 0 : CPML == 4 ? 1 : 2];
+    const int w = b - prefix[lo];
+    if (w >= count) return;
+    tile2_body<float, EPS3, CPML, W>(a, w, count);
+}
+
 // Tiles [lo, hi) along one axis whose cells of both stages avoid every CPML row of that axis: a tile
 // of `size` cells starting at s computes E on [s-1, s+size+2) and H on [s-1, s+size+1).
 inline void free_range2(int tiles, int size, int n, const Face* ef, const Face* hf, int* lo, int* hi, int shift = 0) {
@@ -1283,6 +1302,44 @@ int launch_tile2(const March<float>& a, bool eps3, hipStream_t stream) {
         if (v == 0) return eps3 ? go(tile2_kernel<true, 0, W>, W, "march2 interior") : go(tile2_kernel<false, 0, W>, W, "march2 interior");
         if (v == 1) return eps3 ? go(tile2_kernel<true, 4, 12>, 12, "march2 z faces") : go(tile2_kernel<false, 4, 12>, 12, "march2 z faces");
         return eps3 ? go(tile2_kernel<true, 7, 12>, 12, "march2 boundary") : go(tile2_kernel<false, 7, 12>, 12, "march2 boundary");
+    });
+}
+
+int pack_marches2(const tfdtd_march* d, int count, void* host_table, size_t bytes, int32_t* prefix) {
+    if (!host_table || bytes < sizeof(March<float>) * (size_t)count) return tfdtd_fail(TFDTD_EINVAL, "batch march2: table buffer too small");
+    March<float>* table = static_cast<March<float>*>(host_table);
+    long long total[3] = {0, 0, 0};
+    for (int k = 0; k < count; ++k) {
+        if (d[k].dtype != TFDTD_F32) return tfdtd_fail(TFDTD_EINVAL, "march2: float32 only");
+        if (d[k].tile_y != d[0].tile_y || d[k].eps_stride != d[0].eps_stride)
+            return tfdtd_fail(TFDTD_EINVAL, "batch march2: descriptor %d differs in tile_y or eps_stride", k);
+        const int status = layout_march2(d + k, table + k);
+        if (status) return status;
+        for (int v = 0; v < 3; ++v) {
+            prefix[v * (count + 1) + k] = (int32_t)total[v];
+            total[v] += (table[k].blocks[v] + 7) / 8 * 8;
+            if (total[v] > 0x7fffffffLL) return tfdtd_fail(TFDTD_EINVAL, "batch march2: too many workgroups for one launch");
+        }
+    }
+    for (int v = 0; v < 3; ++v) prefix[v * (count + 1) + count] = (int32_t)total[v];
+    return TFDTD_OK;
+}
+
+template <int W>
+int launch_batch_tiles2(const void* table, const int32_t* prefix, int count, bool eps3, const int32_t* totals, hipStream_t stream) {
+    const March<float>* t = static_cast<const March<float>*>(table);
+    const int blocks[3] = {totals[0], totals[1], totals[2]};
+    return launch_variants(blocks, stream, [&](int v, hipStream_t s) {
+        auto go = [&](auto kernel, int waves, const char* what) {
+            hipLaunchKernelGGL(kernel, dim3((unsigned)blocks[v]), dim3(64 * waves), 0, s, t, prefix + v * (count + 1), count);
+            return tfdtd_check_launch(what);
+        };
+        if (v == 0) return eps3 ? go(batch_tile2_kernel<true, 0, W>, W, "batch march2 interior")
+                                : go(batch_tile2_kernel<false, 0, W>, W, "batch march2 interior");
+        if (v == 1) return eps3 ? go(batch_tile2_kernel<true, 4, 12>, 12, "batch march2 z faces")
+                                : go(batch_tile2_kernel<false, 4, 12>, 12, "batch march2 z faces");
+        return eps3 ? go(batch_tile2_kernel<true, 7, 12>, 12, "batch march2 boundary")
+                    : go(batch_tile2_kernel<false, 7, 12>, 12, "batch march2 boundary");
     });
 }
 
@@ -1366,6 +1423,26 @@ extern "C" int tfdtd_yee_step_march2(const tfdtd_march* d, void* stream) {
     if (!d) return tfdtd_fail(TFDTD_EINVAL, "null march descriptor");
     if (d->dtype != TFDTD_F32) return tfdtd_fail(TFDTD_EINVAL, "march2: float32 only");
     return dispatch_tile2(d, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int tfdtd_batch_march2_pack(const tfdtd_march* steps, int32_t count, void* host_table, size_t bytes,
+                                       int32_t* block_prefix) {
+    if (!steps || count < 1 || !block_prefix) return tfdtd_fail(TFDTD_EINVAL, "batch march2: at least one descriptor and a prefix array");
+    return pack_marches2(steps, count, host_table, bytes, block_prefix);
+}
+
+extern "C" int tfdtd_batch_march2(const void* device_table, const int32_t* device_prefix, int32_t count, int32_t tile_y,
+                                  int32_t eps_stride, const int32_t* totals, void* stream) {
+    if (!device_table || !device_prefix || count < 1 || !totals) return tfdtd_fail(TFDTD_EINVAL, "batch march2: bad launch arguments");
+    if (eps_stride != 1 && eps_stride != 3) return tfdtd_fail(TFDTD_EINVAL, "march: eps_stride must be 1 or 3");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const bool eps3 = eps_stride == 3;
+    switch (tile_y) {
+        case 4: return launch_batch_tiles2<4>(device_table, device_prefix, count, eps3, totals, s);
+        case 6: return launch_batch_tiles2<6>(device_table, device_prefix, count, eps3, totals, s);
+        case 0: case 12: return launch_batch_tiles2<12>(device_table, device_prefix, count, eps3, totals, s);
+        default: return tfdtd_fail(TFDTD_EINVAL, "march2: tile_y (wavefronts of an interior workgroup) must be 4, 6, 12 or 0");
+    }
 }
 
 extern "C" size_t tfdtd_batch_march_bytes(int32_t count, int32_t dtype) {

@@ -249,9 +249,15 @@ class Stepper:
                                march_descriptor(g, self.E_b, self.H_b, g.E, g.H, self.psi_e_b, psi_a, self.sources,
                                                 self.counter, self.waves2, self.chunk_x2, self.materials,
                                                 psi_h=(self.psi_h_b, psi_h))]
+                self.march2_tables = [self._march2_table(d) for d in self.march2]
             self.plane_table_b = self._planes(self.E_b, self.H_b)
             self.trace_table_b = self._trace_table(self.E_b, self.H_b)
             self.observe_b = self._observe(self.trace_table_b)
+
+    def _march2_table(self, d):
+        table, prefix, host = pack_table(lambda n: self.lib.tfdtd_batch_march_bytes(n, N.F32),
+                                         self.lib.tfdtd_batch_march2_pack, [d], 3, self.grid.device)
+        return table, prefix, (ctypes.c_int32*3)(*host[:, 1].tolist())
 
     def states(self):
         extra = [self.E_b, self.H_b, self.psi_e_b] if self.one_pass else []
@@ -279,8 +285,9 @@ class Stepper:
             fresh, _, _ = pack_table(lambda n: self.lib.tfdtd_batch_march_bytes(n, dtype), self.lib.tfdtd_batch_march_pack,
                                      [d], 3, self.grid.device)
             table.copy_(fresh)
-        for d in getattr(self, 'march2', ()) if self.two_step else ():
+        for d, (table, _, _) in zip(self.march2, self.march2_tables) if self.two_step else ():
             m.describe(d)
+            table.copy_(self._march2_table(d)[0])
         return True
 
     def march_step(self, k, stream):
@@ -295,7 +302,10 @@ class Stepper:
         """Two steps in one pass from buffer set k (0: grid buffers -> second set, 1: back); the
         counter is not advanced."""
         self.refresh_materials()
-        N.check(self.lib.tfdtd_yee_step_march2(ctypes.byref(self.march2[k]), stream), 'tfdtd_yee_step_march2')
+        table, prefix, totals = self.march2_tables[k]
+        d = self.march2[k]
+        N.check(self.lib.tfdtd_batch_march2(table.data_ptr(), prefix.data_ptr(), 1, d.tile_y, d.eps_stride, totals, stream),
+                'tfdtd_batch_march2')
 
     def quad(self):
         """Four steps with the two-step kernel: grid buffers -> second set -> grid buffers."""
