@@ -29,6 +29,7 @@
 
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 
 #include "../../include/tfdtd.h"
 #include "common.h"
@@ -820,137 +821,146 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
         for (int t = 0; t < 6; ++t) pe0[k][t] = pe1[k][t] = ph0[k][t] = ph1[k][t] = real(0);
         if (xs > 0) Hn_c[k] = load3(h_in, 3u * ((unsigned)(xs - 1) * plane + (unsigned)min(max(yof(k), 0), ny - 1) * nz + zc));
     }
-    // E(n) and 1/eps (or its material index) of a plane's cells.  The lean variants load them one plane
-    // ahead, right after the barrier, so that the loads overlap a plane of work instead of the barrier
-    // that follows them; the boundary tiles (register-bound) load them just before that barrier.
-    constexpr bool PREFETCH = CPML != 7;
-    V3<real> eo_n[RW], inv3_n[RW];
-    real inv1_n[RW];
-    unsigned idx_n[RW];
-    auto load_plane = [&](int x) {
-        const unsigned base = (unsigned)x * plane;
-#pragma unroll
-        for (int k = 0; k < RW; ++k) {
-            const unsigned cell = base + (unsigned)min(max(yof(k), 0), ny - 1) * nz + zc;
-            eo_n[k] = load3(e_in, 3u * cell);
-            if (EPS3) inv3_n[k] = load3(inv_eps, 3u * cell);
-            else if (material) idx_n[k] = global(material)[cell];
-            else inv1_n[k] = global(inv_eps)[cell];
-        }
-    };
-    if (PREFETCH && xs < nx) load_plane(xs);
-    auto march_plane = [&](const int x, real* const sHx, real* const sHn) {
-        const bool pE1 = x < nx;
-        const bool pH1 = x >= xa && x >= 1 && x - 1 < nx;
-        const bool pE2 = x - 1 >= xa && x - 1 <= xb && x - 1 < nx;
-        const bool pH2 = x - 2 >= xa && x - 2 < xb;
-        if (!PREFETCH && pE1) load_plane(x);
-        __syncthreads();   // H(n) of plane x staged; E1(x-1) and E2(x-2) written to LDS
-        if (x + 1 <= xe && x + 1 < nx) stage(x + 1, sHn);
-        V3<real> eo[RW], inv[RW];
-        if (pE1) {
-#pragma unroll
+    // The march is compiled twice for the interior tiles, with and without the material index, so that
+    // the prefetched index and the prefetched 1/eps never meet in one register: merging them makes the
+    // compiler wait for the loads (and for the staging issued before them) as soon as they are issued.
+    auto run = [&](auto indexed) {
+        constexpr bool MAT = decltype(indexed)::value;
+        const unsigned char* __restrict__ mat = MAT ? material : nullptr;
+        // E(n) and 1/eps (or its material index) of a plane's cells.  The lean variants load them one plane
+        // ahead, right after the barrier, so that the loads overlap a plane of work instead of the barrier
+        // that follows them; the boundary tiles (register-bound) load them just before that barrier.
+        constexpr bool PREFETCH = CPML != 7;
+        V3<real> eo_n[RW], inv3_n[RW];
+        real inv1_n[RW];
+        unsigned idx_n[RW];
+        auto load_plane = [&](int x) {
+            const unsigned base = (unsigned)x * plane;
+    #pragma unroll
             for (int k = 0; k < RW; ++k) {
-                eo[k] = eo_n[k];
-                if (EPS3) {
-                    inv[k] = inv3_n[k];
-                } else {
-                    const real v = material ? sMat[idx_n[k]] : inv1_n[k];
-                    inv[k] = V3<real>{v, v, v};
+                const unsigned cell = base + (unsigned)min(max(yof(k), 0), ny - 1) * nz + zc;
+                eo_n[k] = load3(e_in, 3u * cell);
+                if (EPS3) inv3_n[k] = load3(inv_eps, 3u * cell);
+                else if (mat) idx_n[k] = global(mat)[cell];
+                else inv1_n[k] = global(inv_eps)[cell];
+            }
+        };
+        if (PREFETCH && xs < nx) load_plane(xs);
+        auto march_plane = [&](const int x, real* const sHx, real* const sHn) {
+            const bool pE1 = x < nx;
+            const bool pH1 = x >= xa && x >= 1 && x - 1 < nx;
+            const bool pE2 = x - 1 >= xa && x - 1 <= xb && x - 1 < nx;
+            const bool pH2 = x - 2 >= xa && x - 2 < xb;
+            if (!PREFETCH && pE1) load_plane(x);
+            __syncthreads();   // H(n) of plane x staged; E1(x-1) and E2(x-2) written to LDS
+            if (x + 1 <= xe && x + 1 < nx) stage(x + 1, sHn);
+            V3<real> eo[RW], inv[RW];
+            if (pE1) {
+    #pragma unroll
+                for (int k = 0; k < RW; ++k) {
+                    eo[k] = eo_n[k];
+                    if (EPS3) {
+                        inv[k] = inv3_n[k];
+                    } else {
+                        const real v = mat ? sMat[idx_n[k]] : inv1_n[k];
+                        inv[k] = V3<real>{v, v, v};
+                    }
                 }
             }
-        }
-        if (PREFETCH && x + 1 <= xe && x + 1 < nx) load_plane(x + 1);
-        V3<real> e1[RW], hn[RW], h1[RW], e2[RW];
-        // Stage 1: E1(x).
-        const Hit ex1 = find_hit(a.ef, 0, x);
-#pragma unroll
-        for (int k = 0; k < RW; ++k) {
-            const int r = rowof(k), y = yof(k);
-            const real* p = sHx + 3 * ((r + 1) * SC + lane + 1);
-            hn[k] = lds3(p);
-            e1[k] = zero3<real>();
-            if (pE1 && zin && y >= 0 && y < ny) {
-                const real* pym = p - 3 * SC;
-                const real* pzm = p - 3;
-                const Hit ey = find_hit(a.ef, 1, y);
-                e1[k] = e_cell2<real, CPML, 1>(a, x, y, z, false, ex1, ey, ez_hit, ez_coef, hn[k], Hn_c[k], lds3(pym), pzm[0],
-                                              pzm[1], eo[k], inv[k], pe1[k]);
-                add_sources(a.se, a.nsrc_e, a.ebox_lo, a.ebox_hi, n, a.source_samples, x, y, z, e1[k].x, e1[k].y, e1[k].z);
+            if (PREFETCH && x + 1 <= xe && x + 1 < nx) load_plane(x + 1);
+            V3<real> e1[RW], hn[RW], h1[RW], e2[RW];
+            // Stage 1: E1(x).
+            const Hit ex1 = find_hit(a.ef, 0, x);
+    #pragma unroll
+            for (int k = 0; k < RW; ++k) {
+                const int r = rowof(k), y = yof(k);
+                const real* p = sHx + 3 * ((r + 1) * SC + lane + 1);
+                hn[k] = lds3(p);
+                e1[k] = zero3<real>();
+                if (pE1 && zin && y >= 0 && y < ny) {
+                    const real* pym = p - 3 * SC;
+                    const real* pzm = p - 3;
+                    const Hit ey = find_hit(a.ef, 1, y);
+                    e1[k] = e_cell2<real, CPML, 1>(a, x, y, z, false, ex1, ey, ez_hit, ez_coef, hn[k], Hn_c[k], lds3(pym), pzm[0],
+                                                  pzm[1], eo[k], inv[k], pe1[k]);
+                    add_sources(a.se, a.nsrc_e, a.ebox_lo, a.ebox_hi, n, a.source_samples, x, y, z, e1[k].x, e1[k].y, e1[k].z);
+                }
             }
-        }
-        // Stage 1b: H1(x-1) from E1(x-1) (LDS, with the row and column above) and E1(x).
-        const Hit hx1 = find_hit(a.hf, 0, x - 1);
-#pragma unroll
-        for (int k = 0; k < RW; ++k) {
-            const int r = rowof(k), y = yof(k);
-            h1[k] = zero3<real>();
-            if (pH1 && r < NR - 1 && lane < 63 && zin && y >= 0 && y < ny) {
-                const real* p = sE1 + 3 * (r * 64 + lane);
-                const real* pzn = p + 3;
-                const Hit hy = find_hit(a.hf, 1, y);
-                h1[k] = h_cell2<real, CPML, 1>(a, x - 1, y, z, false, hcoef, hx1, hy, hz_hit, hz_coef, E1_c[k], pE1 ? e1[k] : E1_c[k],
-                                              lds3(p + 3 * 64), pzn[0], pzn[1], Hn_c[k], ph1[k]);
-                add_sources(a.sh, a.nsrc_h, a.hbox_lo, a.hbox_hi, n, a.source_samples, x - 1, y, z, h1[k].x, h1[k].y, h1[k].z);
+            // Stage 1b: H1(x-1) from E1(x-1) (LDS, with the row and column above) and E1(x).
+            const Hit hx1 = find_hit(a.hf, 0, x - 1);
+    #pragma unroll
+            for (int k = 0; k < RW; ++k) {
+                const int r = rowof(k), y = yof(k);
+                h1[k] = zero3<real>();
+                if (pH1 && r < NR - 1 && lane < 63 && zin && y >= 0 && y < ny) {
+                    const real* p = sE1 + 3 * (r * 64 + lane);
+                    const real* pzn = p + 3;
+                    const Hit hy = find_hit(a.hf, 1, y);
+                    h1[k] = h_cell2<real, CPML, 1>(a, x - 1, y, z, false, hcoef, hx1, hy, hz_hit, hz_coef, E1_c[k], pE1 ? e1[k] : E1_c[k],
+                                                  lds3(p + 3 * 64), pzn[0], pzn[1], Hn_c[k], ph1[k]);
+                    add_sources(a.sh, a.nsrc_h, a.hbox_lo, a.hbox_hi, n, a.source_samples, x - 1, y, z, h1[k].x, h1[k].y, h1[k].z);
+                }
+                put3(sH1 + 3 * (r * 64 + lane), h1[k]);
             }
-            put3(sH1 + 3 * (r * 64 + lane), h1[k]);
-        }
-        lds_barrier();   // H1(x-1) of every row in LDS
-        // Stage 2: E2(x-1); the owned cells (rows y0 .. y0+8, lanes 1..61, planes < xb) are stored.
-        const Hit ex2 = find_hit(a.ef, 0, x - 1);
-#pragma unroll
-        for (int k = 0; k < RW; ++k) {
-            const int r = rowof(k), y = yof(k);
-            e2[k] = zero3<real>();
-            if (pE2 && r >= 1 && r < NR - 1 && lane >= 1 && lane < 63 && zin && y >= 0 && y < ny) {
-                const real* p = sH1 + 3 * (r * 64 + lane);
-                const real* pzm = p - 3;
-                const bool owned = r < NR - 2 && lane < 62 && x - 1 < xb;
-                const Hit ey = find_hit(a.ef, 1, y);
-                e2[k] = e_cell2<real, CPML, 2>(a, x - 1, y, z, owned, ex2, ey, ez_hit, ez_coef, h1[k], H1_c[k], lds3(p - 3 * 64),
-                                              pzm[0], pzm[1], E1_c[k], inv_c[k], pe0[k]);
-                add_sources(a.se, a.nsrc_e, a.ebox_lo, a.ebox_hi, n + 1, a.source_samples, x - 1, y, z, e2[k].x, e2[k].y, e2[k].z);
-                if (owned) store3(e_out, 3u * ((unsigned)(x - 1) * plane + (unsigned)y * nz + (unsigned)z), e2[k]);
+            lds_barrier();   // H1(x-1) of every row in LDS
+            // Stage 2: E2(x-1); the owned cells (rows y0 .. y0+8, lanes 1..61, planes < xb) are stored.
+            const Hit ex2 = find_hit(a.ef, 0, x - 1);
+    #pragma unroll
+            for (int k = 0; k < RW; ++k) {
+                const int r = rowof(k), y = yof(k);
+                e2[k] = zero3<real>();
+                if (pE2 && r >= 1 && r < NR - 1 && lane >= 1 && lane < 63 && zin && y >= 0 && y < ny) {
+                    const real* p = sH1 + 3 * (r * 64 + lane);
+                    const real* pzm = p - 3;
+                    const bool owned = r < NR - 2 && lane < 62 && x - 1 < xb;
+                    const Hit ey = find_hit(a.ef, 1, y);
+                    e2[k] = e_cell2<real, CPML, 2>(a, x - 1, y, z, owned, ex2, ey, ez_hit, ez_coef, h1[k], H1_c[k], lds3(p - 3 * 64),
+                                                  pzm[0], pzm[1], E1_c[k], inv_c[k], pe0[k]);
+                    add_sources(a.se, a.nsrc_e, a.ebox_lo, a.ebox_hi, n + 1, a.source_samples, x - 1, y, z, e2[k].x, e2[k].y, e2[k].z);
+                    if (owned) store3(e_out, 3u * ((unsigned)(x - 1) * plane + (unsigned)y * nz + (unsigned)z), e2[k]);
+                }
             }
-        }
-        // Stage 2b: H2(x-2) from E2(x-2) (LDS, with the row and column above) and E2(x-1).
-        const Hit hx2 = find_hit(a.hf, 0, x - 2);
-#pragma unroll
-        for (int k = 0; k < RW; ++k) {
-            const int r = rowof(k), y = yof(k);
-            if (pH2 && r >= 1 && r < NR - 2 && lane >= 1 && lane < 62 && zin && y >= 0 && y < ny) {
-                const real* p = sE2 + 3 * (r * 64 + lane);
-                const real* pzn = p + 3;
-                const Hit hy = find_hit(a.hf, 1, y);
-                V3<real> h2 = h_cell2<real, CPML, 2>(a, x - 2, y, z, true, hcoef, hx2, hy, hz_hit, hz_coef, E2_c[k],
-                                                     pE2 ? e2[k] : E2_c[k], lds3(p + 3 * 64), pzn[0], pzn[1], H1_c[k], ph0[k]);
-                add_sources(a.sh, a.nsrc_h, a.hbox_lo, a.hbox_hi, n + 1, a.source_samples, x - 2, y, z, h2.x, h2.y, h2.z);
-                store3(h_out, 3u * ((unsigned)(x - 2) * plane + (unsigned)y * nz + (unsigned)z), h2);
+            // Stage 2b: H2(x-2) from E2(x-2) (LDS, with the row and column above) and E2(x-1).
+            const Hit hx2 = find_hit(a.hf, 0, x - 2);
+    #pragma unroll
+            for (int k = 0; k < RW; ++k) {
+                const int r = rowof(k), y = yof(k);
+                if (pH2 && r >= 1 && r < NR - 2 && lane >= 1 && lane < 62 && zin && y >= 0 && y < ny) {
+                    const real* p = sE2 + 3 * (r * 64 + lane);
+                    const real* pzn = p + 3;
+                    const Hit hy = find_hit(a.hf, 1, y);
+                    V3<real> h2 = h_cell2<real, CPML, 2>(a, x - 2, y, z, true, hcoef, hx2, hy, hz_hit, hz_coef, E2_c[k],
+                                                         pE2 ? e2[k] : E2_c[k], lds3(p + 3 * 64), pzn[0], pzn[1], H1_c[k], ph0[k]);
+                    add_sources(a.sh, a.nsrc_h, a.hbox_lo, a.hbox_hi, n + 1, a.source_samples, x - 2, y, z, h2.x, h2.y, h2.z);
+                    store3(h_out, 3u * ((unsigned)(x - 2) * plane + (unsigned)y * nz + (unsigned)z), h2);
+                }
             }
-        }
-        lds_barrier();   // every read of sE1, sH1 and sE2 for this plane is done
-#pragma unroll
-        for (int k = 0; k < RW; ++k) {
-            const int r = rowof(k);
-            put3(sE1 + 3 * (r * 64 + lane), e1[k]);
-            put3(sE2 + 3 * (r * 64 + lane), e2[k]);
-            Hn_c[k] = hn[k];
-            H1_c[k] = h1[k];
-            E1_c[k] = e1[k];
-            E2_c[k] = e2[k];
-            inv_c[k] = pE1 ? inv[k] : zero3<real>();
-#pragma unroll
-            for (int t = 0; t < 6; ++t) {
-                pe0[k][t] = pe1[k][t];
-                ph0[k][t] = ph1[k][t];
+            lds_barrier();   // every read of sE1, sH1 and sE2 for this plane is done
+    #pragma unroll
+            for (int k = 0; k < RW; ++k) {
+                const int r = rowof(k);
+                put3(sE1 + 3 * (r * 64 + lane), e1[k]);
+                put3(sE2 + 3 * (r * 64 + lane), e2[k]);
+                Hn_c[k] = hn[k];
+                H1_c[k] = h1[k];
+                E1_c[k] = e1[k];
+                E2_c[k] = e2[k];
+                inv_c[k] = pE1 ? inv[k] : zero3<real>();
+    #pragma unroll
+                for (int t = 0; t < 6; ++t) {
+                    pe0[k][t] = pe1[k][t];
+                    ph0[k][t] = ph1[k][t];
+                }
             }
+        };
+        if (xs < nx) stage(xs, sA);
+        for (int x = xs; x <= xe; x += 2) {
+            march_plane(x, sA, sB);
+            if (x + 1 <= xe) march_plane(x + 1, sB, sA);
         }
     };
-    if (xs < nx) stage(xs, sA);
-    for (int x = xs; x <= xe; x += 2) {
-        march_plane(x, sA, sB);
-        if (x + 1 <= xe) march_plane(x + 1, sB, sA);
-    }
+    if (INDEXED && material) run(std::true_type{});
+    else run(std::false_type{});
 }
 
 #ifndef TFDTD_SERIAL_VARIANTS
