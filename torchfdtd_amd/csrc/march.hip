@@ -82,6 +82,11 @@ struct March {
     int zshift;                 // z tile tz starts at tz*63 - zshift
     int nchunks;
     int blocks[3];              // workgroups of the interior, z-face and boundary variants
+    // Two-step layout only: x is cut at x0 and x1 (the planes whose widened chunks avoid the low and
+    // high x slabs) into three parts of xparts[0..2] chunks each, every part split evenly.
+    int x0, x1;
+    int xparts[3];
+    int yshift;                 // y tile ty starts at ty*9 - yshift
 };
 
 // psi <- psi*b + c*d (stored when `write`), d <- d*inv_k + psi, for slab row q of term k.
@@ -721,6 +726,19 @@ __device__ __forceinline__ V3<real> h_cell2(const March<real>& a, int x, int y, 
     return V3<real>{h.x - coefficient * cx, h.y - coefficient * cy, h.z - coefficient * cz};
 }
 
+// Planes [xa, xb) of chunk c in the two-step layout: part 0 = [0, x0), part 1 = [x0, x1), part 2 =
+// [x1, nx), each split into its xparts[] chunks of near-equal length.
+template <typename real>
+__host__ __device__ __forceinline__ void chunk_range2(const March<real>& a, int c, int& xa, int& xb) {
+    const int n0 = a.xparts[0], n1 = a.xparts[1];
+    int lo = 0, hi = a.x0, i = c, n = n0;
+    if (c >= n0 + n1) { lo = a.x1; hi = a.nx; i = c - n0 - n1; n = a.xparts[2]; }
+    else if (c >= n0) { lo = a.x0; hi = a.x1; i = c - n0; n = n1; }
+    const long long len = hi - lo;
+    xa = lo + (int)(i * len / n);
+    xb = lo + (int)((i + 1) * len / n);
+}
+
 constexpr int kTZ2 = 61;   // owned z columns of a two-step tile
 constexpr int kTY2 = 9;    // owned y rows of a two-step tile
 
@@ -753,8 +771,9 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
     int ty, tz, chunk;
     tile_coords<real, CPML>(a, w, G, ty, tz, chunk);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int y0 = ty * kTY2, z0 = tz * kTZ2 - a.zshift;
-    const int xa = chunk * a.chunk, xb = min(nx, xa + a.chunk);
+    const int y0 = ty * kTY2 - a.yshift, z0 = tz * kTZ2 - a.zshift;
+    int xa, xb;
+    chunk_range2(a, chunk, xa, xb);
     const long long n = *global(a.counter);
     const real* __restrict__ e_in = a.e_in;
     const real* __restrict__ h_in = a.h_in;
@@ -1272,7 +1291,17 @@ int layout_march2(const tfdtd_march* d, March<float>* a) {
     a->hf.psi_out = static_cast<float*>(h_slabs ? d->h.psi_out : const_cast<void*>(d->h.psi_in));
     const int chunk = d->chunk_x > 0 ? d->chunk_x : d->nx;
     a->chunk = chunk;
-    a->ytiles = (d->ny + kTY2 - 1) / kTY2;
+    // Shift the y tiles to the cheapest of the nine phases (slab-free tiles weigh 10, the others 17).
+    auto row_cost = [&](int shift) {
+        const int tiles = (d->ny + shift + kTY2 - 1) / kTY2;
+        int lo, hi;
+        free_range2(tiles, kTY2, d->ny, a->ef.face[1], a->hf.face[1], &lo, &hi, shift);
+        return 10 * (hi - lo) + 17 * (tiles - (hi - lo));
+    };
+    a->yshift = 0;
+    for (int shift = 1; shift < kTY2; ++shift)
+        if (row_cost(shift) < row_cost(a->yshift)) a->yshift = shift;
+    a->ytiles = (d->ny + a->yshift + kTY2 - 1) / kTY2;
     // Shift the z tiles so one tile's widened columns start just past the low z slabs, when that lowers
     // the cost of a z column (slab-free tiles weigh 10, tiles with z slab columns 17).
     const int zend = std::max(a->ef.face[2][0].hi, a->hf.face[2][0].hi);
@@ -1285,11 +1314,25 @@ int layout_march2(const tfdtd_march* d, March<float>* a) {
     const int aligned = zend > 0 ? (kTZ2 - (zend + 1) % kTZ2) % kTZ2 : 0;
     a->zshift = column_cost(aligned) < column_cost(0) ? aligned : 0;
     a->ztiles = (d->nz + a->zshift + kTZ2 - 1) / kTZ2;
-    const int chunks = (d->nx + chunk - 1) / chunk;
+    // x: the slab-free middle [x0, x1) -- a chunk [xa, xb) computes E on [xa-1, xb+2) and H on
+    // [xa-1, xb+1) -- and the two ends, each cut into chunks of at most chunk_x planes, so that the
+    // boundary chunks are no longer than the slabs they cover.
+    const Face& el = a->ef.face[0][0]; const Face& hl = a->hf.face[0][0];
+    const Face& eh = a->ef.face[0][1]; const Face& hh = a->hf.face[0][1];
+    const int low = std::max(el.hi > el.lo ? el.hi : 0, hl.hi > hl.lo ? hl.hi : 0);
+    int x0 = low > 0 ? low + 1 : 0;
+    int x1 = std::min(eh.hi > eh.lo ? eh.lo - 2 : d->nx, hh.hi > hh.lo ? hh.lo - 1 : d->nx);
+    if (x1 <= x0) x0 = x1 = d->nx;     // no slab-free middle: every chunk is a boundary chunk
+    a->x0 = x0;
+    a->x1 = x1;
+    const int part[3] = {x0, x1 - x0, d->nx - x1};
+    for (int k = 0; k < 3; ++k) a->xparts[k] = (part[k] + chunk - 1) / chunk;
+    const int chunks = a->xparts[0] + a->xparts[1] + a->xparts[2];
     const long long blocks = (long long)chunks * a->ytiles * a->ztiles;
     if (blocks > 0x7fffffffLL) return tfdtd_fail(TFDTD_EINVAL, "march2: too many workgroups");
-    free_range2(chunks, chunk, d->nx, a->ef.face[0], a->hf.face[0], &a->box_lo[0], &a->box_hi[0]);
-    free_range2(a->ytiles, kTY2, d->ny, a->ef.face[1], a->hf.face[1], &a->box_lo[1], &a->box_hi[1]);
+    a->box_lo[0] = a->xparts[0];
+    a->box_hi[0] = a->xparts[0] + a->xparts[1];
+    free_range2(a->ytiles, kTY2, d->ny, a->ef.face[1], a->hf.face[1], &a->box_lo[1], &a->box_hi[1], a->yshift);
     a->nchunks = chunks;
     free_range2(a->ztiles, kTZ2, d->nz, a->ef.face[2], a->hf.face[2], &a->zbox_lo, &a->zbox_hi, a->zshift);
     const long long xy = (long long)(a->box_hi[0] - a->box_lo[0]) * (a->box_hi[1] - a->box_lo[1]);
