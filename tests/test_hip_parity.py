@@ -118,13 +118,15 @@ def test_one_pass_step_matches_oracle_bitwise(gpu, precision, layout, tile_y, ch
         np.testing.assert_array_equal(got.cpu().numpy(), expected)
 
 
-@pytest.mark.parametrize('layout,chunk_x2,nz,components', [
-    ('pml3d', 7, 131, 3), ('pml3d', 20, 200, 'index'), ('pec', 5, 131, 1), ('pml3d', 0, 131, 1), ('pec', 20, 200, 3),
-    ('pml3d', 3, 64, 'index')])
-def test_two_step_pass_matches_oracle_bitwise(gpu, layout, chunk_x2, nz, components):
+@pytest.mark.parametrize('layout,chunk_x2,nz,components,waves2', [
+    ('pml3d', 7, 131, 3, 12), ('pml3d', 20, 200, 'index', 12), ('pec', 5, 131, 1, 12), ('pml3d', 0, 131, 1, 12),
+    ('pec', 20, 200, 3, 12), ('pml3d', 3, 64, 'index', 12), ('pml3d', 7, 131, 3, 16), ('pml3d', 20, 200, 'index', 16),
+    ('pec', 5, 131, 1, 16), ('pml3d', 4, 64, 'index', 4)])
+def test_two_step_pass_matches_oracle_bitwise(gpu, layout, chunk_x2, nz, components, waves2):
     """The two-step pass (step n+1 recomputed on widened tiles and kept on chip, E, H and both CPML
     memories double-buffered) equals the reference update bit for bit, starting from random fields
-    and random CPML memory, over five passes of four steps, one pair and one single step."""
+    and random CPML memory, over five passes of four steps, one pair and one single step; interior
+    tiles of 12 wavefronts (9 rows), 16 (13 rows, the last one cut) and 4 (3 rows each)."""
     from torchfdtd_amd import Project
     from torchfdtd_amd.solver import Stepper
     region = region_for(layout, 'float32')
@@ -135,7 +137,7 @@ def test_two_step_pass_matches_oracle_bitwise(gpu, layout, chunk_x2, nz, compone
     if components == 1:
         region.material_sampling = 'cell'
     project = Project(region=region)
-    run = Stepper(project, steps=40, one_pass=True, two_step=True, chunk_x2=chunk_x2)
+    run = Stepper(project, steps=40, one_pass=True, two_step=True, chunk_x2=chunk_x2, waves2=waves2)
     g = run.grid
     scene = Scene(project.model_dump())
     real = scene.real

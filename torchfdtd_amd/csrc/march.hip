@@ -87,6 +87,7 @@ struct March {
     int x0, x1;
     int xparts[3];
     int yshift;                 // y tile ty starts at ty*9 - yshift
+    int iy0, iy1, iytiles;      // tall interior tiles: rows [iy0, iy1) in iytiles tiles of 13 rows
 };
 
 // psi <- psi*b + c*d (stored when `write`), d <- d*inv_k + psi, for slab row q of term k.
@@ -754,10 +755,11 @@ constexpr int kTY2 = 9;    // owned y rows of a two-step tile
 // plane ahead; E1(x-1), H1(x-1) and E2(x-2) are exchanged between wavefronts through LDS, and every
 // value a cell needs from its own previous plane (H(n), E1, H1, E2, 1/eps, CPML memory) is carried in
 // registers.  Cells outside the grid are never computed or stored; their addresses are clamped.
-template <typename real, bool EPS3, int CPML, int W>
+template <typename real, bool EPS3, int CPML, int W, int NR = 12>
 __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, const int G) {
-    constexpr int NR = 12, RW = NR / W;   // widened rows, rows per wavefront
-    static_assert(RW * W == NR, "the wavefronts split the 12 widened rows evenly");
+    constexpr int RW = NR / W;   // NR widened rows (owned NR - 3), RW per wavefront
+    static_assert(RW * W == NR, "the wavefronts split the widened rows evenly");
+    static_assert(NR == 12 || CPML == 0, "tall tiles are interior tiles");
     constexpr int SR = NR + 1, SC = kTZ2 + 4, SH = SR * SC * 3;   // staged H(n): rows y0-2 .. y0+10, columns z0-2 .. z0+62
     constexpr int PL = NR * 64 * 3 + 64 * 3 + 3;                 // one widened plane, padded past its last row
     __shared__ real sA[SH];
@@ -768,10 +770,25 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
     constexpr bool INDEXED = !EPS3 && CPML == 0;
     __shared__ real sMat[INDEXED ? 256 : 1];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    int ty, tz, chunk;
-    tile_coords<real, CPML>(a, w, G, ty, tz, chunk);
+    int ty, tz, chunk, y0, ylim = a.ny;
+    if constexpr (NR != 12) {
+        // Tall interior tiles of NR - 3 rows cover the slab-free rows [iy0, iy1); the last one is cut at
+        // iy1 (its rows past iy1 + 1 are computed without CPML and feed no owned cell).
+        const int full = (G / 8) * 8;
+        int t = w < full ? (w % 8) * (G / 8) + (w / 8) : w;
+        const int my = a.iytiles, mz = a.zbox_hi - a.zbox_lo;
+        ty = t % my;
+        t /= my;
+        tz = a.zbox_lo + t % mz;
+        chunk = a.box_lo[0] + t / mz;
+        y0 = a.iy0 + ty * (NR - 3);
+        ylim = a.iy1;
+    } else {
+        tile_coords<real, CPML>(a, w, G, ty, tz, chunk);
+        y0 = ty * kTY2 - a.yshift;
+    }
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int y0 = ty * kTY2 - a.yshift, z0 = tz * kTZ2 - a.zshift;
+    const int z0 = tz * kTZ2 - a.zshift;
     int xa, xb;
     chunk_range2(a, chunk, xa, xb);
     const long long n = *global(a.counter);
@@ -931,7 +948,7 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                 if (pE2 && r >= 1 && r < NR - 1 && lane >= 1 && lane < 63 && zin && y >= 0 && y < ny) {
                     const real* p = sH1 + 3 * (r * 64 + lane);
                     const real* pzm = p - 3;
-                    const bool owned = r < NR - 2 && lane < 62 && x - 1 < xb;
+                    const bool owned = r < NR - 2 && lane < 62 && x - 1 < xb && y < ylim;
                     const Hit ey = find_hit(a.ef, 1, y);
                     e2[k] = e_cell2<real, CPML, 2>(a, x - 1, y, z, owned, ex2, ey, ez_hit, ez_coef, h1[k], H1_c[k], lds3(p - 3 * 64),
                                                   pzm[0], pzm[1], E1_c[k], inv_c[k], pe0[k]);
@@ -944,7 +961,7 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
     #pragma unroll
             for (int k = 0; k < RW; ++k) {
                 const int r = rowof(k), y = yof(k);
-                if (pH2 && r >= 1 && r < NR - 2 && lane >= 1 && lane < 62 && zin && y >= 0 && y < ny) {
+                if (pH2 && r >= 1 && r < NR - 2 && lane >= 1 && lane < 62 && zin && y >= 0 && y < ny && y < ylim) {
                     const real* p = sE2 + 3 * (r * 64 + lane);
                     const real* pzn = p + 3;
                     const Hit hy = find_hit(a.hf, 1, y);
@@ -1230,14 +1247,14 @@ int launch_tile(const tfdtd_march* d, hipStream_t stream) {
 // rows each, hence fewer registers: the boundary tiles carry the CPML memory of both stages of every
 // axis in registers.
 template <bool EPS3, int CPML, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 ? 6 : 1, 8))) void tile2_kernel(const March<float> a) {
-    tile2_body<float, EPS3, CPML, W>(a, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 16 ? 8 : W == 12 ? 6 : 1, 8))) void tile2_kernel(const March<float> a) {
+    tile2_body<float, EPS3, CPML, W, W == 16 ? 16 : 12>(a, blockIdx.x, gridDim.x);
 }
 
 // Many grids (or one) from a device table, as batch_tile_kernel: the tile parameters are loaded on
 // demand instead of occupying the kernel-argument registers.
 template <bool EPS3, int CPML, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 ? 6 : 1, 8))) void batch_tile2_kernel(
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 16 ? 8 : W == 12 ? 6 : 1, 8))) void batch_tile2_kernel(
     const March<float>* __restrict__ table, const int* __restrict__ prefix, int grids) {
     const int b = blockIdx.x;
     int lo = 0, hi = grids;
@@ -1250,7 +1267,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 12 
     const int count = a.blocks[CPML == 0 ? 0 : CPML == 4 ? 1 : 2];
     const int w = b - prefix[lo];
     if (w >= count) return;
-    tile2_body<float, EPS3, CPML, W>(a, w, count);
+    tile2_body<float, EPS3, CPML, W, W == 16 ? 16 : 12>(a, w, count);
 }
 
 // Tiles [lo, hi) along one axis whose cells of both stages avoid every CPML row of that axis: a tile
@@ -1341,6 +1358,12 @@ int layout_march2(const tfdtd_march* d, March<float>* a) {
     a->blocks[0] = (int)pure;
     a->blocks[1] = (int)(inner - pure);
     a->blocks[2] = (int)(blocks - inner);
+    // Tall interior tiles (tile_y 16: 16 wavefronts, 13 owned rows) retile the slab-free rows of the
+    // 9-row tiles; the z-face and boundary tiles keep the 9-row tiles.
+    a->iy0 = std::max(a->box_lo[1] * kTY2 - a->yshift, 0);
+    a->iy1 = std::min(a->box_hi[1] * kTY2 - a->yshift, d->ny);
+    a->iytiles = a->iy1 > a->iy0 ? (a->iy1 - a->iy0 + 12) / 13 : 0;
+    if (d->tile_y == 16) a->blocks[0] = (a->box_hi[0] - a->box_lo[0]) * a->iytiles * (a->zbox_hi - a->zbox_lo);
     return TFDTD_OK;
 }
 
@@ -1405,7 +1428,8 @@ int dispatch_tile2(const tfdtd_march* d, hipStream_t stream) {
         case 4: return launch_tile2<4>(a, eps3, stream);
         case 6: return launch_tile2<6>(a, eps3, stream);
         case 0: case 12: return launch_tile2<12>(a, eps3, stream);
-        default: return tfdtd_fail(TFDTD_EINVAL, "march2: tile_y (wavefronts of an interior workgroup) must be 4, 6, 12 or 0");
+        case 16: return launch_tile2<16>(a, eps3, stream);
+        default: return tfdtd_fail(TFDTD_EINVAL, "march2: tile_y (wavefronts of an interior workgroup) must be 4, 6, 12, 16 or 0");
     }
 }
 
@@ -1494,7 +1518,8 @@ extern "C" int tfdtd_batch_march2(const void* device_table, const int32_t* devic
         case 4: return launch_batch_tiles2<4>(device_table, device_prefix, count, eps3, totals, s);
         case 6: return launch_batch_tiles2<6>(device_table, device_prefix, count, eps3, totals, s);
         case 0: case 12: return launch_batch_tiles2<12>(device_table, device_prefix, count, eps3, totals, s);
-        default: return tfdtd_fail(TFDTD_EINVAL, "march2: tile_y (wavefronts of an interior workgroup) must be 4, 6, 12 or 0");
+        case 16: return launch_batch_tiles2<16>(device_table, device_prefix, count, eps3, totals, s);
+        default: return tfdtd_fail(TFDTD_EINVAL, "march2: tile_y (wavefronts of an interior workgroup) must be 4, 6, 12, 16 or 0");
     }
 }
 
