@@ -593,12 +593,25 @@ __device__ __forceinline__ real cpml_lane2(real d, const Family<real>& F, const 
     return d * zc.ik + m;
 }
 
+// Loop-invariant scalars of the two-step pass, copied out of the descriptor once: read through the
+// descriptor, the compiler reloads them after every store (the stores might alias the table).
+template <typename real>
+struct Consts2 {
+    int nx, ny, nz, pec_mask;
+    real courant, hcoef;
+    real pec[3];
+    int nse, nsh;
+    long long samples;
+    int ebl[3], ebh[3], hbl[3], hbh[3];
+};
+
 // e_cell with the CPML memory of term t in slot[t].
 template <typename real, int CPML, int STAGE>
-__device__ __forceinline__ V3<real> e_cell2(const March<real>& a, int x, int y, int z, bool write, const Hit& hx, const Hit& hy,
-                                            const Hit& hz, const ZCoef<real>& zc, V3<real> h, V3<real> hxm, V3<real> hym,
-                                            real hx_zm, real hy_zm, V3<real> e, V3<real> inv, real (&slot)[6]) {
-    const int ny = a.ny, nz = a.nz;
+__device__ __forceinline__ V3<real> e_cell2(const March<real>& a, const Consts2<real>& q, int x, int y, int z, bool write,
+                                            const Hit& hx, const Hit& hy, const Hit& hz, const ZCoef<real>& zc, V3<real> h,
+                                            V3<real> hxm, V3<real> hym, real hx_zm, real hy_zm, V3<real> e, V3<real> inv,
+                                            real (&slot)[6]) {
+    const int ny = q.ny, nz = q.nz;
     real cx = real(0), cy = real(0), cz = real(0);
     const unsigned px = ((unsigned)hx.q * ny + y) * nz + z;
     const unsigned py = ((unsigned)x * hy.w + hy.q) * nz + z;
@@ -651,15 +664,16 @@ __device__ __forceinline__ V3<real> e_cell2(const March<real>& a, int x, int y, 
         }
         cy = cy + d;
     }
-    return V3<real>{e.x + (a.courant * inv.x) * cx, e.y + (a.courant * inv.y) * cy, e.z + (a.courant * inv.z) * cz};
+    return V3<real>{e.x + (q.courant * inv.x) * cx, e.y + (q.courant * inv.y) * cy, e.z + (q.courant * inv.z) * cz};
 }
 
 // h_cell with the CPML memory of term t in slot[t].
 template <typename real, int CPML, int STAGE>
-__device__ __forceinline__ V3<real> h_cell2(const March<real>& a, int x, int y, int z, bool write, real coefficient, const Hit& hx,
-                                            const Hit& hy, const Hit& hz, const ZCoef<real>& zc, V3<real> e, V3<real> exn,
-                                            V3<real> eyn, real ex_zp, real ey_zp, V3<real> h, real (&slot)[6]) {
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
+__device__ __forceinline__ V3<real> h_cell2(const March<real>& a, const Consts2<real>& q, int x, int y, int z, bool write,
+                                            real coefficient, const Hit& hx, const Hit& hy, const Hit& hz, const ZCoef<real>& zc,
+                                            V3<real> e, V3<real> exn, V3<real> eyn, real ex_zp, real ey_zp, V3<real> h,
+                                            real (&slot)[6]) {
+    const int nx = q.nx, ny = q.ny, nz = q.nz;
     real cx = real(0), cy = real(0), cz = real(0);
     const unsigned px = ((unsigned)hx.q * ny + y) * nz + z;
     const unsigned py = ((unsigned)x * hy.w + hy.q) * nz + z;
@@ -669,8 +683,8 @@ __device__ __forceinline__ V3<real> h_cell2(const March<real>& a, int x, int y, 
         if (x < nx - 1) {
             d = exn.z - e.z;
             if ((CPML & 1) && hx.face >= 0) d = cpml2<real, STAGE>(d, a.hf, hx, 0, px, write, slot[0]);
-        } else if (a.pec_mask & 1) {
-            d = -a.pec_factor[0] * e.z;
+        } else if (q.pec_mask & 1) {
+            d = -q.pec[0] * e.z;
         }
         cy = cy - d;
     }
@@ -679,8 +693,8 @@ __device__ __forceinline__ V3<real> h_cell2(const March<real>& a, int x, int y, 
         if (x < nx - 1) {
             d = exn.y - e.y;
             if ((CPML & 1) && hx.face >= 0) d = cpml2<real, STAGE>(d, a.hf, hx, 1, px, write, slot[1]);
-        } else if (a.pec_mask & 1) {
-            d = -a.pec_factor[0] * e.y;
+        } else if (q.pec_mask & 1) {
+            d = -q.pec[0] * e.y;
         }
         cz = cz + d;
     }
@@ -689,8 +703,8 @@ __device__ __forceinline__ V3<real> h_cell2(const March<real>& a, int x, int y, 
         if (y < ny - 1) {
             d = eyn.z - e.z;
             if ((CPML & 2) && hy.face >= 0) d = cpml2<real, STAGE>(d, a.hf, hy, 0, py, write, slot[2]);
-        } else if (a.pec_mask & 2) {
-            d = -a.pec_factor[1] * e.z;
+        } else if (q.pec_mask & 2) {
+            d = -q.pec[1] * e.z;
         }
         cx = cx + d;
     }
@@ -699,8 +713,8 @@ __device__ __forceinline__ V3<real> h_cell2(const March<real>& a, int x, int y, 
         if (y < ny - 1) {
             d = eyn.x - e.x;
             if ((CPML & 2) && hy.face >= 0) d = cpml2<real, STAGE>(d, a.hf, hy, 1, py, write, slot[3]);
-        } else if (a.pec_mask & 2) {
-            d = -a.pec_factor[1] * e.x;
+        } else if (q.pec_mask & 2) {
+            d = -q.pec[1] * e.x;
         }
         cz = cz - d;
     }
@@ -709,8 +723,8 @@ __device__ __forceinline__ V3<real> h_cell2(const March<real>& a, int x, int y, 
         if (z < nz - 1) {
             d = ey_zp - e.y;
             if ((CPML & 4) && hz.face >= 0) d = cpml_lane2<real, STAGE>(d, a.hf, hz, zc, 0, pz, write, slot[4]);
-        } else if (a.pec_mask & 4) {
-            d = -a.pec_factor[2] * e.y;
+        } else if (q.pec_mask & 4) {
+            d = -q.pec[2] * e.y;
         }
         cx = cx - d;
     }
@@ -719,8 +733,8 @@ __device__ __forceinline__ V3<real> h_cell2(const March<real>& a, int x, int y, 
         if (z < nz - 1) {
             d = ex_zp - e.x;
             if ((CPML & 4) && hz.face >= 0) d = cpml_lane2<real, STAGE>(d, a.hf, hz, zc, 1, pz, write, slot[5]);
-        } else if (a.pec_mask & 4) {
-            d = -a.pec_factor[2] * e.x;
+        } else if (q.pec_mask & 4) {
+            d = -q.pec[2] * e.x;
         }
         cy = cy + d;
     }
@@ -805,6 +819,21 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
     }
     const real hcoef = a.courant * global(a.inv_mu)[0];
     const unsigned plane = (unsigned)ny * nz;
+    Consts2<real> q;
+    q.nx = nx; q.ny = ny; q.nz = nz; q.pec_mask = a.pec_mask;
+    q.courant = a.courant; q.hcoef = hcoef;
+    q.nse = a.nsrc_e; q.nsh = a.nsrc_h; q.samples = a.source_samples;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        q.pec[ax] = a.pec_factor[ax];
+        q.ebl[ax] = a.ebox_lo[ax]; q.ebh[ax] = a.ebox_hi[ax];
+        q.hbl[ax] = a.hbox_lo[ax]; q.hbh[ax] = a.hbox_hi[ax];
+    }
+    // Interior tiles compute every lane and discard the lanes outside the grid or the stage's columns
+    // afterwards (no CPML memory to address, every LDS read is in bounds): per-lane branches cost more
+    // scalar instructions than the discarded lanes cost vector ones.
+    constexpr bool ALL_LANES = CPML == 0;
+    auto sel3 = [](bool keep, V3<real> v) { return keep ? v : zero3<real>(); };
 
     const int z = z0 - 1 + lane;
     const bool zin = z >= 0 && z < nz;
@@ -913,13 +942,15 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                 const real* p = sHx + 3 * ((r + 1) * SC + lane + 1);
                 hn[k] = lds3(p);
                 e1[k] = zero3<real>();
-                if (pE1 && zin && y >= 0 && y < ny) {
+                const bool lanes = zin;
+                if (pE1 && (ALL_LANES || lanes) && y >= 0 && y < ny) {
                     const real* pym = p - 3 * SC;
                     const real* pzm = p - 3;
                     const Hit ey = find_hit(a.ef, 1, y);
-                    e1[k] = e_cell2<real, CPML, 1>(a, x, y, z, false, ex1, ey, ez_hit, ez_coef, hn[k], Hn_c[k], lds3(pym), pzm[0],
-                                                  pzm[1], eo[k], inv[k], pe1[k]);
-                    add_sources(a.se, a.nsrc_e, a.ebox_lo, a.ebox_hi, n, a.source_samples, x, y, z, e1[k].x, e1[k].y, e1[k].z);
+                    V3<real> v = e_cell2<real, CPML, 1>(a, q, x, y, z, false, ex1, ey, ez_hit, ez_coef, hn[k], Hn_c[k], lds3(pym),
+                                                        pzm[0], pzm[1], eo[k], inv[k], pe1[k]);
+                    add_sources(a.se, q.nse, q.ebl, q.ebh, n, q.samples, x, y, z, v.x, v.y, v.z);
+                    e1[k] = sel3(lanes, v);
                 }
             }
             // Stage 1b: H1(x-1) from E1(x-1) (LDS, with the row and column above) and E1(x).
@@ -928,13 +959,15 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
             for (int k = 0; k < RW; ++k) {
                 const int r = rowof(k), y = yof(k);
                 h1[k] = zero3<real>();
-                if (pH1 && r < NR - 1 && lane < 63 && zin && y >= 0 && y < ny) {
+                const bool lanes = lane < 63 && zin;
+                if (pH1 && r < NR - 1 && (ALL_LANES || lanes) && y >= 0 && y < ny) {
                     const real* p = sE1 + 3 * (r * 64 + lane);
                     const real* pzn = p + 3;
                     const Hit hy = find_hit(a.hf, 1, y);
-                    h1[k] = h_cell2<real, CPML, 1>(a, x - 1, y, z, false, hcoef, hx1, hy, hz_hit, hz_coef, E1_c[k], pE1 ? e1[k] : E1_c[k],
-                                                  lds3(p + 3 * 64), pzn[0], pzn[1], Hn_c[k], ph1[k]);
-                    add_sources(a.sh, a.nsrc_h, a.hbox_lo, a.hbox_hi, n, a.source_samples, x - 1, y, z, h1[k].x, h1[k].y, h1[k].z);
+                    V3<real> v = h_cell2<real, CPML, 1>(a, q, x - 1, y, z, false, hcoef, hx1, hy, hz_hit, hz_coef, E1_c[k],
+                                                        pE1 ? e1[k] : E1_c[k], lds3(p + 3 * 64), pzn[0], pzn[1], Hn_c[k], ph1[k]);
+                    add_sources(a.sh, q.nsh, q.hbl, q.hbh, n, q.samples, x - 1, y, z, v.x, v.y, v.z);
+                    h1[k] = sel3(lanes, v);
                 }
                 put3(sH1 + 3 * (r * 64 + lane), h1[k]);
             }
@@ -945,14 +978,16 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
             for (int k = 0; k < RW; ++k) {
                 const int r = rowof(k), y = yof(k);
                 e2[k] = zero3<real>();
-                if (pE2 && r >= 1 && r < NR - 1 && lane >= 1 && lane < 63 && zin && y >= 0 && y < ny) {
+                const bool lanes = lane >= 1 && lane < 63 && zin;
+                if (pE2 && r >= 1 && r < NR - 1 && (ALL_LANES || lanes) && y >= 0 && y < ny) {
                     const real* p = sH1 + 3 * (r * 64 + lane);
                     const real* pzm = p - 3;
-                    const bool owned = r < NR - 2 && lane < 62 && x - 1 < xb && y < ylim;
+                    const bool owned = lanes && r < NR - 2 && lane < 62 && x - 1 < xb && y < ylim;
                     const Hit ey = find_hit(a.ef, 1, y);
-                    e2[k] = e_cell2<real, CPML, 2>(a, x - 1, y, z, owned, ex2, ey, ez_hit, ez_coef, h1[k], H1_c[k], lds3(p - 3 * 64),
-                                                  pzm[0], pzm[1], E1_c[k], inv_c[k], pe0[k]);
-                    add_sources(a.se, a.nsrc_e, a.ebox_lo, a.ebox_hi, n + 1, a.source_samples, x - 1, y, z, e2[k].x, e2[k].y, e2[k].z);
+                    V3<real> v = e_cell2<real, CPML, 2>(a, q, x - 1, y, z, owned, ex2, ey, ez_hit, ez_coef, h1[k], H1_c[k],
+                                                        lds3(p - 3 * 64), pzm[0], pzm[1], E1_c[k], inv_c[k], pe0[k]);
+                    add_sources(a.se, q.nse, q.ebl, q.ebh, n + 1, q.samples, x - 1, y, z, v.x, v.y, v.z);
+                    e2[k] = sel3(lanes, v);
                     if (owned) store3(e_out, 3u * ((unsigned)(x - 1) * plane + (unsigned)y * nz + (unsigned)z), e2[k]);
                 }
             }
@@ -961,14 +996,15 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
     #pragma unroll
             for (int k = 0; k < RW; ++k) {
                 const int r = rowof(k), y = yof(k);
-                if (pH2 && r >= 1 && r < NR - 2 && lane >= 1 && lane < 62 && zin && y >= 0 && y < ny && y < ylim) {
+                const bool lanes = lane >= 1 && lane < 62 && zin;
+                if (pH2 && r >= 1 && r < NR - 2 && (ALL_LANES || lanes) && y >= 0 && y < ny && y < ylim) {
                     const real* p = sE2 + 3 * (r * 64 + lane);
                     const real* pzn = p + 3;
                     const Hit hy = find_hit(a.hf, 1, y);
-                    V3<real> h2 = h_cell2<real, CPML, 2>(a, x - 2, y, z, true, hcoef, hx2, hy, hz_hit, hz_coef, E2_c[k],
+                    V3<real> h2 = h_cell2<real, CPML, 2>(a, q, x - 2, y, z, lanes, hcoef, hx2, hy, hz_hit, hz_coef, E2_c[k],
                                                          pE2 ? e2[k] : E2_c[k], lds3(p + 3 * 64), pzn[0], pzn[1], H1_c[k], ph0[k]);
-                    add_sources(a.sh, a.nsrc_h, a.hbox_lo, a.hbox_hi, n + 1, a.source_samples, x - 2, y, z, h2.x, h2.y, h2.z);
-                    store3(h_out, 3u * ((unsigned)(x - 2) * plane + (unsigned)y * nz + (unsigned)z), h2);
+                    add_sources(a.sh, q.nsh, q.hbl, q.hbh, n + 1, q.samples, x - 2, y, z, h2.x, h2.y, h2.z);
+                    if (lanes) store3(h_out, 3u * ((unsigned)(x - 2) * plane + (unsigned)y * nz + (unsigned)z), h2);
                 }
             }
             lds_barrier();   // every read of sE1, sH1 and sE2 for this plane is done
