@@ -185,7 +185,8 @@ int tfdtd_yee_step_march(const tfdtd_march* step, void* stream);
  * Sources are added at both steps (samples n and n+1 of the device counter, which the call reads
  * and does not advance).  Tiles are 9 rows x 61 columns; chunk_x is the planes per workgroup
  * (about 20 suits a 512^3 grid) and tile_y the wavefronts of an interior workgroup, 4, 6 or 12
- * (0 = 12) splitting the 12 widened rows, or 16 for tall interior tiles of 13 owned rows (z-face and boundary tiles use 12).
+ * (0 = 12) splitting the 12 widened rows, or 16 for tall interior tiles of 13 owned rows
+ * (z-face and boundary tiles use 12).
  * Concurrency as tfdtd_yee_step_march. */
 int tfdtd_yee_step_march2(const tfdtd_march* step, void* stream);
 
