@@ -486,6 +486,7 @@ def run_summary(project, run, e, completed, termination, seconds, setup_seconds,
                    duration_fs=r.time_step*r.steps*1e15, backend='cuda', precision=r.precision,
                    gpu=props.name, gpu_arch=getattr(props, 'gcnArchName', None),
                    cuda_kernel=run.kernel, cuda_monitor_kernel='hip', engine='torchfdtd_amd HIP gfx950',
+                   steps_per_pass=2 if run.two_step else 1 if run.one_pass else 0,
                    steps=completed, requested_steps=r.steps, cancelled=termination == 'cancelled',
                    termination_reason=termination, auto_shutoff=termination == 'decayed',
                    seconds=seconds, setup_seconds=setup_seconds,
