@@ -88,6 +88,7 @@ struct March {
     int xparts[3];
     int yshift;                 // y tile ty starts at ty*9 - yshift
     int iy0, iy1, iytiles;      // tall interior tiles: rows [iy0, iy1) in iytiles tiles of 13 rows
+    int nxend;                  // boundary tiles in the x end chunks (the rest lie in x-middle chunks)
 };
 
 // psi <- psi*b + c*d (stored when `write`), d <- d*inv_k + psi, for slab row q of term k.
@@ -774,6 +775,8 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
     constexpr int RW = NR / W;   // NR widened rows (owned NR - 3), RW per wavefront
     static_assert(RW * W == NR, "the wavefronts split the widened rows evenly");
     static_assert(NR == 12 || CPML == 0, "tall tiles are interior tiles");
+    // CPML 6 (y and z slabs) takes the boundary tiles of the x-middle chunks, CPML 7 those of the x ends:
+    // both enumerate the boundary tiles in one order.
     constexpr int SR = NR + 1, SC = kTZ2 + 4, SH = SR * SC * 3;   // staged H(n): rows y0-2 .. y0+10, columns z0-2 .. z0+62
     constexpr int PL = NR * 64 * 3 + 64 * 3 + 3;                 // one widened plane, padded past its last row
     __shared__ real sA[SH];
@@ -798,7 +801,7 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
         y0 = a.iy0 + ty * (NR - 3);
         ylim = a.iy1;
     } else {
-        tile_coords<real, CPML>(a, w, G, ty, tz, chunk);
+        tile_coords<real, CPML == 6 ? 7 : CPML>(a, w, G, ty, tz, chunk);
         y0 = ty * kTY2 - a.yshift;
     }
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -1303,6 +1306,8 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 16 
     const int count = a.blocks[CPML == 0 ? 0 : CPML == 4 ? 1 : 2];
     const int w = b - prefix[lo];
     if (w >= count) return;
+    if (CPML == 7 && w >= a.nxend) return;   // an x-middle boundary tile: the CPML 6 launch takes it
+    if (CPML == 6 && w < a.nxend) return;
     tile2_body<float, EPS3, CPML, W, W == 16 ? 16 : 12>(a, w, count);
 }
 
@@ -1400,6 +1405,7 @@ int layout_march2(const tfdtd_march* d, March<float>* a) {
     a->iy1 = std::min(a->box_hi[1] * kTY2 - a->yshift, d->ny);
     a->iytiles = a->iy1 > a->iy0 ? (a->iy1 - a->iy0 + 12) / 13 : 0;
     if (d->tile_y == 16) a->blocks[0] = (a->box_hi[0] - a->box_lo[0]) * a->iytiles * (a->zbox_hi - a->zbox_lo);
+    a->nxend = (chunks - (a->box_hi[0] - a->box_lo[0])) * a->ytiles * a->ztiles;   // blocks A and B of tile_coords
     return TFDTD_OK;
 }
 
@@ -1450,8 +1456,13 @@ int launch_batch_tiles2(const void* table, const int32_t* prefix, int count, boo
                                 : go(batch_tile2_kernel<false, 0, W>, W, "batch march2 interior");
         if (v == 1) return eps3 ? go(batch_tile2_kernel<true, 4, 12>, 12, "batch march2 z faces")
                                 : go(batch_tile2_kernel<false, 4, 12>, 12, "batch march2 z faces");
-        return eps3 ? go(batch_tile2_kernel<true, 7, 12>, 12, "batch march2 boundary")
-                    : go(batch_tile2_kernel<false, 7, 12>, 12, "batch march2 boundary");
+        // Boundary tiles: those of the x end chunks (every slab) and those of the x-middle chunks (y and z
+        // slabs only, fewer registers) are two launches over the same workgroup range, one after the other.
+        const int status = eps3 ? go(batch_tile2_kernel<true, 7, 12>, 12, "batch march2 boundary")
+                                : go(batch_tile2_kernel<false, 7, 12>, 12, "batch march2 boundary");
+        if (status) return status;
+        return eps3 ? go(batch_tile2_kernel<true, 6, 12>, 12, "batch march2 y boundary")
+                    : go(batch_tile2_kernel<false, 6, 12>, 12, "batch march2 y boundary");
     });
 }
 
