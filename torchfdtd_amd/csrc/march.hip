@@ -775,8 +775,8 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
     constexpr int RW = NR / W;   // NR widened rows (owned NR - 3), RW per wavefront
     static_assert(RW * W == NR, "the wavefronts split the widened rows evenly");
     static_assert(NR == 12 || CPML == 0, "tall tiles are interior tiles");
-    // CPML 6 (y and z slabs) takes the boundary tiles of the x-middle chunks, CPML 7 those of the x ends:
-    // both enumerate the boundary tiles in one order.
+    // CPML 6 (y and z slabs) takes the boundary tiles of the x-middle chunks, CPML 7 and 1 (x slabs
+    // only) those of the x ends: all enumerate the boundary tiles in one order.
     constexpr int SR = NR + 1, SC = kTZ2 + 4, SH = SR * SC * 3;   // staged H(n): rows y0-2 .. y0+10, columns z0-2 .. z0+62
     constexpr int PL = NR * 64 * 3 + 64 * 3 + 3;                 // one widened plane, padded past its last row
     __shared__ real sA[SH];
@@ -801,7 +801,7 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
         y0 = a.iy0 + ty * (NR - 3);
         ylim = a.iy1;
     } else {
-        tile_coords<real, CPML == 6 ? 7 : CPML>(a, w, G, ty, tz, chunk);
+        tile_coords<real, (CPML == 6 || CPML == 1) ? 7 : CPML>(a, w, G, ty, tz, chunk);
         y0 = ty * kTY2 - a.yshift;
     }
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -1306,8 +1306,15 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 16 
     const int count = a.blocks[CPML == 0 ? 0 : CPML == 4 ? 1 : 2];
     const int w = b - prefix[lo];
     if (w >= count) return;
-    if (CPML == 7 && w >= a.nxend) return;   // an x-middle boundary tile: the CPML 6 launch takes it
+    if ((CPML == 7 || CPML == 1) && w >= a.nxend) return;   // an x-middle boundary tile: the CPML 6 launch takes it
     if (CPML == 6 && w < a.nxend) return;
+    if (CPML == 7 || CPML == 1) {
+        // x-end tiles whose rows and columns avoid the y and z slabs need the x terms only (CPML 1).
+        int ty, tz, chunk;
+        tile_coords<float, 7>(a, w, count, ty, tz, chunk);
+        const bool x_only = ty >= a.box_lo[1] && ty < a.box_hi[1] && tz >= a.zbox_lo && tz < a.zbox_hi;
+        if ((CPML == 1) != x_only) return;
+    }
     tile2_body<float, EPS3, CPML, W, W == 16 ? 16 : 12>(a, w, count);
 }
 
@@ -1456,10 +1463,14 @@ int launch_batch_tiles2(const void* table, const int32_t* prefix, int count, boo
                                 : go(batch_tile2_kernel<false, 0, W>, W, "batch march2 interior");
         if (v == 1) return eps3 ? go(batch_tile2_kernel<true, 4, 12>, 12, "batch march2 z faces")
                                 : go(batch_tile2_kernel<false, 4, 12>, 12, "batch march2 z faces");
-        // Boundary tiles: those of the x end chunks (every slab) and those of the x-middle chunks (y and z
-        // slabs only, fewer registers) are two launches over the same workgroup range, one after the other.
-        const int status = eps3 ? go(batch_tile2_kernel<true, 7, 12>, 12, "batch march2 boundary")
-                                : go(batch_tile2_kernel<false, 7, 12>, 12, "batch march2 boundary");
+        // Boundary tiles, three launches over the same workgroup range (each skips the others' tiles), one
+        // after the other: the x end chunks' tiles that meet y or z slabs (every slab), the other x end
+        // tiles (x slabs only) and the x-middle chunks' tiles (y and z slabs only); the fewer slabs a
+        // launch compiles in, the fewer registers it spills.
+        int status = eps3 ? go(batch_tile2_kernel<true, 7, 12>, 12, "batch march2 boundary")
+                          : go(batch_tile2_kernel<false, 7, 12>, 12, "batch march2 boundary");
+        if (!status) status = eps3 ? go(batch_tile2_kernel<true, 1, 12>, 12, "batch march2 x boundary")
+                                   : go(batch_tile2_kernel<false, 1, 12>, 12, "batch march2 x boundary");
         if (status) return status;
         return eps3 ? go(batch_tile2_kernel<true, 6, 12>, 12, "batch march2 y boundary")
                     : go(batch_tile2_kernel<false, 6, 12>, 12, "batch march2 y boundary");
