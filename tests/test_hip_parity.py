@@ -177,8 +177,8 @@ def test_two_step_pass_matches_oracle_bitwise(gpu, layout, chunk_x2, nz, compone
 
 def test_two_step_pass_with_sources_equals_one_step_pairs_under_graphs(gpu):
     """The cube workload at 96^3 (16-cell CPML, two-material 1/eps, Ez dipole): graph replays of the
-    two-step pass (quads, then a pair and a single step) equal the one-step pairs bit for bit,
-    source included."""
+    two-step pass (graphs of fours then a pair and a single step, a graph of one four, single steps)
+    equal the one-step pairs bit for bit, source included."""
     from torchfdtd_amd.graphs import StepGraphs
     from torchfdtd_amd.solver import Stepper
     from torchfdtd_amd.workloads import cube, cube_inverse_permittivity, cube_noise_mask
@@ -192,9 +192,14 @@ def test_two_step_pass_with_sources_equals_one_step_pairs_under_graphs(gpu):
                             prepare=run.refresh_materials, quad=run.quad if two_step else None)
         for _ in range(2):
             graphs.replay(23)
+        if two_step:
+            graphs.replay(4)            # the graph of one four
+        else:
+            for _ in range(4):
+                graphs.replay(1)
         graphs.replay(1)
         torch.cuda.synchronize()
-        assert int(run.counter.item()) == 47
+        assert int(run.counter.item()) == 51
         out.append([t.clone() for t in (run.grid.E, run.grid.H, *run.grid.memory_states)])
     assert float(out[0][0].abs().max()) > 0
     for a, b in zip(*out):

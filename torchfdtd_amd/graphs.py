@@ -34,7 +34,8 @@ class StepGraphs:
 
     With `pair` (two steps that return the state to its own buffers) the `width` graph
     is built from pairs, plus one `step` when `width` is odd; with `quad` as well (four steps
-    that return the state to its own buffers) from quads first.  `prepare`, when given, runs
+    that return the state to its own buffers) from quads first, and a graph of one quad is
+    captured too (`replay(4)`).  `prepare`, when given, runs
     before every replay (e.g. Stepper.refresh_materials, which follows permittivity changes).
     """
 
@@ -56,7 +57,10 @@ class StepGraphs:
         torch.cuda.current_stream().wait_stream(stream)
         torch.cuda.synchronize()
         self.graphs = {}
-        for count in sorted({1, self.width}):
+        counts = {1, self.width}
+        if quad is not None and self.width > 4:
+            counts.add(4)       # one four: replays that stop at any multiple of four stay on the quad path
+        for count in sorted(counts):
             graph = torch.cuda.CUDAGraph()
             with torch.cuda.stream(stream):
                 graph.capture_begin()
