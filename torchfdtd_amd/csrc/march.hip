@@ -588,8 +588,9 @@ template <typename real, int STAGE>
 __device__ __forceinline__ real cpml_lane2(real d, const Family<real>& F, const Hit& hit, const ZCoef<real>& zc, int k,
                                            unsigned p, bool write, real& slot) {
     const unsigned o = (k ? hit.psi1 : hit.psi0) + p;
+    // STAGE 3: step n+1 with psi at time n already in the slot (loaded a plane ahead).
     const real m = (STAGE == 1 ? global(F.psi_in)[o] : slot) * zc.b + zc.c * d;
-    if (STAGE == 1) slot = m;
+    if (STAGE == 1 || STAGE == 3) slot = m;
     else if (write) global(F.psi_out)[o] = m;
     return d * zc.ik + m;
 }
@@ -607,7 +608,7 @@ struct Consts2 {
 };
 
 // e_cell with the CPML memory of term t in slot[t].
-template <typename real, int CPML, int STAGE>
+template <typename real, int CPML, int STAGE, bool ZPRE = false>
 __device__ __forceinline__ V3<real> e_cell2(const March<real>& a, const Consts2<real>& q, int x, int y, int z, bool write,
                                             const Hit& hx, const Hit& hy, const Hit& hz, const ZCoef<real>& zc, V3<real> h,
                                             V3<real> hxm, V3<real> hym, real hx_zm, real hy_zm, V3<real> e, V3<real> inv,
@@ -653,7 +654,7 @@ __device__ __forceinline__ V3<real> e_cell2(const March<real>& a, const Consts2<
         real d = real(0);
         if (z > 0) {
             d = h.y - hy_zm;
-            if ((CPML & 4) && hz.face >= 0) d = cpml_lane2<real, STAGE>(d, a.ef, hz, zc, 0, pz, write, slot[4]);
+            if ((CPML & 4) && hz.face >= 0) d = cpml_lane2<real, (ZPRE && STAGE == 1) ? 3 : STAGE>(d, a.ef, hz, zc, 0, pz, write, slot[4]);
         }
         cx = cx - d;
     }
@@ -661,7 +662,7 @@ __device__ __forceinline__ V3<real> e_cell2(const March<real>& a, const Consts2<
         real d = real(0);
         if (z > 0) {
             d = h.x - hx_zm;
-            if ((CPML & 4) && hz.face >= 0) d = cpml_lane2<real, STAGE>(d, a.ef, hz, zc, 1, pz, write, slot[5]);
+            if ((CPML & 4) && hz.face >= 0) d = cpml_lane2<real, (ZPRE && STAGE == 1) ? 3 : STAGE>(d, a.ef, hz, zc, 1, pz, write, slot[5]);
         }
         cy = cy + d;
     }
@@ -669,7 +670,7 @@ __device__ __forceinline__ V3<real> e_cell2(const March<real>& a, const Consts2<
 }
 
 // h_cell with the CPML memory of term t in slot[t].
-template <typename real, int CPML, int STAGE>
+template <typename real, int CPML, int STAGE, bool ZPRE = false>
 __device__ __forceinline__ V3<real> h_cell2(const March<real>& a, const Consts2<real>& q, int x, int y, int z, bool write,
                                             real coefficient, const Hit& hx, const Hit& hy, const Hit& hz, const ZCoef<real>& zc,
                                             V3<real> e, V3<real> exn, V3<real> eyn, real ex_zp, real ey_zp, V3<real> h,
@@ -723,7 +724,7 @@ __device__ __forceinline__ V3<real> h_cell2(const March<real>& a, const Consts2<
         real d = real(0);
         if (z < nz - 1) {
             d = ey_zp - e.y;
-            if ((CPML & 4) && hz.face >= 0) d = cpml_lane2<real, STAGE>(d, a.hf, hz, zc, 0, pz, write, slot[4]);
+            if ((CPML & 4) && hz.face >= 0) d = cpml_lane2<real, (ZPRE && STAGE == 1) ? 3 : STAGE>(d, a.hf, hz, zc, 0, pz, write, slot[4]);
         } else if (q.pec_mask & 4) {
             d = -q.pec[2] * e.y;
         }
@@ -733,7 +734,7 @@ __device__ __forceinline__ V3<real> h_cell2(const March<real>& a, const Consts2<
         real d = real(0);
         if (z < nz - 1) {
             d = ex_zp - e.x;
-            if ((CPML & 4) && hz.face >= 0) d = cpml_lane2<real, STAGE>(d, a.hf, hz, zc, 1, pz, write, slot[5]);
+            if ((CPML & 4) && hz.face >= 0) d = cpml_lane2<real, (ZPRE && STAGE == 1) ? 3 : STAGE>(d, a.hf, hz, zc, 1, pz, write, slot[5]);
         } else if (q.pec_mask & 4) {
             d = -q.pec[2] * e.x;
         }
@@ -899,18 +900,35 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
         // ahead, right after the barrier, so that the loads overlap a plane of work instead of the barrier
         // that follows them; the boundary tiles (register-bound) load them just before that barrier.
         constexpr bool PREFETCH = CPML != 7;
+        // The z-face tiles also load the z-slab CPML memory that step n+1 of the next plane reads (E at
+        // plane x+1, H at plane x) with it.
+        constexpr bool ZPRE = CPML == 4;
         V3<real> eo_n[RW], inv3_n[RW];
         real inv1_n[RW];
         unsigned idx_n[RW];
+        real pzE[RW][2] = {}, pzH[RW][2] = {};
         auto load_plane = [&](int x) {
             const unsigned base = (unsigned)x * plane;
     #pragma unroll
             for (int k = 0; k < RW; ++k) {
-                const unsigned cell = base + (unsigned)min(max(yof(k), 0), ny - 1) * nz + zc;
+                const int yk = yof(k);
+                const unsigned cell = base + (unsigned)min(max(yk, 0), ny - 1) * nz + zc;
                 eo_n[k] = load3(e_in, 3u * cell);
                 if (EPS3) inv3_n[k] = load3(inv_eps, 3u * cell);
                 else if (mat) idx_n[k] = global(mat)[cell];
                 else inv1_n[k] = global(inv_eps)[cell];
+                if (ZPRE && yk >= 0 && yk < ny) {
+                    if (ez_hit.face >= 0) {
+                        const unsigned pz = ((unsigned)x * ny + yk) * ez_hit.w + ez_hit.q;
+                        pzE[k][0] = global(a.ef.psi_in)[ez_hit.psi0 + pz];
+                        pzE[k][1] = global(a.ef.psi_in)[ez_hit.psi1 + pz];
+                    }
+                    if (hz_hit.face >= 0 && x >= 1) {
+                        const unsigned pz = ((unsigned)(x - 1) * ny + yk) * hz_hit.w + hz_hit.q;
+                        pzH[k][0] = global(a.hf.psi_in)[hz_hit.psi0 + pz];
+                        pzH[k][1] = global(a.hf.psi_in)[hz_hit.psi1 + pz];
+                    }
+                }
             }
         };
         if (PREFETCH && xs < nx) load_plane(xs);
@@ -935,6 +953,12 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                     }
                 }
             }
+            real zE[RW][2], zH[RW][2];   // this plane's z-slab CPML memory, before the next plane's loads
+    #pragma unroll
+            for (int k = 0; k < RW; ++k) {
+                zE[k][0] = pzE[k][0]; zE[k][1] = pzE[k][1];
+                zH[k][0] = pzH[k][0]; zH[k][1] = pzH[k][1];
+            }
             if (PREFETCH && x + 1 <= xe && x + 1 < nx) load_plane(x + 1);
             V3<real> e1[RW], hn[RW], h1[RW], e2[RW];
             // Stage 1: E1(x).
@@ -950,8 +974,9 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                     const real* pym = p - 3 * SC;
                     const real* pzm = p - 3;
                     const Hit ey = find_hit(a.ef, 1, y);
-                    V3<real> v = e_cell2<real, CPML, 1>(a, q, x, y, z, false, ex1, ey, ez_hit, ez_coef, hn[k], Hn_c[k], lds3(pym),
-                                                        pzm[0], pzm[1], eo[k], inv[k], pe1[k]);
+                    if (ZPRE) { pe1[k][4] = zE[k][0]; pe1[k][5] = zE[k][1]; }
+                    V3<real> v = e_cell2<real, CPML, 1, ZPRE>(a, q, x, y, z, false, ex1, ey, ez_hit, ez_coef, hn[k], Hn_c[k],
+                                                              lds3(pym), pzm[0], pzm[1], eo[k], inv[k], pe1[k]);
                     add_sources(a.se, q.nse, q.ebl, q.ebh, n, q.samples, x, y, z, v.x, v.y, v.z);
                     e1[k] = sel3(lanes, v);
                 }
@@ -967,8 +992,10 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                     const real* p = sE1 + 3 * (r * 64 + lane);
                     const real* pzn = p + 3;
                     const Hit hy = find_hit(a.hf, 1, y);
-                    V3<real> v = h_cell2<real, CPML, 1>(a, q, x - 1, y, z, false, hcoef, hx1, hy, hz_hit, hz_coef, E1_c[k],
-                                                        pE1 ? e1[k] : E1_c[k], lds3(p + 3 * 64), pzn[0], pzn[1], Hn_c[k], ph1[k]);
+                    if (ZPRE) { ph1[k][4] = zH[k][0]; ph1[k][5] = zH[k][1]; }
+                    V3<real> v = h_cell2<real, CPML, 1, ZPRE>(a, q, x - 1, y, z, false, hcoef, hx1, hy, hz_hit, hz_coef, E1_c[k],
+                                                              pE1 ? e1[k] : E1_c[k], lds3(p + 3 * 64), pzn[0], pzn[1], Hn_c[k],
+                                                              ph1[k]);
                     add_sources(a.sh, q.nsh, q.hbl, q.hbh, n, q.samples, x - 1, y, z, v.x, v.y, v.z);
                     h1[k] = sel3(lanes, v);
                 }
