@@ -15,18 +15,26 @@ from .scene import Scene
 from .yee import Grid
 
 
-def prepare(project):
+def prepare(project, native=False):
+    """(scene, eps, grid); native=True steps the grid with the C restatement (oracle/native.py)."""
     scene = Scene(project)
     eps = scene.voxelize()
     inverse = np.ones((*scene.shape, 3), scene.real)
     inverse[:] = 1/(eps if eps.ndim == 4 else eps[..., None])
-    grid = Grid(scene, inverse)
+    if native:
+        from .native import NativeGrid
+        grid = NativeGrid(scene, inverse)
+    else:
+        grid = Grid(scene, inverse)
     return scene, eps, grid
 
 
-def run(project, steps=None, grid=None, record_frames=True):
-    scene, eps, fresh = prepare(project)
-    g = grid if grid is not None else fresh
+def run(project, steps=None, grid=None, record_frames=True, native=False):
+    if grid is None:
+        scene, eps, g = prepare(project, native=native)
+    else:
+        scene, g = grid.scene, grid
+        eps = scene.voxelize()
     steps = scene.steps if steps is None else steps
     terms = scene.source_terms()
     sources = {'E': [], 'H': []}

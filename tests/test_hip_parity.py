@@ -31,6 +31,12 @@ def region_for(layout, precision):
         r.boundaries.z_max = BoundaryFace(kind='pec')
         r.boundaries.y_max = BoundaryFace(kind='pec')
         r.boundaries.y_min = BoundaryFace(kind='pec')
+    if layout in ('xpec', 'xpec2'):
+        # CPML on y and z only at the high x end (both x ends for xpec2): the slab-free x middle runs
+        # to the grid's last plane, so z-face and y-boundary tiles of its last chunk end at nx.
+        r.boundaries.x_max = BoundaryFace(kind='pec')
+        if layout == 'xpec2':
+            r.boundaries.x_min = BoundaryFace(kind='pec')
     return r
 
 
@@ -70,7 +76,8 @@ def test_random_fields_and_every_cpml_state_bitwise(gpu, precision, layout):
 @pytest.mark.parametrize('precision', ['float32', 'float64'])
 @pytest.mark.parametrize('layout,tile_y,chunk_x,nz,components', [
     ('pml3d', 8, 7, 131, 3), ('pml3d', 16, 64, 200, 3), ('pec', 8, 5, 131, 3), ('pml3d', 16, 0, 131, 3),
-    ('pml3d', 8, 9, 200, 3), ('pec', 16, 6, 200, 3), ('pml3d', 8, 6, 200, 'index'), ('pec', 16, 4, 200, 1)])
+    ('pml3d', 8, 9, 200, 3), ('pec', 16, 6, 200, 3), ('pml3d', 8, 6, 200, 'index'), ('pec', 16, 4, 200, 1),
+    ('xpec', 8, 7, 131, 3), ('xpec2', 16, 6, 131, 1)])
 def test_one_pass_step_matches_oracle_bitwise(gpu, precision, layout, tile_y, chunk_x, nz, components):
     """The one-pass E+H kernel (ping-pong buffers, LDS tiles with halo recompute, x chunks, CPML-free
     interior and CPML boundary launches) equals the reference update bit for bit, with 1/eps per
@@ -121,7 +128,9 @@ def test_one_pass_step_matches_oracle_bitwise(gpu, precision, layout, tile_y, ch
 @pytest.mark.parametrize('layout,chunk_x2,nz,components,waves2', [
     ('pml3d', 7, 131, 3, 12), ('pml3d', 20, 200, 'index', 12), ('pec', 5, 131, 1, 12), ('pml3d', 0, 131, 1, 12),
     ('pec', 20, 200, 3, 12), ('pml3d', 3, 64, 'index', 12), ('pml3d', 7, 131, 3, 16), ('pml3d', 20, 200, 'index', 16),
-    ('pec', 5, 131, 1, 16), ('pml3d', 4, 64, 'index', 4)])
+    ('pec', 5, 131, 1, 16), ('pml3d', 4, 64, 'index', 4),
+    ('xpec', 7, 131, 3, 12), ('xpec', 20, 131, 1, 12), ('xpec', 9, 200, 'index', 16), ('xpec2', 6, 131, 3, 12),
+    ('xpec2', 20, 64, 1, 4)])
 def test_two_step_pass_matches_oracle_bitwise(gpu, layout, chunk_x2, nz, components, waves2):
     """The two-step pass (step n+1 recomputed on widened tiles and kept on chip, E, H and both CPML
     memories double-buffered) equals the reference update bit for bit, starting from random fields

@@ -907,6 +907,18 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
         real inv1_n[RW];
         unsigned idx_n[RW];
         real pzE[RW][2] = {}, pzH[RW][2] = {};
+        // H-family z-slab memory of plane x-1, which stage 1b of plane x reads (x <= nx).
+        auto load_zh = [&](int x) {
+    #pragma unroll
+            for (int k = 0; k < RW; ++k) {
+                const int yk = yof(k);
+                if (ZPRE && yk >= 0 && yk < ny && hz_hit.face >= 0 && x >= 1) {
+                    const unsigned pz = ((unsigned)(x - 1) * ny + yk) * hz_hit.w + hz_hit.q;
+                    pzH[k][0] = global(a.hf.psi_in)[hz_hit.psi0 + pz];
+                    pzH[k][1] = global(a.hf.psi_in)[hz_hit.psi1 + pz];
+                }
+            }
+        };
         auto load_plane = [&](int x) {
             const unsigned base = (unsigned)x * plane;
     #pragma unroll
@@ -917,19 +929,13 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                 if (EPS3) inv3_n[k] = load3(inv_eps, 3u * cell);
                 else if (mat) idx_n[k] = global(mat)[cell];
                 else inv1_n[k] = global(inv_eps)[cell];
-                if (ZPRE && yk >= 0 && yk < ny) {
-                    if (ez_hit.face >= 0) {
-                        const unsigned pz = ((unsigned)x * ny + yk) * ez_hit.w + ez_hit.q;
-                        pzE[k][0] = global(a.ef.psi_in)[ez_hit.psi0 + pz];
-                        pzE[k][1] = global(a.ef.psi_in)[ez_hit.psi1 + pz];
-                    }
-                    if (hz_hit.face >= 0 && x >= 1) {
-                        const unsigned pz = ((unsigned)(x - 1) * ny + yk) * hz_hit.w + hz_hit.q;
-                        pzH[k][0] = global(a.hf.psi_in)[hz_hit.psi0 + pz];
-                        pzH[k][1] = global(a.hf.psi_in)[hz_hit.psi1 + pz];
-                    }
+                if (ZPRE && yk >= 0 && yk < ny && ez_hit.face >= 0) {
+                    const unsigned pz = ((unsigned)x * ny + yk) * ez_hit.w + ez_hit.q;
+                    pzE[k][0] = global(a.ef.psi_in)[ez_hit.psi0 + pz];
+                    pzE[k][1] = global(a.ef.psi_in)[ez_hit.psi1 + pz];
                 }
             }
+            load_zh(x);
         };
         if (PREFETCH && xs < nx) load_plane(xs);
         auto march_plane = [&](const int x, real* const sHx, real* const sHn) {
@@ -960,6 +966,9 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                 zH[k][0] = pzH[k][0]; zH[k][1] = pzH[k][1];
             }
             if (PREFETCH && x + 1 <= xe && x + 1 < nx) load_plane(x + 1);
+            // A chunk that ends at the grid's last plane (no high-x slab) still computes H1(nx-1) at
+            // x = nx: its z-slab memory is loaded without a plane nx of E to go with it.
+            else if (ZPRE && x + 1 <= xe && x + 1 == nx) load_zh(nx);
             V3<real> e1[RW], hn[RW], h1[RW], e2[RW];
             // Stage 1: E1(x).
             const Hit ex1 = find_hit(a.ef, 0, x);
