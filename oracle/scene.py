@@ -259,6 +259,21 @@ def pulse_parameters(s):
     return frequency, sigma, offset, span, chirped
 
 
+def gaussian_envelope(u, taper):
+    """waveforms.py:45-54 gaussian_envelope: exp(-u^2/2), with taper the tail beyond TAIL_INNER is the cubic
+    a d^2 + b d^3 in d = TAIL_OUTER - |u| that meets the Gaussian's 1e-4 value and slope at TAIL_INNER and
+    reaches 0 with zero slope at TAIL_OUTER."""
+    u = np.asarray(u, dtype=np.float64)
+    envelope = np.exp(-.5*u*u)
+    if taper:
+        h = TAIL_OUTER-TAIL_INNER
+        distance = np.maximum(TAIL_OUTER-np.abs(u), 0)
+        a = (3e-4-TAIL_INNER*1e-4*h)/(h*h)
+        b = (TAIL_INNER*1e-4*h-2e-4)/(h*h*h)
+        envelope = np.where(np.abs(u) <= TAIL_INNER, envelope, a*distance**2+b*distance**3)
+    return envelope
+
+
 def source_time_signal(s, times):
     """waveforms.py:57-74 for gaussian, broadband and continuous pulses."""
     times = np.asarray(times, dtype=np.float64)
@@ -273,13 +288,7 @@ def source_time_signal(s, times):
         envelope = 1-np.exp(-(times/sigma)**2)
         carrier = omega*times
     else:
-        envelope = np.exp(-.5*u*u)
-        if s['eliminate_discontinuities']:
-            h = TAIL_OUTER-TAIL_INNER
-            distance = np.maximum(TAIL_OUTER-np.abs(u), 0)
-            a = (3e-4-TAIL_INNER*1e-4*h)/(h*h)
-            b = (TAIL_INNER*1e-4*h-2e-4)/(h*h*h)
-            envelope = np.where(np.abs(u) <= TAIL_INNER, envelope, a*distance**2+b*distance**3)
+        envelope = gaussian_envelope(u, s['eliminate_discontinuities'])
         carrier = omega*times if s['time_definition'] == 'cycles' else -omega*x
         if chirped:
             inside = span*sigma*(4*np.cos(np.pi*u/4)-2*np.pi)
