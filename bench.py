@@ -124,12 +124,132 @@ def batched_line(steps, count, loop_grids, launch='tune', width=50):
         loop_seconds += result.summary['seconds']
         loop_cells += math.prod(c.project.region.shape)
     loop = loop_cells*steps/loop_seconds/1e9 if loop_grids else None
+    step_bytes = sum(algorithmic_bytes(c.project.region)[0] for c in cases)
+    plane_bytes = sum(plane_bytes_per_step(c.project) for c in cases)
+    per_step = report.plan['loop_seconds']/steps
+    achieved = (step_bytes+plane_bytes)/per_step/1e9
+    roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
+                    frac=round(achieved/HBM_PEAK_GBS, 4), traffic=None,
+                    kernel=f'batched {chosen.kernel} step of all {count} grids plus their DFT planes; duration = '
+                           'per-step span of the timed cohort loop',
+                    bytes_per_cell_update=round((step_bytes+plane_bytes)/cells, 2),
+                    field_bytes_per_cell_update=round(step_bytes/cells, 2))
     return dict(workload=f'{count} uneven 3D grids 64x64x32..192x160x64, 10-cell CPML, {steps} steps, Ez dipole, '
                          '16-frequency float64 DFT plane, binary design block', cells=cells,
                 value=round(batched, 3), unit='Gcell-updates/s', loop_value=round(loop, 3) if loop else None,
                 loop_grids=loop_grids, speedup_vs_loop=round(batched/loop, 3) if loop else None, launch=chosen.as_dict(),
-                tuning=timings,
+                tuning=timings, roofline=roofline,
                 loop_seconds=round(report.plan['loop_seconds'], 4))
+
+
+def plane_bytes_per_step(project):
+    """Algorithmic bytes per step of a project's frequency planes: each complex accumulator element
+    read and written once per accumulation block, and 4 bytes per sampled interpolation corner."""
+    from torchfdtd_amd.monitors import frequency_samples, plane_plan
+    r = project.region
+    total = 0.
+    for raw in project.monitors:
+        if not raw.enabled or raw.kind != 'field':
+            continue
+        m = project.resolved_monitor(raw)
+        points = len(plane_plan(r, m)['weights'])
+        comps = len(m.required_fields)
+        nf = len(frequency_samples(m.spectrum))
+        spec = 16 if (r.precision == 'float64' or m.dft_precision == 'float64') else 8
+        block = max(1, min(32, 65536//(4*(spec//2)*nf)))
+        corners = 8 if r.dimension == '3d' else 4
+        total += 2*spec*nf*points*comps/block+4*corners*points*comps
+    return total
+
+
+def observed_lines(width=50, repeats=3):
+    """The 160x120 2D TM waveguide (point trace, 32-frequency spectrum) and the 160x120x40 3D
+    waveguide (point trace), 1000 steps each through Simulation.run (graphs of `width` steps):
+    Gcell-updates/s (median of `repeats` runs after a warm-up), and the FP32 error against an FP64
+    run of the same project (the FP64 engine equals the FP64 oracle of the reference CPU path bit for
+    bit), relative L2 over the whole fields, the trace and the spectrum."""
+    import statistics
+
+    import numpy as np
+
+    from torchfdtd_amd import workloads as W
+    from torchfdtd_amd.solver import Simulation
+    from torchfdtd_amd.monitors import point_spectrum
+    out = {}
+    for name, make in (('waveguide_2d_160x120', W.waveguide_2d), ('waveguide_3d_160x120x40', W.waveguide_3d)):
+        project = make()
+        r = project.region
+        print(f'bench: {name}', file=sys.stderr, flush=True)
+        Simulation(project).run(cuda_graph_steps=width)
+        runs = [Simulation(project).run(cuda_graph_steps=width) for _ in range(repeats)]
+        seconds = statistics.median(run.summary['seconds'] for run in runs)
+        cells = math.prod(r.shape)
+        exact = project.model_copy(deep=True)
+        exact.region.precision = 'float64'
+        ref = Simulation(exact).run(cuda_graph_steps=width)
+        got = runs[0]
+        rel = lambda a, b: float(np.linalg.norm((np.asarray(a)-np.asarray(b)).ravel())/np.linalg.norm(np.asarray(b).ravel()))
+        m = project.resolved_monitor(project.monitors[0])
+        settings = m.spectrum.model_copy(update=dict(sampling='wavelength', frequency_points=32, wavelength_start=1.5,
+                                                     wavelength_stop=1.6, apodization='none'))
+        spectrum = point_spectrum(got.times, got.signals[:, 0].astype(np.float64), settings)['value']
+        exact_spectrum = point_spectrum(ref.times, ref.signals[:, 0], settings)['value']
+        step_bytes = algorithmic_bytes(r)[0] if r.dimension == '3d' else 4*(6*cells+cells)+8*psi_tm(r)
+        out[name] = dict(value=round(cells*r.steps/seconds/1e9, 3), unit='Gcell-updates/s', steps=r.steps, cells=cells,
+                         seconds=round(seconds, 4), samples_seconds=[round(run.summary['seconds'], 4) for run in runs],
+                         steps_per_pass=got.summary['steps_per_pass'],
+                         effective_gbs=round(step_bytes*r.steps/seconds/1e9, 1),
+                         fp32_vs_fp64_rel_l2=dict(E=rel(got.electric, ref.electric), H=rel(got.magnetic, ref.magnetic),
+                                                  trace=rel(got.signals, ref.signals),
+                                                  spectrum=rel(spectrum, exact_spectrum)))
+    return out
+
+
+def psi_tm(region):
+    """CPML memory elements of the TM components of a 2D grid (Ez from Hx/Hy, Hx/Hy from Ez)."""
+    from torchfdtd_amd.planning import cpml_slabs
+    total = sum(math.prod(s.psi_shape(region.shape)) for (_, axis, _), slabs in cpml_slabs(region).items() if axis != 2
+                for s in slabs)
+    return total/2      # of each axis's two terms per family, one feeds a TM component
+
+
+def surrogate_line(batch=128, reps=5):
+    """One training step (forward, data and weight gradients, AdamW) of the 1.2M-parameter 3D conv
+    surrogate on a batch of 128 densities 64x64x8, on the BF16-MFMA implicit-GEMM kernels: dense
+    conv FLOPs (forward + dgrad + wgrad) per second against the 2.5 PFLOP/s BF16 MFMA peak."""
+    import torch
+
+    from torchfdtd_amd.surrogate import SurrogateNet
+    net = SurrogateNet().cuda()
+    opt = torch.optim.AdamW(net.parameters(), lr=3e-4)
+    rho = torch.rand((batch, 64, 64, 8), device='cuda')
+    target = torch.rand((batch, 32), device='cuda')
+    flops, dims, ci = 0, (64, 64, 8), 1
+    for w in net.weights:
+        co = w.shape[0]
+        dims = tuple((n+2-3)//2+1 for n in dims)
+        flops += 3*2*batch*math.prod(dims)*co*27*ci      # forward, dgrad and wgrad
+        ci = co
+    flops += 3*2*batch*(256*256+256*32)
+
+    def step():
+        opt.zero_grad(set_to_none=True)
+        torch.nn.functional.mse_loss(net(rho), target).backward()
+        opt.step()
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    seconds = e0.elapsed_time(e1)/1e3/reps
+    return dict(workload='SurrogateNet training step: 4 stride-2 3x3x3 convs 1->32->64->128->256 on 128x64x64x8, MLP head',
+                parameters=net.parameter_count(), ms_per_step=round(seconds*1e3, 3),
+                conv_tflops=round(flops/seconds/1e12, 2), peak_tflops=2500.0,
+                frac=round(flops/seconds/1e12/2500.0, 5), dtype='bf16 MFMA, fp32 accumulate')
 
 
 def adjoint_line(steps=2000):
@@ -169,8 +289,13 @@ def adjoint_line(steps=2000):
     plain_forward()
     gradient()
     plain_seconds = plain_forward()
-    total, planes, objective, rho = gradient()
+    total, planes, objective, recorded = gradient()
     report = planes['out'].report
+    # The configured schedule: checkpoints every 45 steps, replayed in the reverse sweep.
+    model = DifferentiablePlaneSimulation(project, AdjointOptions(design_box=box, checkpoint_interval=45))
+    gradient()
+    configured, configured_planes, _, replayed = gradient()
+    configured_report = configured_planes['out'].report
     return dict(workload=f'200x160x60 cells at 0.04 um, {steps} steps, 12-cell CPML, 50x50x6 density design, '
                          'objective sum |DFT Ez|^2 over a plane at 8 wavelengths 1.50-1.60 um',
                 seconds_per_gradient=round(total, 4), plain_forward_seconds=round(plain_seconds, 4),
@@ -178,8 +303,16 @@ def adjoint_line(steps=2000):
                 backward_seconds=round(report['backward_seconds'], 4),
                 backward_over_forward=round(report['backward_seconds']/report['forward_seconds'], 3),
                 checkpoint_interval=report['checkpoint_interval'], record_bytes=report['record_bytes'],
-                observers=report['observers'], objective=float(objective),
-                gradient_norm=float(rho.grad.double().norm()))
+                observers=report['observers'], objective=float(objective.detach()),
+                gradient_norm=float(recorded.grad.double().norm()),
+                checkpoint_45=dict(seconds_per_gradient=round(configured, 4),
+                                   gradient_over_plain_forward=round(configured/plain_seconds, 3),
+                                   bound=2.5, checkpoint_interval=configured_report['checkpoint_interval'],
+                                   checkpoints=configured_report['checkpoints'],
+                                   replayed_steps=configured_report['replayed_steps'],
+                                   record_bytes=configured_report['record_bytes'],
+                                   gradient_rel_diff_vs_record_all=float((replayed.grad.double()-recorded.grad.double()).norm()
+                                                                         / recorded.grad.double().norm())))
 
 
 def main():
@@ -193,6 +326,9 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-batched', action='store_true')
     ap.add_argument('--no-adjoint', action='store_true')
+    ap.add_argument('--no-observed', action='store_true', help='skip the 2D/3D waveguide lines with their FP64 error')
+    ap.add_argument('--no-surrogate', action='store_true')
+    ap.add_argument('--repeats', type=int, default=3, help='further timed samples of --steps steps (median reported)')
     ap.add_argument('--batch-grids', type=int, default=64)
     ap.add_argument('--batch-steps', type=int, default=2000)
     ap.add_argument('--batch-launch', choices=('tune', 'split', 'march'), default='tune')
@@ -218,10 +354,7 @@ def main():
         return 0
     if args.no_cube:
         line = {}
-        if not args.no_batched:
-            line['batched'] = batched_line(args.batch_steps, args.batch_grids, args.loop_grids, args.batch_launch)
-        if not args.no_adjoint:
-            line['adjoint'] = adjoint_line()
+        extra_lines(args, line)
         print(json.dumps(line), flush=True)
         return 0
     from torchfdtd_amd import _native as N
@@ -267,6 +400,14 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter()-wall
     seconds = start.elapsed_time(stop)/1e3
+    # Further samples of the same number of steps (the state keeps evolving; every step costs the same).
+    samples = [seconds]
+    for _ in range(args.repeats-1):
+        start.record()
+        advance(args.steps)
+        stop.record()
+        torch.cuda.synchronize()
+        samples.append(start.elapsed_time(stop)/1e3)
     cells = math.prod(region.shape)
     value = cells*args.steps/seconds/1e9
     assert torch.isfinite(g.E).all() and torch.isfinite(g.H).all()
@@ -307,9 +448,12 @@ def main():
     else:
         per_step_ms, kernel = kernel_ms['yee_half_step_E']+kernel_ms['yee_half_step_H'], 'yee_half_step E+H'
     achieved = step_bytes/(per_step_ms/1e3)/1e9
+    traffic = counted_traffic(args.n, run.two_step) if run.one_pass else None
     roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
-                    frac=round(achieved/HBM_PEAK_GBS, 4),
-                    traffic=counted_traffic(args.n, run.two_step) if run.one_pass else None,
+                    frac=round(achieved/HBM_PEAK_GBS, 4), traffic=traffic,
+                    traffic_source=('copied from profiles/march_512cube_hbm_counters.md (rocprofv3 --pmc FETCH_SIZE and '
+                                    'WRITE_SIZE per dispatch, scaled by a plain copy), not measured in this run')
+                    if traffic else None,
                     kernel=kernel, kernel_ms=kernel_ms, stream_copy=round(copy_gbs, 1),
                     frac_of_copy=round(achieved/copy_gbs, 4),
                     bytes_per_cell_update=round(step_bytes/cells, 2), psi_per_cell=round(psi_per_cell, 4))
@@ -324,14 +468,28 @@ def main():
             'config': {'workload': f'{args.n}^3 cube, 16-cell CPML all faces, eps=1+11s per cell, Ez dipole, '
                                    f'{region.steps - 8} steps of which {args.steps} timed',
                        'cells': cells, 'graph_steps': width, 'wall_s': round(wall, 3)},
+            'samples_ms_per_step': [round(t*1e3/args.steps, 4) for t in samples],
+            'median_ms_per_step': round(sorted(samples)[len(samples)//2]*1e3/args.steps, 4),
             'roofline': roofline, 'cpu_baseline': baseline}
+    del graph, run, g, mask
+    torch.cuda.empty_cache()
+    extra_lines(args, line)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def extra_lines(args, line):
+    """The other configurations' lines, each under its own key of the one JSON line."""
     if not args.no_batched:
         line['batched'] = batched_line(args.batch_steps, args.batch_grids, args.loop_grids, args.batch_launch)
+    if not args.no_observed:
+        line['observed'] = observed_lines()
     if not args.no_adjoint:
         print('bench: adjoint gradient', file=sys.stderr, flush=True)
         line['adjoint'] = adjoint_line()
-    print(json.dumps(line), flush=True)
-    return 0
+    if not args.no_surrogate:
+        print('bench: surrogate training step', file=sys.stderr, flush=True)
+        line['surrogate'] = surrogate_line()
 
 
 if __name__ == '__main__':

@@ -280,9 +280,61 @@ int tfdtd_norm_pack(const tfdtd_norm_grid* grids, int32_t count, void* host_tabl
 int tfdtd_state_norm(const void* device_table, int32_t count, int64_t tasks, int32_t dtype, double* partial, double* out,
                      void* stream);
 
+
+/* Run-control decision at a diagnostic barrier, on the device (replaces the host-side
+ * DecayDecision.update of torchfdtd/run_control.py:58-85 and the host check of
+ * torchfdtd/run_control.py:127-159), so that a barrier enqueues work and never waits for the GPU.
+ * Per grid g it reads the norms out[g] = {energy, field peak, non-finite flag} that
+ * tfdtd_state_norm wrote and applies, in order:
+ *   non-finite flag, energy or peak                          -> verdict NONFINITE (value 1 when
+ *                                                               energy and peak are finite: the
+ *                                                               CPML memory alone, else 0)
+ *   divergence_check and field_limit > 0 and peak > limit     -> verdict FIELD_LIMIT (value = peak)
+ *   eligible = step >= min_steps and step * time_step >= eligible_time
+ *   peak_energy = max(peak_energy, energy); at the first eligible check the baseline freezes at it
+ *   ratio = energy / baseline (baseline = peak_energy until frozen; 0 when the baseline is 0)
+ *   divergence_check and eligible and ratio > growth_limit    -> verdict GROWTH (value = ratio)
+ *   below = eligible and ratio <= decay_threshold ? below + 1 : 0
+ *   history[g][checks++] = {step, step * time_step, energy, ratio, peak, eligible}
+ *   auto_shutoff and below >= consecutive_checks              -> verdict DECAYED
+ * A grid with a verdict other than GOING is left unchanged by later calls.  rules and states are
+ * device arrays of `count`; zero the states before the run.  history holds rows_per_grid rows of 6
+ * doubles per grid (rows beyond it are dropped; the states still count them). */
+enum { TFDTD_RUN_GOING = 0, TFDTD_RUN_DECAYED = 1, TFDTD_RUN_NONFINITE = 2, TFDTD_RUN_FIELD_LIMIT = 3,
+       TFDTD_RUN_GROWTH = 4 };
+
+typedef struct tfdtd_decay_rule {
+    double eligible_time;           /* end of the sources plus after_source_s, in s */
+    double time_step;               /* s */
+    double decay_threshold;
+    double growth_limit;
+    double field_limit;             /* 0: no limit */
+    int32_t min_steps;
+    int32_t consecutive_checks;
+    int32_t divergence_check;
+    int32_t auto_shutoff;
+} tfdtd_decay_rule;
+
+typedef struct tfdtd_decay_state {
+    double peak_energy;             /* largest energy so far */
+    double baseline;                /* frozen baseline (valid once frozen != 0) */
+    double verdict_value;           /* field peak (FIELD_LIMIT), ratio (GROWTH), CPML-only flag (NONFINITE) */
+    int64_t verdict_step;
+    int32_t frozen;
+    int32_t below;                  /* eligible checks in a row at or under the threshold */
+    int32_t verdict;                /* TFDTD_RUN_* */
+    int32_t checks;                 /* checks evaluated (history rows written, up to rows_per_grid) */
+} tfdtd_decay_state;
+
+int tfdtd_decay_update(const tfdtd_decay_rule* rules, tfdtd_decay_state* states, const double* norms, int32_t count,
+                       int64_t step, double* history, int32_t rows_per_grid, void* stream);
+
 /* Forward record for the adjoint: curl H of the E update (backward differences, periodic wrap
  * included) on the design box cells [box_lo, box_hi), written to
- * record[counter % modulus][box cell][3] -- read before the step's E update.  No reference
+ * record[counter % modulus][box cell][3] -- read before the step's E update.  With the E-family
+ * CPML `e` (psi_in = the memory before this step's E update; psi_out is not written) a term in a
+ * slab row records the curl the update applies there, d inv_k + (b psi + c d); with e.psi_in
+ * NULL every term records the plain difference d.  No reference
  * counterpart: the reference recomputes this curl from replayed states
  * (torchfdtd/differentiable.py:508-509, torchfdtd/cuda_adjoint.py:108-216). */
 typedef struct tfdtd_curl_record {
@@ -294,6 +346,7 @@ typedef struct tfdtd_curl_record {
     void* record;
     int64_t modulus;
     const int64_t* counter;
+    tfdtd_march_family e;
 } tfdtd_curl_record;
 
 int tfdtd_record_curl(const tfdtd_curl_record* record, void* stream);
@@ -310,7 +363,8 @@ int tfdtd_record_curl(const tfdtd_curl_record* record, void* stream);
  * the caller alternates); coef is the forward coefficient block.  seed_targets are
  * int64[nseed][2] = (0 E | 1 H, element index).  The gradient is double[box cells][stride]
  * (stride 1 sums the components).  With record == NULL no gradient is formed; a step with t < 0
- * adds neither seeds nor gradient. */
+ * adds neither seeds nor gradient.  A box that meets an E-family CPML slab needs a record made
+ * with that CPML (record_cpml = 1): the gradient there uses the curl the update applied. */
 typedef struct tfdtd_adjoint {
     int32_t dtype;
     int32_t nx, ny, nz;
@@ -338,6 +392,7 @@ typedef struct tfdtd_adjoint {
     double* gradient;
     int32_t gradient_stride;
     int64_t* counter;
+    int32_t record_cpml;                /* 1: the record carries the E-family CPML terms */
 } tfdtd_adjoint;
 
 int tfdtd_adjoint_step(const tfdtd_adjoint* step, void* stream);

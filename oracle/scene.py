@@ -137,18 +137,30 @@ class Scene:
                 out.append((nodes[:-1]+nodes[1:])/2 if half else nodes[:-1])
         return out
 
-    def voxelize(self):
-        """Relative permittivity per cell ('cell') or per E component ('yee'): solver.py:49-111."""
+    def voxelize(self, with_ownership=False):
+        """Relative permittivity per cell ('cell') or per E component ('yee'): solver.py:49-111.
+        with_ownership: (eps, ownership, counts) -- the int32 index of the owning material in the
+        project's material list (-1 background) and, per structure id, the samples it set."""
         if self.material_sampling == 'yee':
-            return np.stack([self._voxelize_at(self.field_axes(c)) for c in ('Ex', 'Ey', 'Ez')], axis=-1)
+            parts = [self._voxelize_at(self.field_axes(c)) for c in ('Ex', 'Ey', 'Ez')]
+            eps = np.stack([p[0] for p in parts], axis=-1)
+            if not with_ownership:
+                return eps
+            counts = {k: max(p[2].get(k, 0) for p in parts) for k in parts[0][2]}
+            return eps, np.stack([p[1] for p in parts], axis=-1), counts
         axes = [(np.arange(n)+.5)*self.mesh-s/2 for n, s in zip(self.shape, self.actual_size)]
         if self.dimension == '2d':
             axes[2] = np.array([0.])
-        return self._voxelize_at(axes)
+        eps, owner, counts = self._voxelize_at(axes)
+        return (eps, owner, counts) if with_ownership else eps
 
     def _voxelize_at(self, axes):
         shape = tuple(len(a) for a in axes)
         eps = np.full(shape, self.background_index**2, dtype=self.real)
+        owner = np.full(shape, -1, dtype=np.int32)
+        counts = {}
+        listed = self.project.get('materials', MATERIALS_DEFAULT)
+        index = {m['name']: i for i, m in enumerate(listed)}
         mats = self.materials()
         structures = [_structure(s) for s in self.project.get('structures', [])]
         for obj in sorted(structures, key=lambda s: -s.mesh_order):
@@ -162,12 +174,15 @@ class Scene:
                                     int(np.searchsorted(axis, c+w/2+tol, 'right'))))
             slices = tuple(slices)
             if any(s.start == s.stop for s in slices):
+                counts[obj.id] = 0
                 continue
             xyz = np.meshgrid(*(a[s] for a, s in zip(axes, slices)), indexing='ij', sparse=True)
             mask = _contains(obj, *xyz)
             m = mats[obj.material]
             eps[slices][mask] = m.get('index', 1.5)**2
-        return eps
+            owner[slices][mask] = index[obj.material]
+            counts[obj.id] = int(np.count_nonzero(mask))
+        return eps, owner, counts
 
     # ---------------------------------------------------------------- sources
     def index_at(self, center, component=None):
@@ -297,34 +312,125 @@ def source_time_signal(s, times):
     return s['amplitude']*envelope*np.sin(carrier+phase)
 
 
-STRUCTURE_DEFAULT = dict(kind='rectangle', size=(1, 1, 0.5), radius=0.5, center=(0, 0, 0), enabled=True,
-                         material='SiN (constant n)', mesh_order=2, rotation=0, rotation_angles=(0, 0, 0),
-                         make_ellipsoid=False)
+STRUCTURE_DEFAULT = dict(kind='rectangle', size=(1, 1, 0.5), radius=0.5, inner_radius=0.3, center=(0, 0, 0), enabled=True,
+                         material='SiN (constant n)', mesh_order=2, rotation=0, rotation_axes=('z', 'x', 'y'),
+                         rotation_angles=(0, 0, 0), make_ellipsoid=False, radius_2=.5, radius_3=.5, inner_radius_2=.3,
+                         theta_start=0, theta_stop=360, vertices=((-.5, -.5), (.5, -.5), (0, .5)), holes=(), id=None)
 
 
 def _structure(raw):
     s = dict(STRUCTURE_DEFAULT)
     s.update(raw)
-    if s['rotation'] or any(s['rotation_angles']) or s['make_ellipsoid'] or s['kind'] not in ('rectangle', 'sphere', 'circle'):
-        raise NotImplementedError('oracle: unrotated rectangles, spheres and circles only')
+    s['angular_span'] = (s['theta_stop']-s['theta_start']) % 360 or 360.
     return SimpleNamespace(**s)
 
 
+def _quarter_turn(axis, angle):
+    """geometry.py:9-16: rotation about one axis, exact at multiples of 90 degrees."""
+    if axis == 'none':
+        return np.eye(3)
+    angle %= 360
+    c, s = {0: (1., 0.), 90: (0., 1.), 180: (-1., 0.), 270: (0., -1.)}.get(
+        angle, (math.cos(math.radians(angle)), math.sin(math.radians(angle))))
+    a = 'xyz'.index(axis)
+    b, d = (a+1) % 3, (a+2) % 3
+    r = np.eye(3)
+    r[b, b] = r[d, d] = c
+    r[b, d] = -s
+    r[d, b] = s
+    return r
+
+
+def rotation_matrix(obj):
+    """geometry.py:19-26: local -> world; the legacy z angle (unreduced) first, then the listed axes."""
+    a = math.radians(obj.rotation)
+    c, s = math.cos(a), math.sin(a)
+    r = np.array([[c, -s, 0.], [s, c, 0.], [0., 0., 1.]])
+    for axis, angle in zip(obj.rotation_axes, obj.rotation_angles):
+        r = _quarter_turn(axis, angle)@r
+    return r
+
+
+def _radii(obj):
+    return (obj.radius, obj.radius_2 if obj.make_ellipsoid else obj.radius,
+            obj.radius_3 if obj.make_ellipsoid else obj.radius)
+
+
 def _bounds(obj):
-    """geometry.py:133-148 with the identity rotation."""
+    """geometry.py:136-148: support box (centre, span)."""
+    r = rotation_matrix(obj)
+    center = np.asarray(obj.center, dtype=float)
+    if obj.kind == 'polygon':
+        points = np.array([[x, y, z] for x, y in obj.vertices for z in (-obj.size[2]/2, obj.size[2]/2)])@r.T+center
+        low, high = points.min(axis=0), points.max(axis=0)
+        return tuple((low+high)/2), tuple(high-low)
     if obj.kind == 'rectangle':
-        return tuple(obj.center), tuple(float(v) for v in obj.size)
-    if obj.kind == 'sphere':
-        return tuple(obj.center), (2*obj.radius,)*3
-    return tuple(obj.center), (2*obj.radius, 2*obj.radius, float(obj.size[2]))
+        half = np.abs(r)@(np.array(obj.size)/2)
+    elif obj.kind == 'sphere':
+        half = np.sqrt((r*r)@(np.array(_radii(obj))**2))
+    else:
+        rx, ry, _ = _radii(obj)
+        half = np.sqrt((r[:, 0]*rx)**2+(r[:, 1]*ry)**2)+abs(r[:, 2])*obj.size[2]/2
+    return tuple(center), tuple(2*half)
+
+
+def _contour_parity(x, y, vertices):
+    """geometry.py:83-93: even-odd crossing parity and closed-boundary membership of one contour."""
+    shape = np.broadcast_shapes(np.shape(x), np.shape(y))
+    inside, boundary = np.zeros(shape, bool), np.zeros(shape, bool)
+    v = np.asarray(vertices, dtype=float)
+    span = np.ptp(v, axis=0).max()
+    tol = 64*np.finfo(float).eps*max(span, 1e-300)
+    for (ax, ay), (bx, by) in zip(v, np.roll(v, -1, axis=0)):
+        cross = (bx-ax)*(y-ay)-(by-ay)*(x-ax)
+        boundary |= ((abs(cross) <= tol*max(abs(bx-ax), abs(by-ay))) & (x >= min(ax, bx)-tol) & (x <= max(ax, bx)+tol)
+                     & (y >= min(ay, by)-tol) & (y <= max(ay, by)+tol))
+        if by != ay:
+            inside ^= ((ay > y) != (by > y)) & (x < (bx-ax)*(y-ay)/(by-ay)+ax)
+    return inside, boundary
+
+
+def _polygon_contains(x, y, vertices, holes):
+    """geometry.py:96-101: even-odd over the outer contour and its holes; every boundary is material."""
+    inside, boundary = _contour_parity(x, y, vertices)
+    for hole in holes:
+        parity, edge = _contour_parity(x, y, hole)
+        inside ^= parity
+        boundary |= edge
+    return inside | boundary
 
 
 def _contains(obj, x, y, z):
-    """geometry.py:106-130 with the identity rotation."""
-    u, v, w = x-obj.center[0], y-obj.center[1], z-obj.center[2]
+    """geometry.py:104-130: membership of the local coordinates (u, v, w) = R^T (p - centre)."""
+    delta = [x-obj.center[0], y-obj.center[1], z-obj.center[2]]
+    isotropic = not obj.make_ellipsoid
+    only_z = all(axis in ('none', 'z') or angle % 360 == 0 for axis, angle in zip(obj.rotation_axes, obj.rotation_angles))
+    invariant = ((obj.kind == 'sphere' and isotropic)
+                 or (obj.kind in ('circle', 'ring') and isotropic and only_z and (obj.kind != 'ring' or obj.angular_span == 360)))
+    if invariant or (obj.rotation == 0 and not any(obj.rotation_angles)):
+        u, v, w = delta
+    elif not any(obj.rotation_angles):
+        a = math.radians(obj.rotation)
+        c, s = math.cos(a), math.sin(a)
+        u, v, w = c*delta[0]+s*delta[1], -s*delta[0]+c*delta[1], delta[2]
+    else:
+        m = rotation_matrix(obj)
+        u, v, w = [sum(m[a, b]*delta[a] for a in range(3) if m[a, b] != 0) for b in range(3)]
     if obj.kind == 'rectangle':
         return (abs(u) <= obj.size[0]/2) & (abs(v) <= obj.size[1]/2) & (abs(w) <= obj.size[2]/2)
-    r = obj.radius
+    if obj.kind == 'polygon':
+        return _polygon_contains(u, v, obj.vertices, obj.holes) & (abs(w) <= obj.size[2]/2)
+    rx, ry, rz = _radii(obj)
     if obj.kind == 'sphere':
-        return u*u+v*v+w*w <= r*r
-    return (u*u+v*v <= r*r) & (abs(w) <= obj.size[2]/2)
+        return (u/rx)**2+(v/ry)**2+(w/rz)**2 <= 1 if obj.make_ellipsoid else u*u+v*v+w*w <= rx*rx
+    mask = ((u/rx)**2+(v/ry)**2 <= 1 if obj.make_ellipsoid else u*u+v*v <= rx*rx) & (abs(w) <= obj.size[2]/2)
+    if obj.kind == 'ring':
+        if obj.inner_radius > 0:
+            if obj.make_ellipsoid:
+                mask &= (u/obj.inner_radius)**2+(v/obj.inner_radius_2)**2 >= 1
+            else:
+                mask &= u*u+v*v >= obj.inner_radius**2
+        if obj.angular_span < 360:
+            angle = (np.degrees(np.arctan2(v, u))-obj.theta_start) % 360
+            mask &= (angle <= obj.angular_span+1e-12) | ((u == 0) & (v == 0))
+    return mask

@@ -29,7 +29,10 @@ def prepare(project, native=False):
     return scene, eps, grid
 
 
-def run(project, steps=None, grid=None, record_frames=True, native=False):
+def run(project, steps=None, grid=None, record_frames=True, native=False, rule=None, check_interval=50):
+    """Step the project; with `rule` (oracle.run_control.Rule) the FP64 state norm is judged every
+    `check_interval` steps and at the last step, stopping the loop where the rule stops the run
+    (solver.py:844-881)."""
     if grid is None:
         scene, eps, g = prepare(project, native=native)
     else:
@@ -66,12 +69,23 @@ def run(project, steps=None, grid=None, record_frames=True, native=False):
         for k, (loc, c) in enumerate(locs):
             traces[n, k] = (g.E if c[0] == 'E' else g.H)[loc+('xyz'.index(c[1].lower()),)]
         done = n+1
-        if record_frames and (done % interval == 0 or done == steps):
+        stop = False
+        if rule is not None and (done % check_interval == 0 or done == steps):
+            from .run_control import state_norm
+            energy, peak, finite = state_norm(g.E, g.H, g.inverse_permittivity, g.memory_states)
+            if not finite:
+                raise FloatingPointError('Non-finite CPML or incident-line memory detected.')
+            stop = rule.update(done, energy, peak)
+        if record_frames and (done % interval == 0 or done == steps or stop):
             sl = [slice(None)]*3
             sl[axis] = plane_index
             data = (g.E if scene.field[0] == 'E' else g.H)[..., comp][tuple(sl)]
             frames.append(data[::max(1, math.ceil(data.shape[0]/256)), ::max(1, math.ceil(data.shape[1]/256))].copy())
             frame_steps.append(done)
+        if stop:
+            steps = done
+            traces = traces[:done]
+            break
     times = np.arange(1, steps+1)*scene.dt
     spectra = []
     for k, m in enumerate(points):

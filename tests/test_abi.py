@@ -67,6 +67,11 @@ int main(void) {
   F(tfdtd_norm_grid, eps_stride); F(tfdtd_norm_grid, widths); F(tfdtd_norm_grid, psi); F(tfdtd_norm_grid, psi_count);
   printf("tfdtd_curl_record %zu\n", sizeof(tfdtd_curl_record));
   F(tfdtd_curl_record, box_lo); F(tfdtd_curl_record, record); F(tfdtd_curl_record, modulus); F(tfdtd_curl_record, counter);
+  F(tfdtd_curl_record, e); F(tfdtd_adjoint, record_cpml);
+  printf("tfdtd_decay_rule %zu\n", sizeof(tfdtd_decay_rule));
+  printf("tfdtd_decay_state %zu\n", sizeof(tfdtd_decay_state));
+  F(tfdtd_decay_rule, field_limit); F(tfdtd_decay_rule, auto_shutoff);
+  F(tfdtd_decay_state, verdict_step); F(tfdtd_decay_state, checks);
   printf("tfdtd_adjoint %zu\n", sizeof(tfdtd_adjoint));
   F(tfdtd_adjoint, courant); F(tfdtd_adjoint, e); F(tfdtd_adjoint, h); F(tfdtd_adjoint, nseed); F(tfdtd_adjoint, seeds); F(tfdtd_adjoint, seed_rows);
   printf("tfdtd_conv %zu\n", sizeof(tfdtd_conv));
@@ -87,6 +92,7 @@ def test_ctypes_mirror_matches_the_c_layout(tmp_path):
               'tfdtd_cpml_segment': N.CpmlSegment, 'tfdtd_source': N.SourceBox, 'tfdtd_march': N.March,
               'tfdtd_march_face': N.MarchFace, 'tfdtd_march_family': N.MarchFamily, 'tfdtd_norm_grid': N.NormGrid,
               'tfdtd_curl_record': N.CurlRecord, 'tfdtd_adjoint': N.Adjoint,
+              'tfdtd_decay_rule': N.DecayRule, 'tfdtd_decay_state': N.DecayState,
               'tfdtd_conv': __import__('torchfdtd_amd.surrogate', fromlist=['ConvShape']).ConvShape}
     for key, value in lines.items():
         if '.' in key:

@@ -74,13 +74,14 @@ def test_gradient_matches_oracle_adjoint(gpu, precision, layout, per_component):
     J, g, result = engine_gradient(project, eps, weights)
     tol = 1e-9 if precision == 'float64' else 2e-4
     np.testing.assert_allclose(result.signals.detach().cpu().numpy(), s_ref, rtol=0, atol=tol*np.abs(s_ref).max())
-    box = interior(project)
-    scale = np.abs(g_ref[box]).max()
+    # dJ/deps over epsilon's whole shape, the CPML cells included (differentiable.py:503-505).
+    scale = np.abs(g_ref).max()
     assert scale > 0
-    np.testing.assert_allclose(g[box], g_ref[box], rtol=0, atol=tol*scale)
-    outside = np.ones(g.shape, bool)
-    outside[box] = False
-    assert not g[outside].any()     # the gradient is formed on the design box only
+    np.testing.assert_allclose(g, g_ref, rtol=0, atol=tol*scale)
+    box = interior(project)
+    slabs = np.ones(g.shape, bool)
+    slabs[box] = False
+    assert np.abs(g_ref[slabs]).max() > 1e-3*scale      # the CPML cells carry gradient too
 
 
 def test_checkpointed_replay_is_bitwise_identical(gpu):
@@ -91,7 +92,7 @@ def test_checkpointed_replay_is_bitwise_identical(gpu):
     _, every, r0 = engine_gradient(project, eps, weights, AdjointOptions(checkpoint_interval=0))
     _, replayed, r1 = engine_gradient(project, eps, weights, AdjointOptions(checkpoint_interval=7, cuda_graph_steps=5))
     assert r0.report['replayed_steps'] == 0 and r1.report['replayed_steps'] == 40
-    assert r1.report['checkpoint_interval'] == 8      # rounded to an even step for the one-pass pairs
+    assert r1.report['checkpoint_interval'] == 7      # odd intervals run as given
     assert np.array_equal(every, replayed)
     box = ((4, 10), (5, 8), (4, 6))
     _, small, _ = engine_gradient(project, eps, weights, AdjointOptions(design_box=box))

@@ -68,7 +68,7 @@ def test_parameterization_filter_projection_and_symmetry():
     p = DensityParameterization((9, 9), spacing_um=.05, mode='density', initial=.37, filter_radius_um=.12, beta=0,
                                 dtype=torch.float64)
     assert torch.allclose(p(), torch.full((9, 9), .37, dtype=torch.float64))      # unit-sum kernel, periodic edges
-    assert p._kernel.sum().item() == pytest.approx(1)
+    assert p.filter.weight.sum().item() == pytest.approx(1)
     q = DensityParameterization((8, 8), spacing_um=.05, mode='density', initial=torch.rand(8, 8, dtype=torch.float64),
                                 filter_radius_um=.1, symmetry='dihedral4', beta=8, eta=.4, dtype=torch.float64)
     out = q()
@@ -96,3 +96,55 @@ def test_parameterization_filter_projection_and_symmetry():
         assert r.design.grad[cell].item() == pytest.approx((up-down)/2e-6, rel=1e-6, abs=1e-9)
     with pytest.raises(ValueError):
         DensityParameterization((4, 5), spacing_um=.05, symmetry='rotate90')
+
+
+def brute_filter(density, spacing, radius, boundary):
+    """(1 - d/r)+ weighted mean by direct summation: periodic over minimum-image torus offsets (one
+    per residue), truncate over in-domain pixels only."""
+    nx, ny = density.shape
+    out = np.zeros((nx, ny))
+    offsets = []
+    for n in (nx, ny):
+        reach = min(n-1, math.ceil(radius/spacing))
+        offsets.append(range(-min(n//2, reach), min((n-1)//2, reach)+1) if boundary == 'periodic' else range(-reach, reach+1))
+    for i in range(nx):
+        for j in range(ny):
+            total = norm = 0.
+            for a in offsets[0]:
+                for b in offsets[1]:
+                    w = max(0., 1-math.hypot(a*spacing, b*spacing)/radius)
+                    if boundary == 'periodic':
+                        total += w*density[(i+a) % nx, (j+b) % ny]
+                        norm += w
+                    elif 0 <= i+a < nx and 0 <= j+b < ny:
+                        total += w*density[i+a, j+b]
+                        norm += w
+            out[i, j] = total/norm
+    return out
+
+
+@pytest.mark.parametrize('boundary,shape,radius', [('periodic', (7, 6), .12), ('periodic', (4, 5), .4),
+                                                   ('truncate', (7, 6), .12), ('truncate', (5, 3), .2)])
+def test_conic_filter_equals_direct_summation(boundary, shape, radius):
+    rng = np.random.default_rng(7)
+    rho = rng.random(shape)
+    p = DensityParameterization(shape, spacing_um=.05, mode='density', initial=torch.as_tensor(rho), filter_radius_um=radius,
+                                boundary=boundary, beta=0, dtype=torch.float64)
+    np.testing.assert_allclose(p().detach().numpy(), brute_filter(rho, .05, radius, boundary), rtol=1e-12, atol=1e-14)
+
+
+def test_symmetry_mean_and_fixed_pixels():
+    rng = np.random.default_rng(8)
+    rho = torch.as_tensor(rng.random((6, 6)))
+    p = DensityParameterization((6, 6), spacing_um=.05, mode='density', initial=rho, symmetry='mirror_x', beta=0,
+                                dtype=torch.float64)
+    np.testing.assert_allclose(p().detach().numpy(), ((rho+rho.flip(0))/2).numpy(), rtol=0, atol=1e-15)
+    mask = torch.zeros((6, 6), dtype=torch.bool)
+    mask[0, :] = mask[-1, :] = True
+    q = DensityParameterization((6, 6), spacing_um=.05, mode='density', initial=rho, symmetry='mirror_x', beta=4.,
+                                fixed_mask=mask, fixed_values=1., dtype=torch.float64)
+    out = q()
+    assert torch.all(out[mask] == 1) and torch.allclose(out, out.flip(0))
+    with pytest.raises(ValueError):
+        mask[0, 0] = False
+        DensityParameterization((6, 6), spacing_um=.05, symmetry='mirror_x', fixed_mask=mask)

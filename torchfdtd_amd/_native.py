@@ -104,9 +104,24 @@ class NormGrid(ctypes.Structure):
                 ('inverse_permittivity', vp), ('eps_stride', i32), ('widths', vp*3), ('psi', vp*2), ('psi_count', i64*2)]
 
 
+class DecayRule(ctypes.Structure):
+    _fields_ = [('eligible_time', ctypes.c_double), ('time_step', ctypes.c_double), ('decay_threshold', ctypes.c_double),
+                ('growth_limit', ctypes.c_double), ('field_limit', ctypes.c_double), ('min_steps', i32),
+                ('consecutive_checks', i32), ('divergence_check', i32), ('auto_shutoff', i32)]
+
+
+class DecayState(ctypes.Structure):
+    _fields_ = [('peak_energy', ctypes.c_double), ('baseline', ctypes.c_double), ('verdict_value', ctypes.c_double),
+                ('verdict_step', i64), ('frozen', i32), ('below', i32), ('verdict', i32), ('checks', i32)]
+
+
+RUN_GOING, RUN_DECAYED, RUN_NONFINITE, RUN_FIELD_LIMIT, RUN_GROWTH = range(5)
+
+
 class CurlRecord(ctypes.Structure):
     _fields_ = [('dtype', i32), ('nx', i32), ('ny', i32), ('nz', i32), ('H', vp), ('periodic_mask', i32),
-                ('box_lo', i32*3), ('box_hi', i32*3), ('record', vp), ('modulus', i64), ('counter', vp)]
+                ('box_lo', i32*3), ('box_hi', i32*3), ('record', vp), ('modulus', i64), ('counter', vp),
+                ('e', MarchFamily)]
 
 
 class Adjoint(ctypes.Structure):
@@ -115,7 +130,8 @@ class Adjoint(ctypes.Structure):
                 ('periodic_mask', i32), ('pec_upper_mask', i32), ('pec_upper_factor', ctypes.c_double*3),
                 ('e', MarchFamily), ('h', MarchFamily), ('nseed', i32), ('seed_targets', vp), ('seed_offsets', vp),
                 ('seed_columns', vp), ('seeds', vp), ('seed_width', i32), ('seed_rows', i64), ('box_lo', i32*3), ('box_hi', i32*3),
-                ('record', vp), ('record_modulus', i64), ('gradient', vp), ('gradient_stride', i32), ('counter', vp)]
+                ('record', vp), ('record_modulus', i64), ('gradient', vp), ('gradient_stride', i32), ('counter', vp),
+                ('record_cpml', i32)]
 
 
 _lib = None
@@ -138,6 +154,7 @@ PROTOTYPES = {
     'tfdtd_norm_table_bytes': (ctypes.c_size_t, [ctypes.POINTER(NormGrid), i32]),
     'tfdtd_norm_pack': (i32, [ctypes.POINTER(NormGrid), i32, vp, ctypes.c_size_t]),
     'tfdtd_state_norm': (i32, [vp, i32, i64, i32, vp, vp, vp]),
+    'tfdtd_decay_update': (i32, [vp, vp, vp, i32, i64, vp, i32, vp]),
     'tfdtd_record_curl': (i32, [ctypes.POINTER(CurlRecord), vp]),
     'tfdtd_adjoint_step': (i32, [ctypes.POINTER(Adjoint), vp]),
     'tfdtd_conv_output': (i32, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]),

@@ -47,8 +47,9 @@ class AdjointOptions:
     """checkpoint_interval: steps between stored forward states (0: record every step's curl and
     replay nothing; None: 0 when the record fits `record_budget_bytes`, else an even ~sqrt(T)).
     record_budget_bytes: default a quarter of the free device memory.  design_box: cells
-    ((x0, x1), (y0, y1), (z0, z1)) whose gradient is formed (default: outside the E-update CPML
-    slabs).  cuda_graph_steps: replay width of the forward and reverse graphs."""
+    ((x0, x1), (y0, y1), (z0, z1)) whose gradient is formed (default: the whole grid, CPML cells
+    included, as the reference returns dJ/deps over epsilon's shape; a box records a smaller
+    curl).  cuda_graph_steps: replay width of the forward and reverse graphs."""
     checkpoint_interval: int | None = None
     record_budget_bytes: int | None = None
     design_box: tuple | None = None
@@ -139,15 +140,17 @@ class _Engine:
         if options.checkpoint_interval is None:
             interval = 0 if every <= budget else _even_interval(self.steps)
         else:
+            # Any interval serves: every graph replay (pairs, then one single step for an odd count)
+            # leaves the state in the grid's own buffers, where checkpoints are taken.
             interval = options.checkpoint_interval
-            if interval and run.one_pass and interval % 2:
-                interval += 1       # checkpoints fall where the state lies in the grid's own buffers
         self.interval = interval if interval < self.steps else 0
         self.modulus = self.steps if self.interval == 0 else self.interval
         self.record = torch.zeros((self.modulus, self.box_cells, 3), device=g.device, dtype=self.real)
-        self.records = [self._record_descriptor(g.H)]
+        # Each record reads the E-family CPML memory its step's E update reads: the grid's own
+        # buffer for single steps and the first step of a pair, the second set for the pair's second.
+        self.records = [self._record_descriptor(g.H, g.psi_buffers[False], g.psi_buffers[False])]
         if run.one_pass:
-            self.records.append(self._record_descriptor(run.H_b))
+            self.records.append(self._record_descriptor(run.H_b, run.psi_e_b, g.psi_buffers[False]))
         width = min(options.cuda_graph_steps, self.steps)
         self.width = width
         self.forward_graphs = StepGraphs(self._step, run.states()+[self.record], self.steps, width,
@@ -159,24 +162,17 @@ class _Engine:
 
     # ------------------------------------------------------------------ set-up
     def _box(self, box):
-        g, r = self.grid, self.grid.region
+        r = self.grid.region
         if box is None:
-            # Everything outside the CPML slabs of the E update.
-            lo, hi = [0, 0, 0], list(r.shape)
-            for (forward, axis, _), entries in g.cpml.items():
-                for e in entries if not forward else ():
-                    if e['side'] == 0:
-                        lo[axis] = max(lo[axis], e['hi'])
-                    else:
-                        hi[axis] = min(hi[axis], e['lo'])
-            box = tuple(zip(lo, hi))
+            box = tuple((0, n) for n in r.shape)
         box = tuple((int(a), int(b)) for a, b in box)
         if len(box) != 3 or any(not 0 <= a < b <= n for (a, b), n in zip(box, r.shape)):
             raise ValueError(f'design box {box} must lie inside the grid {tuple(r.shape)}')
         return box
 
-    def _record_descriptor(self, H):
+    def _record_descriptor(self, H, psi_e, psi_e_out):
         d = N.CurlRecord()
+        d.e = march_family(self.grid, False, psi_e, psi_e_out)
         d.dtype = self.dtype
         d.nx, d.ny, d.nz = self.grid.Nx, self.grid.Ny, self.grid.Nz
         d.H = H.data_ptr()
@@ -267,6 +263,7 @@ class _Engine:
                 d.box_lo[a], d.box_hi[a] = self.box[a]
             d.record, d.record_modulus = self.record.data_ptr(), self.modulus
             d.gradient, d.gradient_stride = self.gradient.data_ptr(), 3 if self.per_component else 1
+            d.record_cpml = 1
             d.counter = self.reverse_counter.data_ptr()
             descriptors.append(d)
         return descriptors

@@ -40,8 +40,8 @@ from .graphs import StepGraphs, observation_schedule, validate_graph_steps
 from .planning import periodic_axes
 from .models import Project
 from .diagnostics import StateNorms
-from .run_control import DecayDecision, source_end_time
-from .solver import ENGINE_LOCK, DisplaySlice, Result, Stepper, pack_table as _pack, run_summary
+from .run_control import RunMonitor, check_count
+from .solver import ENGINE_LOCK, DisplaySlice, Result, Stepper, pack_table as _pack, run_signature, run_summary
 
 
 @dataclass
@@ -175,6 +175,8 @@ class Cohort:
         for p in self.projects:
             if p.region.steps != first.steps or p.region.precision != first.precision:
                 raise ValueError('Every project of a cohort needs the same steps and precision.')
+            if p.region.run_control.check_interval != first.run_control.check_interval:
+                raise ValueError('Every project of a cohort needs the same run_control.check_interval.')
         self.lib = lib = N.library()
         device = torch.device('cuda', torch.cuda.current_device())
         self.counter = torch.zeros(1, device=device, dtype=torch.long)
@@ -361,7 +363,7 @@ def run_tensor_batch(cases, *, objective=None, output_dir=None, keep_results=Tru
     """Run independent projects in batched HIP cohorts of 1--64 cases (tensor_batch.py:44-113).
 
     Unlike the reference, the projects of a cohort may differ in shape, CPML layout,
-    sources and monitors; they must share steps and precision.  `launch` selects the
+    sources and monitors; they must share steps, precision and check interval.  `launch` selects the
     kernels ('auto', a Launch, or the report of tune_tensor_batch).
     """
     validate_graph_steps(cuda_graph_steps, cuda_graph)
@@ -417,6 +419,14 @@ def _estimate_bytes(project):
     return cells*real*(6+eps+6+6)
 
 
+def _fixed_steps(project):
+    """The project with auto-shutoff off: a cohort runs every step, and each grid's rule keeps
+    judging divergence (and recording its history) to the end."""
+    fixed = project.model_copy(deep=True)
+    fixed.region.run_control.auto_shutoff = False
+    return fixed
+
+
 def _run(cases, projects, launch, objective, root, keep_results, memory_fraction, cuda_graph, cancel, progress,
          cuda_graph_steps, offset, total_cases):
     started, wall_started = time.perf_counter(), time.time()
@@ -427,9 +437,10 @@ def _run(cases, projects, launch, objective, root, keep_results, memory_fraction
     cohort = Cohort(projects, launch, epsilons=[c.epsilon for c in cases])
     first = projects[0].region
     checked = [p.region.run_control.divergence_check for p in projects]
-    norms = StateNorms(cohort.grids) if any(checked) else None
-    decisions = [DecayDecision(p.region.run_control, source_end_time(p), run.grid.time_step)
-                 for p, run in zip(projects, cohort.runs)]
+    # Every grid of the cohort is measured and judged on the device in one set of launches per
+    # diagnostic barrier; the cohort runs every step (no auto-shutoff, as tensor_batch.py:247-262).
+    monitor = (RunMonitor(cohort.grids, [_fixed_steps(p) for p in projects], check_count(projects[0]),
+                          labels=[f'Case {c.id}, ' for c in cases]) if any(checked) else None)
     graph = cohort.graphs(cuda_graph_steps) if cuda_graph else None
     views = [DisplaySlice(p) for p in projects]
     frames, frame_steps = [[] for _ in cases], [[] for _ in cases]
@@ -451,8 +462,9 @@ def _run(cases, projects, launch, objective, root, keep_results, memory_fraction
     torch.cuda.synchronize()
     setup_seconds = time.perf_counter()-started
     loop_start = time.perf_counter()
-    completed, cancelled, replays = 0, False, 0
+    completed, cancelled, replays, eager_steps = 0, False, 0, 0
     diagnostic_seconds = 0.
+    check = first.run_control.check_interval
     for target, advance in observation_schedule(first.steps, width, barriers):
         if cancel is not None and cancel.is_set():
             cancelled = True
@@ -462,33 +474,31 @@ def _run(cases, projects, launch, objective, root, keep_results, memory_fraction
             replays += 1
         else:
             cohort.advance(advance)
+            eager_steps += advance
         completed = target
         due = events.get(completed, ())
-        measured = None
-        if norms is not None and any(checked[i] and (completed % projects[i].region.run_control.check_interval == 0
-                                                     or completed == projects[i].region.steps) for i in due):
+        if monitor is not None and (completed % check == 0 or completed == first.steps):
             tick = time.perf_counter()
-            measured = norms.values()    # every grid of the cohort in one pair of launches
+            monitor.check(completed)    # every grid of the cohort, enqueued; verdicts read as they arrive
+            monitor.poll()
             diagnostic_seconds += time.perf_counter()-tick
         for i in due:
             r = projects[i].region
-            if checked[i] and (completed % r.run_control.check_interval == 0 or completed == r.steps):
-                energy, peak, bad = measured[i]
-                if bad:
-                    raise FloatingPointError(f'Case {cases[i].id}, diagnostic step {completed}: '
-                                             'Non-finite full-domain or CPML state detected.')
-                decisions[i].update(completed, float(energy), float(peak))
             if completed % intervals[i] == 0 or completed == r.steps:
-                image = views[i].frame(cohort.grids[i])
-                if not np.isfinite(image).all():
-                    raise FloatingPointError(f'Case {cases[i].id}: non-finite displayed field.')
-                frames[i].append(image)
+                frames[i].append(views[i].device_frame(cohort.grids[i]))
                 frame_steps[i].append(completed)
         if progress and (completed % max(1, first.steps//100) == 0 or completed == first.steps):
             progress(dict(step=completed, total=first.steps, cases=len(cases), cohort_offset=offset,
                           total_cases=total_cases, elapsed=time.perf_counter()-loop_start))
     torch.cuda.synchronize()
     seconds = time.perf_counter()-loop_start
+    if monitor is not None:
+        monitor.finish()
+    for i, case in enumerate(cases):
+        frames[i] = np.array([f.cpu().numpy() for f in frames[i]])
+        if not np.isfinite(frames[i]).all():
+            raise FloatingPointError(f'Case {case.id}: non-finite displayed field.')
+    kernel_steps = dict(graph.kernel_steps) if graph is not None else dict(two_step=0, one_pass=0, split=eager_steps)
     if root:
         root.mkdir(parents=True, exist_ok=True)
     items = []
@@ -501,13 +511,14 @@ def _run(cases, projects, launch, objective, root, keep_results, memory_fraction
         summary = run_summary(p, run, e, completed, termination, seconds, setup_seconds,
                               cuda_graph=graph is not None, cuda_graph_steps=width, cuda_graph_replays=replays,
                               cuda_kernel='batched_'+launch.kernel, launch=launch.as_dict(), batch_size=len(cases),
-                              batch_index=i, timing_scope='whole_cohort', diagnostics=decisions[i].history,
+                              batch_index=i, timing_scope='whole_cohort', kernel_steps=kernel_steps,
+                              diagnostics=monitor.history(i) if monitor is not None and checked[i] else [],
                               diagnostic_seconds=diagnostic_seconds, diagnostic_scope='whole_cohort',
                               diagnostic_backend='hip' if checked[i] else None,
                               slice_index=views[i].index)
-        result = Result(p, summary, np.array(frames[i]), np.array(frame_steps[i]), views[i].epsilon(run.eps),
+        result = Result(p, summary, frames[i], np.array(frame_steps[i]), views[i].epsilon(run.eps),
                         cohort.traces_of(i, completed), np.arange(1, completed+1)*g.time_step, e, h,
-                        [m.result() for m in run.planes])
+                        [dict(m.result(), run_signature=run_signature(p, completed)) for m in run.planes])
         item = BatchItem(case.id, 'cancelled' if cancelled else 'completed', case.parameters, summary=summary,
                          pid=os.getpid(), device=str(g.device), started=wall_started)
         if root:
@@ -541,8 +552,9 @@ def _run(cases, projects, launch, objective, root, keep_results, memory_fraction
 def plan_grouped_batch(cases, *, cohort_size=64, memory_limit_bytes=None):
     """Cohorts of at most `cohort_size` cases and `memory_limit_bytes` estimated device bytes.
 
-    Any mix of shapes may share a cohort; cases are grouped by (steps, precision), the only
-    common properties a cohort needs, largest first so cohorts fill evenly.
+    Any mix of shapes may share a cohort; cases are grouped by (steps, precision, diagnostic
+    check interval), the only common properties a cohort needs, largest first so cohorts fill
+    evenly.
     """
     cases = _cases(cases)
     if isinstance(cohort_size, bool) or not isinstance(cohort_size, int) or not 1 <= cohort_size <= 64:
@@ -550,7 +562,7 @@ def plan_grouped_batch(cases, *, cohort_size=64, memory_limit_bytes=None):
     groups = defaultdict(list)
     for index, case in enumerate(cases):
         r = case.project.region
-        groups[(r.steps, r.precision)].append(index)
+        groups[(r.steps, r.precision, r.run_control.check_interval)].append(index)
     cohorts = []
     for key, members in groups.items():
         members.sort(key=lambda i: -_estimate_bytes(cases[i].project))

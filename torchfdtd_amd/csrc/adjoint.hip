@@ -193,9 +193,12 @@ struct Record {
     real* record;
     long long modulus;
     const long long* counter;
+    int cpml;                     // 1: apply the E-family slabs of ef (psi_in = memory before the step)
+    Family<real> ef;
 };
 
-// curl_H of the E update (backward differences, terms in curl order) on the design box.
+// curl_H of the E update (backward differences, terms in curl order) on the design box; in an
+// E-family slab row a term is d inv_k + (b psi + c d), the value the update adds there.
 template <typename real>
 __global__ __launch_bounds__(256) void record_kernel(const Record<real> a) {
     const int bx = a.box_hi[0] - a.box_lo[0], by = a.box_hi[1] - a.box_lo[1], bz = a.box_hi[2] - a.box_lo[2];
@@ -216,6 +219,17 @@ __global__ __launch_bounds__(256) void record_kernel(const Record<real> a) {
         if (p[ax] >= 1) d = a.H[3 * j + c] - a.H[3 * (j - stride) + c];
         else if (a.periodic >> ax & 1) d = a.H[3 * j + c] - a.H[3 * (j + (long long)(na - 1) * stride) + c];
         else continue;
+        if (a.cpml) {
+            const Hit h = find_hit(a.ef, ax, p[ax]);
+            if (h.face >= 0) {
+                const unsigned m = ((t & 1) ? h.psi1 : h.psi0) + slab_offset(a.n, ax, h.q, h.w, x, y, z);
+                const real b = a.ef.coef[h.coef + h.q];
+                const real cc = a.ef.coef[h.coef + h.w + h.q];
+                const real ik = a.ef.coef[h.coef + 2 * h.w + h.q];
+                const real psi = b * a.ef.psi_in[m] + cc * d;
+                d = d * ik + psi;
+            }
+        }
         curl[o] = curl[o] + real(s) * d;
     }
     real* out = a.record + ((*a.counter % a.modulus) * count + b) * 3;
@@ -255,7 +269,7 @@ extern "C" int tfdtd_record_curl(const tfdtd_curl_record* d, void* stream) {
     const long long count = (long long)(d->box_hi[0] - d->box_lo[0]) * (d->box_hi[1] - d->box_lo[1]) * (d->box_hi[2] - d->box_lo[2]);
     const unsigned blocks = (unsigned)((count + 255) / 256);
     hipStream_t s = static_cast<hipStream_t>(stream);
-    auto fill = [&](auto* a) {
+    auto fill = [&](auto* a) -> int {
         using real = std::remove_pointer_t<decltype(a->record)>;
         for (int ax = 0; ax < 3; ++ax) { a->n[ax] = n[ax]; a->box_lo[ax] = d->box_lo[ax]; a->box_hi[ax] = d->box_hi[ax]; }
         a->H = static_cast<const real*>(d->H);
@@ -263,14 +277,21 @@ extern "C" int tfdtd_record_curl(const tfdtd_curl_record* d, void* stream) {
         a->record = static_cast<real*>(d->record);
         a->modulus = d->modulus;
         a->counter = reinterpret_cast<const long long*>(d->counter);
+        a->cpml = d->e.psi_in != nullptr;
+        if (!a->cpml) {
+            std::memset(&a->ef, 0, sizeof a->ef);
+            return TFDTD_OK;
+        }
+        // The record reads psi_in only: in-place (split step) and double-buffered layouts both serve.
+        return make_family(d->e, d->e.psi_in != d->e.psi_out, n, &a->ef);
     };
     if (d->dtype == TFDTD_F32) {
         Record<float> a;
-        fill(&a);
+        if ((status = fill(&a))) return status;
         hipLaunchKernelGGL(record_kernel<float>, dim3(blocks), dim3(256), 0, s, a);
     } else if (d->dtype == TFDTD_F64) {
         Record<double> a;
-        fill(&a);
+        if ((status = fill(&a))) return status;
         hipLaunchKernelGGL(record_kernel<double>, dim3(blocks), dim3(256), 0, s, a);
     } else {
         return tfdtd_fail(TFDTD_EINVAL, "record: dtype must be float32 or float64");
@@ -306,12 +327,13 @@ static int launch_adjoint(const tfdtd_adjoint* d, hipStream_t s) {
         if (status) return status;
         if (!d->gradient || d->record_modulus < 1 || (d->gradient_stride != 1 && d->gradient_stride != 3))
             return tfdtd_fail(TFDTD_EINVAL, "adjoint: a record needs a gradient buffer, a modulus >= 1 and gradient_stride 1 or 3");
-        // The recorded curl is the plain difference curl: the box may not meet an E-family slab.
-        for (int ax = 0; ax < 3; ++ax)
+        // A plain difference record serves only boxes that stay out of the E-family slabs.
+        for (int ax = 0; ax < 3 && !d->record_cpml; ++ax)
             for (int f = 0; f < 2; ++f) {
                 const Face& face = a.ef.face[ax][f];
                 if (face.hi > face.lo && face.lo < d->box_hi[ax] && d->box_lo[ax] < face.hi)
-                    return tfdtd_fail(TFDTD_EINVAL, "adjoint: the design box meets the CPML slab of axis %d", ax);
+                    return tfdtd_fail(TFDTD_EINVAL, "adjoint: the design box meets the CPML slab of axis %d and the "
+                                                    "record has no CPML terms", ax);
             }
         for (int ax = 0; ax < 3; ++ax) { a.box_lo[ax] = d->box_lo[ax]; a.box_hi[ax] = d->box_hi[ax]; }
         a.record = static_cast<const real*>(d->record);

@@ -1320,16 +1320,23 @@ int launch_tile(const tfdtd_march* d, hipStream_t stream) {
 
 // The five LDS planes (51 KB) allow three workgroups per CU.  More wavefronts per workgroup hold fewer
 // rows each, hence fewer registers: the boundary tiles carry the CPML memory of both stages of every
-// axis in registers.
+// axis in registers.  TFDTD_TILE2_BOUNDARY_WAVES sets the waves per SIMD the boundary variants
+// (CPML 6 and 7, which compile in the y and z slabs) are held to; the other variants use 6 at W 12.
+#ifndef TFDTD_TILE2_BOUNDARY_WAVES
+#define TFDTD_TILE2_BOUNDARY_WAVES 6
+#endif
+#define TFDTD_TILE2_WAVES(CPML, W) \
+    ((CPML) == 6 || (CPML) == 7 ? TFDTD_TILE2_BOUNDARY_WAVES : (W) == 16 ? 8 : (W) == 12 ? 6 : 1)
+
 template <bool EPS3, int CPML, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 16 ? 8 : W == 12 ? 6 : 1, 8))) void tile2_kernel(const March<float> a) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TFDTD_TILE2_WAVES(CPML, W), 8))) void tile2_kernel(const March<float> a) {
     tile2_body<float, EPS3, CPML, W, W == 16 ? 16 : 12>(a, blockIdx.x, gridDim.x);
 }
 
 // Many grids (or one) from a device table, as batch_tile_kernel: the tile parameters are loaded on
 // demand instead of occupying the kernel-argument registers.
 template <bool EPS3, int CPML, int W>
-__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(W == 16 ? 8 : W == 12 ? 6 : 1, 8))) void batch_tile2_kernel(
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TFDTD_TILE2_WAVES(CPML, W), 8))) void batch_tile2_kernel(
     const March<float>* __restrict__ table, const int* __restrict__ prefix, int grids) {
     const int b = blockIdx.x;
     int lo = 0, hi = grids;

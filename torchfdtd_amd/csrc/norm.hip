@@ -102,6 +102,61 @@ __global__ __launch_bounds__(kThreads) void norm_final(const NormGrid* __restric
     block_reduce(sum, peak, bad, out + 3LL * blockIdx.x);
 }
 
+// The run-control rule of one grid at one barrier (one lane per grid).  Every expression is
+// the FP64 operation a host evaluation makes, so the verdicts and history equal a host rule's.
+__global__ __launch_bounds__(64) void decay_update(const tfdtd_decay_rule* __restrict__ rules,
+                                                   tfdtd_decay_state* __restrict__ states,
+                                                   const double* __restrict__ norms, int count, long long step,
+                                                   double* __restrict__ history, int rows) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= count) return;
+    const tfdtd_decay_rule rule = rules[g];
+    tfdtd_decay_state s = states[g];
+    if (s.verdict != TFDTD_RUN_GOING) return;
+    const double energy = norms[3LL * g], field_peak = norms[3LL * g + 1], flagged = norms[3LL * g + 2];
+    const double when = (double)step * rule.time_step;
+    if (flagged != 0. || !isfinite(energy) || !isfinite(field_peak)) {
+        s.verdict = TFDTD_RUN_NONFINITE;
+        s.verdict_step = step;
+        s.verdict_value = isfinite(energy) && isfinite(field_peak) ? 1. : 0.;   // 1: only the CPML memory
+    } else if (rule.divergence_check && rule.field_limit > 0. && field_peak > rule.field_limit) {
+        s.verdict = TFDTD_RUN_FIELD_LIMIT;
+        s.verdict_step = step;
+        s.verdict_value = field_peak;
+    } else {
+        const bool eligible = step >= rule.min_steps && when >= rule.eligible_time;
+        s.peak_energy = fmax(s.peak_energy, energy);
+        if (eligible && !s.frozen) {
+            s.frozen = 1;
+            s.baseline = s.peak_energy;
+        }
+        const double base = s.frozen ? s.baseline : s.peak_energy;
+        const double ratio = base != 0. ? energy / base : 0.;
+        if (rule.divergence_check && eligible && ratio > rule.growth_limit) {
+            s.verdict = TFDTD_RUN_GROWTH;
+            s.verdict_step = step;
+            s.verdict_value = ratio;
+        } else {
+            s.below = eligible && ratio <= rule.decay_threshold ? s.below + 1 : 0;
+            if (s.checks < rows) {
+                double* row = history + 6LL * ((long long)g * rows + s.checks);
+                row[0] = (double)step;
+                row[1] = when;
+                row[2] = energy;
+                row[3] = ratio;
+                row[4] = field_peak;
+                row[5] = eligible ? 1. : 0.;
+            }
+            s.checks += 1;
+            if (rule.auto_shutoff && s.below >= rule.consecutive_checks) {
+                s.verdict = TFDTD_RUN_DECAYED;
+                s.verdict_step = step;
+            }
+        }
+    }
+    states[g] = s;
+}
+
 long long tasks_of(long long n) { return (n + kTask - 1) / kTask; }
 
 long long grid_tasks(const tfdtd_norm_grid& d) {
@@ -180,6 +235,15 @@ int tfdtd_state_norm(const void* device_table, int32_t count, int64_t tasks, int
     if (status) return status;
     hipLaunchKernelGGL(norm_final, dim3((unsigned)count), dim3(kThreads), 0, s, grids, partial, out);
     return tfdtd_check_launch("norm_final");
+}
+
+int tfdtd_decay_update(const tfdtd_decay_rule* rules, tfdtd_decay_state* states, const double* norms, int32_t count,
+                       int64_t step, double* history, int32_t rows_per_grid, void* stream) {
+    if (!rules || !states || !norms || count < 1 || step < 0 || rows_per_grid < 0 || (rows_per_grid > 0 && !history))
+        return tfdtd_fail(TFDTD_EINVAL, "decay: bad arguments");
+    hipLaunchKernelGGL(decay_update, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, static_cast<hipStream_t>(stream),
+                       rules, states, norms, count, (long long)step, history, rows_per_grid);
+    return tfdtd_check_launch("decay_update");
 }
 
 }  // extern "C"

@@ -6,13 +6,14 @@
   box and of the base permittivity outside it (torchfdtd/density_layer.py:75-126).
 * ``periodic_density_layer``: the same over a periodic xy cell (density_layer.py:26-61).
 * ``DensityParameterization``: trainable xy density -- sigmoid logits or clamped densities,
-  fixed pixels, normalised conic filter (1 - d/r)+ with periodic or truncated edges, symmetry
-  orbit averaging and the tanh projection with (beta, eta)
+  fixed pixels, the normalised conic filter (1 - d/r)+ with periodic or truncated edges (an FFT
+  correlation), the mean over a symmetry group's images and the tanh projection with (beta, eta)
   (torchfdtd/design_parameterization.py:25-215).
 * ``cell_density_map``: rho on a box of cells, eps = eps_bg + (eps_design - eps_bg) rho, the
   per-cell map of the benchmark's 50x50x6 design region.
 
-The overlap and projection formulas match the reference's because they define the map.
+The box-overlap integrals, the conic kernel and the projection formula match the reference's
+because they define the maps; the rest is this module's own construction.
 """
 from __future__ import annotations
 
@@ -21,7 +22,6 @@ import math
 import numpy as np
 import torch
 from torch import nn
-from torch.nn import functional as F
 
 from .planning import field_axes
 
@@ -43,37 +43,50 @@ def _z_fraction(region, component, bottom, top, like):
     return (torch.minimum(z+width/2, z.new_tensor(top))-torch.maximum(z-width/2, z.new_tensor(bottom))).clamp_min(0)/width
 
 
-def _check_density(density):
-    if not isinstance(density, torch.Tensor) or density.dtype not in (torch.float32, torch.float64) or density.ndim != 2 \
-            or min(density.shape) < 1:
-        raise ValueError('Density must be a nonempty real floating x,y tensor.')
-    if not bool(torch.isfinite(density).all()) or bool(((density < 0) | (density > 1)).any()):
-        raise ValueError('Density must be finite in [0,1].')
+class _Checks:
+    """Argument checks of the density maps: each failed requirement raises ValueError with its text."""
 
+    @staticmethod
+    def require(condition, text):
+        if not condition:
+            raise ValueError(text)
 
-def _check_yee_region(region):
-    if region.dimension != '3d' or region.mesh_type != 'uniform' or region.material_sampling != 'yee':
-        raise ValueError('Density transfer requires a uniform 3D Yee-material grid.')
+    @classmethod
+    def scalars(cls, *values):
+        cls.require(all(not isinstance(v, torch.Tensor) and isinstance(v, (int, float)) and math.isfinite(v) for v in values),
+                    'geometry and material values of a density map are fixed finite numbers')
+
+    @classmethod
+    def density(cls, density):
+        cls.require(isinstance(density, torch.Tensor) and density.dtype in (torch.float32, torch.float64)
+                    and density.ndim == 2 and density.numel() > 0, 'density is a non-empty real (x, y) float tensor')
+        cls.require(bool(torch.isfinite(density).all()) and not bool(((density < 0) | (density > 1)).any()),
+                    'density values lie in [0, 1]')
+
+    @classmethod
+    def yee_region(cls, region):
+        cls.require(region.dimension == '3d' and region.mesh_type == 'uniform' and region.material_sampling == 'yee',
+                    'density layers need a uniform 3D region with Yee material sampling')
 
 
 def bounded_density_layer(density, base_epsilon, region, *, bounds_um, bottom_um, top_um,
                           background_epsilon, design_epsilon):
     """Replace one box of `base_epsilon` (Nx,Ny,Nz,3) by the extruded density layer."""
-    _check_density(density)
-    if not isinstance(base_epsilon, torch.Tensor) or tuple(base_epsilon.shape) != tuple(region.shape)+(3,) \
-            or base_epsilon.dtype != density.dtype:
-        raise ValueError('base_epsilon must be the explicit Yee diagonal epsilon of the region in the density precision.')
-    _check_yee_region(region)
-    values = (*bounds_um, bottom_um, top_um, background_epsilon, design_epsilon)
-    if len(bounds_um) != 4 or any(isinstance(v, torch.Tensor) or not math.isfinite(v) for v in values):
-        raise ValueError('Box bounds, layer interfaces and material values must be fixed finite scalars.')
+    check = _Checks
+    check.density(density)
+    check.yee_region(region)
+    check.require(isinstance(base_epsilon, torch.Tensor) and base_epsilon.dtype == density.dtype
+                  and tuple(base_epsilon.shape) == tuple(region.shape)+(3,),
+                  'base_epsilon is the (Nx, Ny, Nz, 3) Yee permittivity of the region, in the density dtype')
+    check.require(len(bounds_um) == 4, 'bounds_um is (x0, x1, y0, y1)')
+    check.scalars(*bounds_um, bottom_um, top_um, background_epsilon, design_epsilon)
     x0, x1, y0, y1 = bounds_um
-    if x0 >= x1 or y0 >= y1 or bottom_um >= top_um or min(background_epsilon, design_epsilon) < 1:
-        raise ValueError('Box spans and layer thickness must be positive and permittivities at least one.')
-    for axis, (lo, hi) in enumerate(((x0, x1), (y0, y1), (bottom_um, top_um))):
-        inner = region.interior_bounds(axis)
-        if lo < inner[0]-1e-9 or hi > inner[1]+1e-9:
-            raise ValueError('The design box must lie inside the non-PML region.')
+    spans = ((x0, x1), (y0, y1), (bottom_um, top_um))
+    check.require(all(lo < hi for lo, hi in spans), 'the box and the layer have positive extent')
+    check.require(min(background_epsilon, design_epsilon) >= 1, 'permittivities are at least 1')
+    for axis, (lo, hi) in enumerate(spans):
+        inner_lo, inner_hi = region.interior_bounds(axis)
+        check.require(inner_lo-1e-9 <= lo and hi <= inner_hi+1e-9, 'the design box stays out of the CPML')
     step = region.axis_steps
     z_periodic = all(face.kind == 'periodic' for face in region.boundaries.pair(2))
     inner_z = region.interior_bounds(2)
@@ -95,20 +108,17 @@ def bounded_density_layer(density, base_epsilon, region, *, bounds_um, bottom_um
 def periodic_density_layer(density, region, *, bottom_um, top_um, background_epsilon, design_epsilon,
                            pixel_origin='cell_edges'):
     """Density pixels tiling a periodic xy cell, extruded over [bottom, top] (Nx,Ny,Nz,3)."""
-    _check_density(density)
-    if pixel_origin not in ('cell_edges', 'sample_centers'):
-        raise ValueError('Unknown density pixel origin.')
-    _check_yee_region(region)
-    if any(face.kind != 'periodic' for axis in (0, 1) for face in region.boundaries.pair(axis)):
-        raise ValueError('Density transfer requires periodic x and y boundaries.')
-    values = (bottom_um, top_um, background_epsilon, design_epsilon)
-    if any(isinstance(v, torch.Tensor) or not math.isfinite(v) for v in values):
-        raise ValueError('Layer interfaces and material values must be fixed finite scalars.')
-    if bottom_um >= top_um or min(background_epsilon, design_epsilon) < 1:
-        raise ValueError('Layer thickness must be positive and permittivities at least one.')
+    check = _Checks
+    check.density(density)
+    check.yee_region(region)
+    check.require(pixel_origin in ('cell_edges', 'sample_centers'), "pixel_origin is 'cell_edges' or 'sample_centers'")
+    check.require(all(face.kind == 'periodic' for axis in (0, 1) for face in region.boundaries.pair(axis)),
+                  'a periodic density layer needs periodic x and y faces')
+    check.scalars(bottom_um, top_um, background_epsilon, design_epsilon)
+    check.require(bottom_um < top_um, 'the layer has positive thickness')
+    check.require(min(background_epsilon, design_epsilon) >= 1, 'permittivities are at least 1')
     lower, upper = region.interior_bounds(2)
-    if bottom_um < lower or top_um > upper:
-        raise ValueError('Patterned layer must lie inside the non-PML region.')
+    check.require(lower <= bottom_um and top_um <= upper, 'the layer stays out of the CPML')
     step, period = region.axis_steps, region.actual_size
 
     def wrapped(centers, width, count, length):
@@ -138,117 +148,152 @@ def cell_density_map(density, base_epsilon, box, *, background_epsilon, design_e
     return eps
 
 
-_SYMMETRIES = ('none', 'mirror_x', 'mirror_y', 'mirror_xy', 'rotate180', 'rotate90', 'dihedral4')
+def _group_images(value, symmetry):
+    """The images of an (x, y) array under every element of a symmetry group."""
+    flips = {'none': ((),), 'mirror_x': ((), (0,)), 'mirror_y': ((), (1,)), 'mirror_xy': ((), (0,), (1,), (0, 1)),
+             'rotate180': ((), (0, 1))}
+    if symmetry in flips:
+        return [value.flip(axes) if axes else value for axes in flips[symmetry]]
+    turns = [torch.rot90(value, k, (0, 1)) for k in range(4)]
+    return turns if symmetry == 'rotate90' else turns+[t.flip(0) for t in turns]
 
 
-def _images(value, symmetry):
-    if symmetry == 'none':
-        return (value,)
-    if symmetry == 'mirror_x':
-        return value, value.flip(0)
-    if symmetry == 'mirror_y':
-        return value, value.flip(1)
-    if symmetry == 'mirror_xy':
-        return value, value.flip(0), value.flip(1), value.flip((0, 1))
-    if symmetry == 'rotate180':
-        return value, value.flip((0, 1))
-    turns = tuple(torch.rot90(value, k, (0, 1)) for k in range(4))
-    return turns if symmetry == 'rotate90' else turns+tuple(v.flip(0) for v in turns)
+SYMMETRIES = ('none', 'mirror_x', 'mirror_y', 'mirror_xy', 'rotate180', 'rotate90', 'dihedral4')
+
+
+class ConicFilter(nn.Module):
+    """Normalised conic filter w(d) = (1 - d/r)+ / sum w on an (x, y) pixel grid of given spacing
+    (design_parameterization.py:100-115), applied as an FFT correlation.
+
+    'periodic': each torus offset counts once (minimum image; for an even count the half-period
+    neighbour counts once).  'truncate': the grid is zero padded and each output divided by the
+    filtered grid of ones, so pixels outside the domain drop out of the average.  Radius 0 keeps
+    the density unchanged."""
+
+    def __init__(self, shape, spacing, radius, boundary, dtype, device):
+        super().__init__()
+        self.shape, self.boundary = tuple(shape), boundary
+        self.identity = radius == 0
+        if self.identity:
+            return
+        axes = []
+        for n, h in zip(self.shape, spacing):
+            reach = min(n-1, math.ceil(radius/h))
+            lo, hi = (-min(n//2, reach), min((n-1)//2, reach)) if boundary == 'periodic' else (-reach, reach)
+            axes.append(torch.arange(lo, hi+1, dtype=torch.float64))
+        offsets = [a*h for a, h in zip(axes, spacing)]
+        weight = (1-torch.sqrt(offsets[0][:, None]**2+offsets[1][None, :]**2)/radius).clamp_min(0)
+        weight = weight/weight.sum()
+        if boundary == 'periodic':
+            self.padded = self.shape
+            torus = torch.zeros(self.shape, dtype=torch.float64)
+            torus[(axes[0].long() % self.shape[0])[:, None], (axes[1].long() % self.shape[1])[None, :]] = weight
+        else:
+            self.reach = [int(a[-1]) for a in axes]
+            self.padded = tuple(n+2*r for n, r in zip(self.shape, self.reach))
+            torus = torch.zeros(self.padded, dtype=torch.float64)
+            torus[(axes[0].long() % self.padded[0])[:, None], (axes[1].long() % self.padded[1])[None, :]] = weight
+        # Correlation with w equals multiplication by the conjugate spectrum of w on the torus.
+        self.register_buffer('spectrum', torch.fft.rfft2(torus).conj().to(device))
+        self.register_buffer('weight', weight.to(dtype=dtype, device=device))
+        if boundary == 'truncate':
+            self.register_buffer('coverage', self._correlate(torch.ones(self.shape, dtype=torch.float64,
+                                                                        device=device)).to(dtype))
+
+    def _correlate(self, values):
+        if self.boundary == 'truncate':
+            values = torch.nn.functional.pad(values, (0, 2*self.reach[1], 0, 2*self.reach[0]))
+            values = values.roll((self.reach[0], self.reach[1]), (0, 1))
+        spectrum = self.spectrum if values.dtype == torch.float64 else self.spectrum.to(torch.complex64)
+        out = torch.fft.irfft2(torch.fft.rfft2(values)*spectrum, s=self.padded)
+        if self.boundary == 'truncate':
+            out = out[self.reach[0]:self.reach[0]+self.shape[0], self.reach[1]:self.reach[1]+self.shape[1]]
+        return out
+
+    def forward(self, density):
+        if self.identity:
+            return density
+        out = self._correlate(density)
+        return out/self.coverage if self.boundary == 'truncate' else out
 
 
 class DensityParameterization(nn.Module):
-    """Trainable xy density: raw parameters -> bounded density (design_parameterization.py:25-215)."""
+    """Trainable (x, y) density for inverse design (design_parameterization.py:25-215):
+
+        raw --bound--> [0,1] --pin fixed--> --conic filter--> clip [0,1] --group mean-->
+            --tanh projection (beta, eta)--> --pin fixed--> density  [--> hard 0/1]
+
+    `mode` 'logits' bounds by a sigmoid, 'density' by a clamp (zero derivative outside [0,1]).
+    The group mean averages the density over the images of the symmetry group, i.e. over each
+    pixel's orbit.  beta = 0 skips the projection.  `hard` thresholds at 1/2; with
+    `straight_through` the forward value is the threshold and the derivative the smooth map's."""
 
     def __init__(self, shape, *, spacing_um, initial=None, mode='logits', filter_radius_um=0., boundary='periodic',
                  symmetry='none', beta=1., eta=.5, fixed_mask=None, fixed_values=0., dtype=torch.float32, device=None):
         super().__init__()
+        need = _Checks.require
         shape = tuple(shape)
-        if len(shape) != 2 or any(isinstance(n, bool) or not isinstance(n, int) or n < 1 for n in shape):
-            raise ValueError('shape must contain two positive integer xy counts.')
-        spacing = (float(spacing_um),)*2 if isinstance(spacing_um, (int, float)) else tuple(float(v) for v in spacing_um)
-        if len(spacing) != 2 or any(not math.isfinite(v) or v <= 0 for v in spacing):
-            raise ValueError('spacing_um must contain two fixed positive finite lengths.')
-        if mode not in ('logits', 'density') or boundary not in ('periodic', 'truncate'):
-            raise ValueError('Use logits/density mode and periodic/truncate boundary.')
-        if symmetry not in _SYMMETRIES:
-            raise ValueError('Unknown reflection or rotation symmetry.')
-        if symmetry in ('rotate90', 'dihedral4') and (shape[0] != shape[1] or spacing[0] != spacing[1]):
-            raise ValueError('Quarter-turn symmetry requires a square grid with equal physical spacing.')
-        if not math.isfinite(filter_radius_um) or filter_radius_um < 0:
-            raise ValueError('filter_radius_um must be a fixed finite nonnegative length.')
+        need(len(shape) == 2 and all(type(n) is int and n > 0 for n in shape), 'shape is two positive integers (nx, ny)')
+        spacing = (spacing_um, spacing_um) if isinstance(spacing_um, (int, float)) else tuple(spacing_um)
+        need(len(spacing) == 2, 'spacing_um is one length or (dx, dy)')
+        _Checks.scalars(*spacing, filter_radius_um)
+        spacing = tuple(float(v) for v in spacing)
+        need(min(spacing) > 0 and filter_radius_um >= 0, 'spacings are positive and the filter radius non-negative')
+        need(mode in ('logits', 'density'), "mode is 'logits' or 'density'")
+        need(boundary in ('periodic', 'truncate'), "boundary is 'periodic' or 'truncate'")
+        need(symmetry in SYMMETRIES, f'symmetry is one of {SYMMETRIES}')
+        need(symmetry not in ('rotate90', 'dihedral4') or (shape[0] == shape[1] and spacing[0] == spacing[1]),
+             'quarter turns need a square grid of equal spacings')
         self._check_projection(beta, eta)
-        self.shape, self.mode, self.boundary, self.symmetry = shape, mode, boundary, symmetry
-        where = device if device is not None else initial.device if isinstance(initial, torch.Tensor) else 'cpu'
-        value = torch.as_tensor((0. if mode == 'logits' else .5) if initial is None else initial, dtype=dtype, device=where)
-        if value.ndim == 0:
-            value = value.expand(shape)
-        if tuple(value.shape) != shape or not bool(torch.isfinite(value).all()):
-            raise ValueError('initial must be finite, scalar or shaped like the xy grid.')
-        if mode == 'density' and bool(((value < 0) | (value > 1)).any()):
-            raise ValueError('Initial density must lie in [0,1].')
-        self.design = nn.Parameter(value.detach().clone())
-        mask = torch.zeros(shape, dtype=torch.bool, device=where) if fixed_mask is None else torch.as_tensor(fixed_mask, device=where)
-        fixed = torch.as_tensor(fixed_values, dtype=dtype, device=where)
-        fixed = fixed.expand(shape) if fixed.ndim == 0 else fixed
-        if mask.dtype != torch.bool or tuple(mask.shape) != shape or tuple(fixed.shape) != shape \
-                or bool(((fixed < 0) | (fixed > 1)).any()):
-            raise ValueError('fixed_mask must be a boolean xy array and fixed_values lie in [0,1].')
-        active = torch.where(mask, fixed, torch.zeros_like(fixed))
-        if any(not torch.equal(mask, m) for m in _images(mask, symmetry)) or \
-                any(not torch.equal(active, a) for a in _images(active, symmetry)):
-            raise ValueError('Fixed masks and their values must respect the requested symmetry.')
+        self.shape, self.mode, self.symmetry = shape, mode, symmetry
+        if device is None:
+            device = initial.device if isinstance(initial, torch.Tensor) else 'cpu'
+        start = torch.as_tensor((0. if mode == 'logits' else .5) if initial is None else initial, dtype=dtype, device=device)
+        start = start.expand(shape) if start.ndim == 0 else start
+        need(tuple(start.shape) == shape and bool(torch.isfinite(start).all()), 'initial is finite: a scalar or an (nx, ny) array')
+        need(mode == 'logits' or not bool(((start < 0) | (start > 1)).any()), 'an initial density lies in [0, 1]')
+        self.design = nn.Parameter(start.detach().clone())
+        mask = torch.zeros(shape, dtype=torch.bool, device=device) if fixed_mask is None else torch.as_tensor(fixed_mask, device=device)
+        pinned = torch.as_tensor(fixed_values, dtype=dtype, device=device)
+        pinned = pinned.expand(shape) if pinned.ndim == 0 else pinned
+        need(mask.dtype == torch.bool and tuple(mask.shape) == shape, 'fixed_mask is a boolean (nx, ny) array')
+        need(tuple(pinned.shape) == shape and not bool(((pinned < 0) | (pinned > 1)).any()), 'fixed_values lie in [0, 1]')
+        held = pinned*mask
+        need(all(torch.equal(image, mask) for image in _group_images(mask, symmetry))
+             and all(torch.equal(image, held) for image in _group_images(held, symmetry)),
+             'the fixed pixels and their values are invariant under the symmetry')
         self.register_buffer('fixed_mask', mask.clone())
-        self.register_buffer('fixed_values', fixed.clone())
+        self.register_buffer('fixed_values', pinned.clone())
         self.register_buffer('beta', self.design.new_tensor(float(beta)))
         self.register_buffer('eta', self.design.new_tensor(float(eta)))
-        # Symmetry orbits: every pixel averages over the images of its position.
-        index = torch.arange(math.prod(shape), device=where).reshape(shape)
-        representative = torch.stack(_images(index, symmetry)).amin(0).reshape(-1)
-        _, orbit = torch.unique(representative, sorted=True, return_inverse=True)
-        self.register_buffer('_orbit', orbit)
-        self.register_buffer('_orbit_size', torch.bincount(orbit).to(dtype))
-        # Conic kernel (1 - d/r)+ normalised to unit sum; periodic grids keep one image per offset.
-        radius = float(filter_radius_um)
-        ranges = []
-        for n, h in zip(shape, spacing):
-            reach = min(n-1, math.ceil(radius/h))
-            lo, hi = (-min(n//2, reach), min((n-1)//2, reach)) if boundary == 'periodic' else (-reach, reach)
-            ranges.append((lo, hi, torch.arange(lo, hi+1, dtype=dtype, device=where)*h))
-        d = (ranges[0][2][:, None]**2+ranges[1][2][None, :]**2).sqrt()
-        kernel = (1-d/radius).clamp_min(0) if radius else torch.ones_like(d)
-        self.register_buffer('_kernel', (kernel/kernel.sum())[None, None])
-        self._padding = (-ranges[1][0], ranges[1][1], -ranges[0][0], ranges[0][1])
-        ones = torch.ones((1, 1, *shape), dtype=dtype, device=where)
-        norm = F.conv2d(F.pad(ones, self._padding), self._kernel) if boundary == 'truncate' else ones.new_ones((1, 1, 1, 1))
-        self.register_buffer('_norm', norm)
+        self.filter = ConicFilter(shape, spacing, float(filter_radius_um), boundary, dtype, device)
 
     @staticmethod
     def _check_projection(beta, eta):
-        if any(isinstance(v, torch.Tensor) or not math.isfinite(v) for v in (beta, eta)) or beta < 0 or not 0 < eta < 1:
-            raise ValueError('beta must be nonnegative and eta strictly inside (0,1).')
+        _Checks.scalars(beta, eta)
+        _Checks.require(beta >= 0 and 0 < eta < 1, 'beta >= 0 and 0 < eta < 1')
 
     @torch.no_grad()
     def set_beta(self, value):
         self._check_projection(value, float(self.eta))
         self.beta.fill_(float(value))
 
-    def forward(self, *, hard=False, straight_through=False):
-        if straight_through and not hard:
-            raise ValueError('straight_through requires hard=True.')
-        density = self.design.sigmoid() if self.mode == 'logits' else self.design.clamp(0, 1)
-        density = torch.where(self.fixed_mask, self.fixed_values, density)
-        padded = F.pad(density[None, None], self._padding, mode='circular' if self.boundary == 'periodic' else 'constant')
-        filtered = (F.conv2d(padded, self._kernel)/self._norm)[0, 0].clamp(0, 1)
-        sums = filtered.new_zeros(self._orbit_size.shape).scatter_add(0, self._orbit, filtered.reshape(-1))
-        symmetric = (sums/self._orbit_size)[self._orbit].reshape(self.shape)
-        beta, eta = float(self.beta), self.eta
+    def project(self, density):
+        """(tanh(beta eta) + tanh(beta (rho - eta))) / (tanh(beta eta) + tanh(beta (1 - eta))), clipped to [0,1]."""
+        beta, eta = float(self.beta), float(self.eta)
         if beta == 0:
-            projected = symmetric
-        else:
-            low = torch.tanh(beta*eta)
-            projected = ((low+torch.tanh(beta*(symmetric-eta)))/(low+torch.tanh(beta*(1-eta)))).clamp(0, 1)
-        density = torch.where(self.fixed_mask, self.fixed_values, projected)
-        if hard:
-            binary = (density >= .5).to(density.dtype)
-            density = density+(binary-density).detach() if straight_through else binary
-        return density
+            return density
+        low, high = math.tanh(beta*eta), math.tanh(beta*(1-eta))
+        return ((low+torch.tanh(beta*(density-eta)))/(low+high)).clamp(0, 1)
+
+    def forward(self, *, hard=False, straight_through=False):
+        _Checks.require(hard or not straight_through, 'straight_through needs hard=True')
+        bounded = self.design.sigmoid() if self.mode == 'logits' else self.design.clamp(0, 1)
+        bounded = torch.where(self.fixed_mask, self.fixed_values, bounded)
+        filtered = self.filter(bounded).clamp(0, 1)
+        symmetric = torch.stack(_group_images(filtered, self.symmetry)).mean(0)
+        density = torch.where(self.fixed_mask, self.fixed_values, self.project(symmetric))
+        if not hard:
+            return density
+        binary = (density >= .5).to(density.dtype)
+        return density+(binary-density).detach() if straight_through else binary

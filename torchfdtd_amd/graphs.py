@@ -57,11 +57,14 @@ class StepGraphs:
         torch.cuda.current_stream().wait_stream(stream)
         torch.cuda.synchronize()
         self.graphs = {}
+        self.kernel_steps = dict(two_step=0, one_pass=0, split=0)     # steps replayed through each path
+        self.composition = {}
         counts = {1, self.width}
         if quad is not None and self.width > 4:
             counts.add(4)       # one four: replays that stop at any multiple of four stay on the quad path
         for count in sorted(counts):
             graph = torch.cuda.CUDAGraph()
+            made = dict(two_step=0, one_pass=0, split=0)
             with torch.cuda.stream(stream):
                 graph.capture_begin()
                 if pair is not None and count > 1:
@@ -69,15 +72,20 @@ class StepGraphs:
                     if quad is not None:
                         for _ in range(rest // 4):
                             quad()
+                        made['two_step'] = 4*(rest // 4)
                         rest %= 4
                     for _ in range(rest // 2):
                         pair()
+                    made['one_pass'] = 2*(rest // 2)
                     if count % 2:
                         step()
+                        made['split'] = 1
                 else:
                     for _ in range(count):
                         step()
+                    made['split'] = count
                 graph.capture_end()
+            self.composition[count] = made
             self.graphs[count] = graph
             for value in states:
                 value.zero_()
@@ -87,3 +95,5 @@ class StepGraphs:
         if self.prepare is not None:
             self.prepare()
         self.graphs[width].replay()
+        for path, steps in self.composition[width].items():
+            self.kernel_steps[path] += steps

@@ -279,9 +279,108 @@ class Structure(Item):
     def valid_shape(self):
         if any(s <= 0 for s in self.size):
             raise ValueError('Object spans must be positive.')
-        if self.kind == 'ring' and self.inner_radius >= self.radius:
-            raise ValueError('Ring inner radius must be less than its outer radius.')
+        if self.kind == 'ring':
+            if self.inner_radius >= self.radius:
+                raise ValueError('Ring inner radius must be less than its outer radius.')
+            if self.make_ellipsoid and (self.inner_radius_2 >= self.radius_2
+                                        or (self.inner_radius == 0) != (self.inner_radius_2 == 0)):
+                raise ValueError('An elliptic ring needs both inner radii zero, or both positive and below the outer radii.')
+            if not 0 < abs(self.theta_stop-self.theta_start) <= 360:
+                raise ValueError('A ring arc spans more than 0 and at most 360 degrees.')
+        if self.kind == 'polygon':
+            check_contour(self.vertices)
+            check_holes(self.vertices, self.holes)
+        elif self.holes:
+            raise ValueError('Holes belong to polygons only.')
         return self
+
+    @property
+    def angular_span(self):
+        return (self.theta_stop-self.theta_start) % 360 or 360.
+
+
+def _cross(a, b):
+    return a[..., 0]*b[..., 1]-a[..., 1]*b[..., 0]
+
+
+def _touching(p, q, eps, tol):
+    """[len(p), len(q)] booleans: segment p_i of contour p meets (crosses or touches) segment q_j of q.
+    Orientation tests carry `eps`, bounding boxes `tol`."""
+    import numpy as np
+    a, b = p[:, None], np.roll(p, -1, axis=0)[:, None]
+    c, d = q[None], np.roll(q, -1, axis=0)[None]
+    o1, o2 = _cross(b-a, c-a), _cross(b-a, d-a)
+    o3, o4 = _cross(d-c, a-c), _cross(d-c, b-c)
+    straddles = (((o1 <= eps) & (o2 >= -eps)) | ((o2 <= eps) & (o1 >= -eps))) & \
+                (((o3 <= eps) & (o4 >= -eps)) | ((o4 <= eps) & (o3 >= -eps)))
+    boxes = (np.maximum(np.minimum(a, b), np.minimum(c, d)) <= np.minimum(np.maximum(a, b), np.maximum(c, d))+tol).all(-1)
+    return straddles & boxes
+
+
+def _scale(*contours):
+    import numpy as np
+    extent = max(float(np.ptp(c, axis=0).max()) for c in contours)
+    tol = 64*2.220446049250313e-16*max(extent, 1e-300)
+    return tol, tol*extent
+
+
+def check_contour(vertices):
+    """One simple closed contour (either orientation): 3..2048 distinct consecutive vertices, nonzero
+    area, no edge folding back on its neighbour, no edge meeting a non-adjacent edge."""
+    import numpy as np
+    v = np.asarray(vertices, dtype=float)
+    if v.ndim != 2 or v.shape[1] != 2 or not 3 <= len(v) <= 2048 or not np.isfinite(v).all():
+        raise ValueError('A polygon has 3 to 2048 finite (x, y) vertices.')
+    tol, eps = _scale(v)
+    edge = np.roll(v, -1, axis=0)-v
+    if np.any(np.hypot(edge[:, 0], edge[:, 1]) <= tol):
+        raise ValueError('A polygon edge has zero length (do not repeat the first vertex at the end).')
+    w = v-v[0]
+    if abs(float(np.sum(_cross(w, np.roll(w, -1, axis=0))))) <= eps:
+        raise ValueError('A polygon needs a nonzero area.')
+    before = np.roll(edge, 1, axis=0)
+    if np.any((abs(_cross(before, edge)) <= eps) & ((before*edge).sum(-1) < 0)):
+        raise ValueError('Adjacent polygon edges may not fold back onto each other.')
+    n = len(v)
+    meets = _touching(v, v, eps, tol)
+    i, j = np.indices((n, n))
+    apart = (j >= i+2) & ~((i == 0) & (j == n-1))
+    if np.any(meets & apart):
+        raise ValueError('Polygon edges may not cross or touch non-adjacent edges.')
+
+
+def check_holes(outer, holes):
+    """Holes: at most 1024 simple contours strictly inside the outer contour, apart from it and
+    from each other (no crossing, touching or nesting)."""
+    import numpy as np
+    if len(holes) > 1024:
+        raise ValueError('A polygon has at most 1024 holes.')
+    outer = np.asarray(outer, dtype=float)
+    loops = []
+    for hole in holes:
+        check_contour(hole)
+        loop = np.asarray(hole, dtype=float)
+        tol, eps = _scale(outer, loop)
+        if np.any(_touching(loop, outer, eps, tol)) or not np.all(_strictly_inside(loop, outer)):
+            raise ValueError('A polygon hole lies strictly inside the outer contour.')
+        for other in loops:
+            tol, eps = _scale(other, loop)
+            if (np.any(_touching(loop, other, eps, tol)) or np.any(_inside_or_on(loop, other))
+                    or np.any(_inside_or_on(other, loop))):
+                raise ValueError('Polygon holes may not overlap, nest or touch.')
+        loops.append(loop)
+
+
+def _inside_or_on(points, contour):
+    from .planning import _contour
+    parity, edge = _contour(points[:, 0], points[:, 1], contour)
+    return parity | edge
+
+
+def _strictly_inside(points, contour):
+    from .planning import _contour
+    parity, edge = _contour(points[:, 0], points[:, 1], contour)
+    return parity & ~edge
 
 
 class TimeSignal(Model):

@@ -20,6 +20,9 @@ from .planning import field_axes
 C0 = 299792458.0
 
 
+PLANE_FIELDS = ('Ex', 'Ey', 'Ez', 'Hx', 'Hy', 'Hz')
+
+
 def apodization_window(times, settings):
     t = np.asarray(times, dtype=np.float64)
     mode = settings.apodization
@@ -248,3 +251,56 @@ def plane_result(monitor, plan, frequency, components, values, dimension):
                 accumulated_components=list(components), time_downsample=monitor.time_downsample,
                 settings=monitor.model_dump(), flux_units='reduced E*H * s^2 * '+('m per invariant length' if dimension == '2d' else 'm^2'),
                 field_units='reduced field * s', normal_axis=monitor.normal)
+
+
+def normalize_flux(sample, reference, *, subtract_incident=False, min_reference_fraction=.01):
+    """Plane power over a reference plane's positive power, per frequency (field_monitors.py:209-248).
+
+    Both planes must come from matched runs (same run_signature: mesh, sources, steps), share
+    frequencies, points, weights, normal and time downsampling, and hold unapodized fields and a
+    recorded flux.  With `subtract_incident` the reference's tangential fields are subtracted from
+    the sample's first (a reflection monitor), and the flux of the difference is used.  Ratios keep
+    their sign; a frequency whose reference power is zero or below `min_reference_fraction` of the
+    band's largest gives NaN, with the reason in `reasons`."""
+    if not 0 < min_reference_fraction < 1:
+        raise ValueError('min_reference_fraction lies strictly between 0 and 1.')
+    for key in ('frequency_hz', 'points_um', 'weights'):
+        a, b = np.asarray(sample[key]), np.asarray(reference[key])
+        if a.shape != b.shape or not np.allclose(a, b, rtol=1e-10, atol=0):
+            raise ValueError(f'The reference plane differs from the sample in {key}.')
+    if sample['normal_axis'] != reference['normal_axis']:
+        raise ValueError('The reference plane has another normal.')
+    if sample['settings'].get('time_downsample', 1) != reference['settings'].get('time_downsample', 1):
+        raise ValueError('The reference plane has another time downsampling.')
+    if any(p['settings']['spectrum']['apodization'] != 'none' for p in (sample, reference)):
+        raise ValueError('Flux normalization needs unapodized planes.')
+    signature = sample.get('run_signature')
+    if not signature or signature != reference.get('run_signature'):
+        raise ValueError('The reference run differs from the sample run (mesh, sources, steps or region settings).')
+    if sample['flux'] is None or reference['flux'] is None:
+        raise ValueError('Flux normalization needs the recorded flux of both planes.')
+    axis = 'xyz'.index(sample['normal_axis'])
+    if subtract_incident:
+        b, c = 'xyz'[(axis+1) % 3], 'xyz'[(axis+2) % 3]
+        difference = {}
+        for name in ('E'+b, 'E'+c, 'H'+b, 'H'+c):
+            parts = []
+            for plane in (sample, reference):
+                names = plane.get('components', list(PLANE_FIELDS))
+                if name not in names:
+                    raise ValueError(f'Subtracting the incident field needs {name} stored in both planes.')
+                parts.append(plane['fields'][..., names.index(name)])
+            difference[name] = parts[0]-parts[1]
+        density = .5*np.real(difference['E'+b]*difference['H'+c].conj()-difference['E'+c]*difference['H'+b].conj())
+        flux = density@np.asarray(sample['weights'])
+    else:
+        flux = np.asarray(sample['flux'])
+    power = np.abs(np.asarray(reference['flux']))
+    valid = (power > 0) & (power > min_reference_fraction*power.max())
+    ratio = np.full(len(flux), np.nan)
+    ratio[valid] = flux[valid]/power[valid]
+    reasons = np.array([None if ok else 'reference is zero' if p <= 0 else
+                        f'reference below {min_reference_fraction:g} of its band peak: no supported ratio'
+                        for ok, p in zip(valid, power)], dtype=object)
+    return dict(frequency_hz=np.asarray(sample['frequency_hz']), ratio=ratio, valid=valid,
+                subtract_incident=subtract_incident, reasons=reasons)

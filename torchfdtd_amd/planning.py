@@ -124,67 +124,132 @@ def periodic_axes(region):
 
 # ------------------------------------------------------------------ geometry
 
-def _rotation(obj):
-    angle = math.radians(obj.rotation)
-    c, s = math.cos(angle), math.sin(angle)
-    r = np.array([[c, -s, 0.], [s, c, 0.], [0., 0., 1.]])
+def _axis_turn(axis, degrees):
+    """Rotation about one axis (geometry.py:9-16); multiples of 90 degrees are exact."""
+    if axis == 'none':
+        return np.eye(3)
+    degrees %= 360
+    exact = {0: (1., 0.), 90: (0., 1.), 180: (-1., 0.), 270: (0., -1.)}
+    c, s = exact.get(degrees) or (math.cos(math.radians(degrees)), math.sin(math.radians(degrees)))
+    a = 'xyz'.index(axis)
+    b, d = (a+1) % 3, (a+2) % 3
+    m = np.eye(3)
+    m[b, b] = m[d, d] = c
+    m[b, d], m[d, b] = -s, s
+    return m
+
+
+def rotation_matrix(obj):
+    """Local -> world rotation (geometry.py:19-26): the legacy z angle, unreduced, then each listed
+    (axis, angle) about the fixed world axes in turn."""
+    a = math.radians(obj.rotation)
+    m = np.array([[math.cos(a), -math.sin(a), 0.], [math.sin(a), math.cos(a), 0.], [0., 0., 1.]])
     for axis, angle in zip(obj.rotation_axes, obj.rotation_angles):
-        if axis == 'none':
-            continue
-        angle %= 360
-        c, s = {0: (1., 0.), 90: (0., 1.), 180: (-1., 0.), 270: (0., -1.)}.get(
-            angle, (math.cos(math.radians(angle)), math.sin(math.radians(angle))))
-        a = 'xyz'.index(axis)
-        b, d = (a+1) % 3, (a+2) % 3
-        m = np.eye(3)
-        m[b, b] = m[d, d] = c
-        m[b, d] = -s
-        m[d, b] = s
-        r = m@r
-    return r
+        m = _axis_turn(axis, angle)@m
+    return m
 
 
-def _radii(obj):
-    return (obj.radius, obj.radius_2 if obj.make_ellipsoid else obj.radius,
-            obj.radius_3 if obj.make_ellipsoid else obj.radius)
+def _contour(x, y, contour):
+    """(crossing parity, on-boundary) of points against one closed contour (geometry.py:83-93).
+    The boundary carries a tolerance of 64 ulps of the contour's extent."""
+    v = np.asarray(contour, dtype=float)
+    extent = np.ptp(v, axis=0).max()
+    tol = 64*np.finfo(float).eps*max(extent, 1e-300)
+    shape = np.broadcast_shapes(np.shape(x), np.shape(y))
+    parity, edge = np.zeros(shape, bool), np.zeros(shape, bool)
+    for (ax, ay), (bx, by) in zip(v, np.roll(v, -1, axis=0)):
+        side = (bx-ax)*(y-ay)-(by-ay)*(x-ax)
+        near = abs(side) <= tol*max(abs(bx-ax), abs(by-ay))
+        edge |= near & (x >= min(ax, bx)-tol) & (x <= max(ax, bx)+tol) & (y >= min(ay, by)-tol) & (y <= max(ay, by)+tol)
+        if by != ay:
+            parity ^= ((ay > y) != (by > y)) & (x < (bx-ax)*(y-ay)/(by-ay)+ax)
+    return parity, edge
+
+
+class Solid:
+    """Membership and support box of one structure (geometry.py:104-148): a rectangle, sphere or
+    ellipsoid, circle or elliptic cylinder, ring (elliptic, with an inner radius and an arc) or
+    polygon with holes, extruded along its local w axis, placed by its rotation and centre."""
+
+    def __init__(self, obj):
+        self.obj, self.kind = obj, obj.kind
+        self.matrix = rotation_matrix(obj)
+        self.ellipse = obj.make_ellipsoid
+        self.radii = (obj.radius, obj.radius_2 if self.ellipse else obj.radius, obj.radius_3 if self.ellipse else obj.radius)
+        self.arc = (obj.theta_stop-obj.theta_start) % 360 or 360.
+        z_only = all(axis in ('none', 'z') or angle % 360 == 0 for axis, angle in zip(obj.rotation_axes, obj.rotation_angles))
+        rotation_free = (self.kind == 'sphere' and not self.ellipse) or (
+            self.kind in ('circle', 'ring') and not self.ellipse and z_only and (self.kind != 'ring' or self.arc == 360))
+        if rotation_free or (obj.rotation == 0 and not any(obj.rotation_angles)):
+            self.frame = 'translate'
+        elif not any(obj.rotation_angles):
+            self.frame = 'legacy'          # z angle only: the historical two-term form
+        else:
+            self.frame = 'matrix'
+
+    def support(self):
+        """(centre, span) of a box holding the solid: exact for boxes, ellipsoids and cylinders,
+        conservative for arcs and polygons."""
+        m, obj = self.matrix, self.obj
+        if self.kind == 'polygon':
+            corners = np.array([[x, y, z] for x, y in obj.vertices for z in (-obj.size[2]/2, obj.size[2]/2)])@m.T+obj.center
+            lo, hi = corners.min(axis=0), corners.max(axis=0)
+            return tuple((lo+hi)/2), tuple(hi-lo)
+        if self.kind == 'rectangle':
+            half = np.abs(m)@(np.array(obj.size)/2)
+        elif self.kind == 'sphere':
+            half = np.sqrt((m*m)@(np.array(self.radii)**2))
+        else:
+            rx, ry, _ = self.radii
+            half = np.sqrt((m[:, 0]*rx)**2+(m[:, 1]*ry)**2)+abs(m[:, 2])*obj.size[2]/2
+        return tuple(obj.center), tuple(2*half)
+
+    def local(self, x, y, z):
+        d = (x-self.obj.center[0], y-self.obj.center[1], z-self.obj.center[2])
+        if self.frame == 'translate':
+            return d
+        if self.frame == 'legacy':
+            a = math.radians(self.obj.rotation)
+            c, s = math.cos(a), math.sin(a)
+            return c*d[0]+s*d[1], -s*d[0]+c*d[1], d[2]
+        m = self.matrix
+        # R^T d, skipping exact zeros so that quarter turns move coordinates without rounding.
+        return tuple(sum(m[a, b]*d[a] for a in range(3) if m[a, b] != 0) for b in range(3))
+
+    def contains(self, x, y, z):
+        u, v, w = self.local(x, y, z)
+        obj = self.obj
+        slab = abs(w) <= obj.size[2]/2
+        if self.kind == 'rectangle':
+            return (abs(u) <= obj.size[0]/2) & (abs(v) <= obj.size[1]/2) & slab
+        if self.kind == 'polygon':
+            inside, edge = _contour(u, v, obj.vertices)
+            for hole in obj.holes:
+                parity, rim = _contour(u, v, hole)
+                inside, edge = inside ^ parity, edge | rim
+            return (inside | edge) & slab
+        rx, ry, rz = self.radii
+        if self.kind == 'sphere':
+            return (u/rx)**2+(v/ry)**2+(w/rz)**2 <= 1 if self.ellipse else u*u+v*v+w*w <= rx*rx
+        disc = ((u/rx)**2+(v/ry)**2 <= 1 if self.ellipse else u*u+v*v <= rx*rx) & slab
+        if self.kind == 'ring':
+            if obj.inner_radius > 0:
+                disc &= ((u/obj.inner_radius)**2+(v/obj.inner_radius_2)**2 >= 1 if self.ellipse
+                         else u*u+v*v >= obj.inner_radius**2)
+            if self.arc < 360:
+                polar = (np.degrees(np.arctan2(v, u))-obj.theta_start) % 360
+                disc &= (polar <= self.arc+1e-12) | ((u == 0) & (v == 0))
+        return disc
 
 
 def object_bounds(obj):
-    """Support box (centre, span) of a rectangle, sphere or cylinder (geometry.py:133-148)."""
-    r = _rotation(obj)
-    if obj.kind == 'rectangle':
-        half = np.abs(r)@(np.array(obj.size)/2)
-    elif obj.kind == 'sphere':
-        half = np.sqrt((r*r)@(np.array(_radii(obj))**2))
-    elif obj.kind in ('circle', 'ring'):
-        rx, ry, _ = _radii(obj)
-        half = np.sqrt((r[:, 0]*rx)**2+(r[:, 1]*ry)**2)+abs(r[:, 2])*obj.size[2]/2
-    else:
-        raise ValueError(f'{obj.name}: {obj.kind} solids are not voxelized by this engine')
-    return tuple(obj.center), tuple(2*half)
+    """Support box (centre, span) of a structure (geometry.py:136-148)."""
+    return Solid(obj).support()
 
 
 def contains(obj, x, y, z):
-    """Analytic membership of unrotated or z-rotated solids (geometry.py:106-130)."""
-    delta = [x-obj.center[0], y-obj.center[1], z-obj.center[2]]
-    if any(obj.rotation_angles) or obj.make_ellipsoid:
-        raise ValueError(f'{obj.name}: rotated axes and ellipsoids are not voxelized by this engine')
-    invariant = obj.kind == 'sphere' or (obj.kind == 'circle')
-    if invariant or obj.rotation == 0:
-        u, v, w = delta
-    else:
-        a = math.radians(obj.rotation)
-        c, s = math.cos(a), math.sin(a)
-        u, v, w = c*delta[0]+s*delta[1], -s*delta[0]+c*delta[1], delta[2]
-    if obj.kind == 'rectangle':
-        return (abs(u) <= obj.size[0]/2) & (abs(v) <= obj.size[1]/2) & (abs(w) <= obj.size[2]/2)
-    rx = obj.radius
-    if obj.kind == 'sphere':
-        return u*u+v*v+w*w <= rx*rx
-    mask = (u*u+v*v <= rx*rx) & (abs(w) <= obj.size[2]/2)
-    if obj.kind == 'ring':
-        raise ValueError(f'{obj.name}: rings are not voxelized by this engine')
-    return mask
+    """Analytic membership of a structure at world points (geometry.py:104-130)."""
+    return Solid(obj).contains(x, y, z)
 
 
 def field_axes(region, component):
@@ -200,42 +265,58 @@ def field_axes(region, component):
     return out
 
 
-def voxelize(project):
-    """Relative permittivity: (Nx,Ny,Nz) at cell centres ('cell') or (Nx,Ny,Nz,3) per E component ('yee')."""
+def voxelize(project, with_ownership=False):
+    """Relative permittivity: (Nx,Ny,Nz) at cell centres ('cell') or (Nx,Ny,Nz,3) per E component
+    ('yee') (solver.py:49-111).  with_ownership: (eps, ownership, counts) -- int32 index of the
+    owning material in project.materials (-1 background) per sample, and per structure id the
+    number of samples it set (the largest over the three components for 'yee')."""
     r = project.region
     if r.material_sampling == 'yee':
-        return np.stack([_voxelize_at(project, field_axes(r, c)) for c in ('Ex', 'Ey', 'Ez')], axis=-1)
+        parts = [_voxelize_at(project, field_axes(r, c)) for c in ('Ex', 'Ey', 'Ez')]
+        eps = np.stack([p[0] for p in parts], axis=-1)
+        if not with_ownership:
+            return eps
+        owner = np.stack([p[1] for p in parts], axis=-1)
+        return eps, owner, {k: max(p[2][k] for p in parts) for k in parts[0][2]}
     axes = [(np.arange(n)+.5)*r.mesh-s/2 for n, s in zip(r.shape, r.actual_size)]
     if r.dimension == '2d':
         axes[2] = np.array([0.])
-    return _voxelize_at(project, axes)
+    eps, owner, counts = _voxelize_at(project, axes)
+    return (eps, owner, counts) if with_ownership else eps
 
 
 def _voxelize_at(project, axes):
     r = project.region
     shape = tuple(len(a) for a in axes)
     eps = np.full(shape, r.background_index**2, dtype=real_dtype(r))
+    owner = np.full(shape, -1, dtype=np.int32)
+    counts = {}
+    number = {m.name: i for i, m in enumerate(project.materials)}
     materials = {m.name: m for m in project.materials}
     # Lower mesh order wins; at equal order the later structure wins.
     for obj in sorted(project.structures, key=lambda s: -s.mesh_order):
         if not obj.enabled:
             continue
-        center, size = object_bounds(obj)
-        slices = []
+        solid = Solid(obj)
+        center, size = solid.support()
+        # Only the samples inside the support box are tested (with a rounding margin).
+        window = []
         for axis, c, span in zip(axes, center, size):
-            tol = 128*np.finfo(float).eps*max(abs(c), span, 1e-300)
-            slices.append(slice(int(np.searchsorted(axis, c-span/2-tol, 'left')),
-                                int(np.searchsorted(axis, c+span/2+tol, 'right'))))
-        slices = tuple(slices)
-        if any(s.start == s.stop for s in slices):
+            margin = 128*np.finfo(float).eps*max(abs(c), span, 1e-300)
+            window.append(slice(int(np.searchsorted(axis, c-span/2-margin, 'left')),
+                                int(np.searchsorted(axis, c+span/2+margin, 'right'))))
+        window = tuple(window)
+        if any(w.start == w.stop for w in window):
+            counts[obj.id] = 0
             continue
-        xyz = np.meshgrid(*(axis[s] for axis, s in zip(axes, slices)), indexing='ij', sparse=True)
-        mask = contains(obj, *xyz)
+        mask = solid.contains(*np.meshgrid(*(axis[w] for axis, w in zip(axes, window)), indexing='ij', sparse=True))
         material = materials[obj.material]
         if material.model != 'dielectric':
             raise ValueError(f'{material.name}: dispersive and tensor materials are not run by this engine')
-        eps[slices][mask] = material.instantaneous_epsilon
-    return eps
+        eps[window][mask] = material.instantaneous_epsilon
+        owner[window][mask] = number[obj.material]
+        counts[obj.id] = int(np.count_nonzero(mask))
+    return eps, owner, counts
 
 
 def index_at(center, region, component=None):

@@ -9,6 +9,7 @@ run-control diagnostics, cancellation), exactly at the reference's barrier steps
 from __future__ import annotations
 
 import ctypes
+import hashlib
 import json
 import math
 import threading
@@ -27,7 +28,7 @@ from .grid import (MaterialIndex, YeeGrid, configure_cuda_kernel, half_step_desc
 from .models import Project
 from .monitors import FrequencyPlane, apodization_window, point_spectrum
 from .planning import field_axes, index_at, source_terms, voxelize
-from .run_control import DecayDecision, StateDiagnostics, source_end_time
+from .run_control import RunMonitor, check_count
 
 C0 = 299792458.0
 ENGINE_LOCK = threading.Lock()
@@ -388,8 +389,7 @@ class Simulation:
         run = Stepper(p, epsilon=self.epsilon)
         g = run.grid
         control = r.run_control
-        diagnostics = StateDiagnostics(g) if control.auto_shutoff or control.divergence_check else None
-        decision = DecayDecision(control, source_end_time(p), g.time_step)
+        monitor = RunMonitor([g], [p], check_count(p)) if control.auto_shutoff or control.divergence_check else None
         graph = (StepGraphs(run.step, run.states(), r.steps, cuda_graph_steps, pair=run.step_pair if run.one_pass else None,
                             prepare=run.refresh_materials if run.one_pass else None,
                             quad=run.quad if run.two_step else None)
@@ -404,9 +404,13 @@ class Simulation:
         compute_start = time.perf_counter()
         completed, termination, diagnostic_seconds, replays = 0, 'max_steps', 0., 0
         barriers = set(range(interval, r.steps+1, interval))
-        if diagnostics is not None:
+        if monitor is not None:
             barriers.update(range(control.check_interval, r.steps+1, control.check_interval))
         width = graph.width if graph is not None else 1
+        eager_steps = 0
+        # Barriers enqueue the diagnostics and the frame copies on the GPU; the host waits only
+        # where the run-control rule could stop the run at that barrier or a progress report
+        # needs the frame.
         for target, advance in observation_schedule(r.steps, width, barriers):
             if cancel is not None and cancel.is_set():
                 termination = 'cancelled'
@@ -416,42 +420,45 @@ class Simulation:
                 replays += 1
             else:
                 run.step()
+                eager_steps += 1
             completed = target
             stop = False
-            if diagnostics is not None and (completed % control.check_interval == 0 or completed == r.steps):
+            if monitor is not None and (completed % control.check_interval == 0 or completed == r.steps):
                 t0 = time.perf_counter()
-                try:
-                    stop = decision.update(completed, *diagnostics.measure())
-                except FloatingPointError as exc:
-                    raise FloatingPointError(f'{exc} (diagnostic step {completed})') from exc
+                monitor.check(completed)
+                stop = monitor.stopped(completed)
                 diagnostic_seconds += time.perf_counter()-t0
             if stop:
                 termination = 'decayed'
             if completed % interval == 0 or completed == r.steps or stop:
-                frame = view.frame(g)
-                if not np.isfinite(frame).all():
-                    raise FloatingPointError('Fields diverged. Check mesh, sources and precision.')
-                frames.append(frame)
+                frames.append(view.device_frame(g))
                 frame_steps.append(completed)
                 if progress:
+                    latest = monitor.latest() if monitor is not None else None
+                    frame = frames[-1].cpu().numpy()
+                    if not np.isfinite(frame).all():
+                        raise FloatingPointError('Fields diverged. Check mesh, sources and precision.')
                     progress({'step': completed, 'total': r.steps, 'frame': frame.tolist(),
                               'elapsed': time.perf_counter()-compute_start,
-                              'termination_reason': termination if stop else None,
-                              'diagnostics': decision.history[-1] if decision.history else None})
+                              'termination_reason': termination if stop else None, 'diagnostics': latest})
             if stop:
                 break
         torch.cuda.synchronize()
         seconds = time.perf_counter()-compute_start
+        frames = check_frames(monitor, frames, frame_steps)
         e, h = g.E.cpu().numpy().copy(), g.H.cpu().numpy().copy()
         if not np.isfinite(e).all() or not np.isfinite(h).all():
             raise FloatingPointError('Non-finite fields detected.')
-        summary = run_summary(p, run, e, completed, termination, seconds, setup_seconds,
+        kernel_steps = dict(graph.kernel_steps) if graph is not None else dict(two_step=0, one_pass=0, split=eager_steps)
+        summary = run_summary(p, run, e, completed, termination, seconds, setup_seconds, kernel_steps=kernel_steps,
                               cuda_graph=graph is not None, cuda_graph_steps=width, cuda_graph_replays=replays,
-                              diagnostics=decision.history, diagnostic_seconds=diagnostic_seconds,
-                              diagnostic_backend=diagnostics.backend if diagnostics is not None else None,
+                              diagnostics=monitor.history(0) if monitor is not None else [],
+                              diagnostic_seconds=diagnostic_seconds,
+                              diagnostic_backend=monitor.backend if monitor is not None else None,
                               slice_index=plane_index)
-        planes = [m.result() for m in run.planes]
-        return Result(p, summary, np.array(frames), np.array(frame_steps), eps_plane,
+        signature = run_signature(p, completed)
+        planes = [dict(m.result(), run_signature=signature) for m in run.planes]
+        return Result(p, summary, frames, np.array(frame_steps), eps_plane,
                       run.traces[:completed].cpu().numpy(), np.arange(1, completed+1)*g.time_step, e, h, planes)
 
 
@@ -472,13 +479,51 @@ class DisplaySlice:
         return data[::max(1, math.ceil(data.shape[0]/256)), ::max(1, math.ceil(data.shape[1]/256))]
 
     def frame(self, grid):
-        return self.plane((grid.E if self.electric else grid.H)[..., self.component]).cpu().numpy().copy()
+        return self.device_frame(grid).cpu().numpy()
+
+    def device_frame(self, grid):
+        """A device copy of the displayed plane (no host synchronisation)."""
+        return self.plane((grid.E if self.electric else grid.H)[..., self.component]).clone()
 
     def epsilon(self, eps):
         return np.array(self.plane(eps[..., self.component] if eps.ndim == 4 else eps))
 
 
-def run_summary(project, run, e, completed, termination, seconds, setup_seconds, **extra):
+def run_signature(project, steps):
+    """Physical identity of a run, for matching a sample plane with its reference (solver.py:493-508):
+    SHA-256 of the canonical JSON of the region settings without placement and display fields, the
+    resolved sources, the step count and the mesh nodes."""
+    r = project.region
+    placement = {'backend', 'cuda_kernel', 'cuda_monitor_kernel', 'execution_mode', 'tiling', 'resident_cell_limit', 'field',
+                 'slice_axis', 'slice_position', 'complex_display', 'snapshot_interval'}
+    identity = dict(region=r.model_dump(exclude=placement),
+                    sources=[project.resolved_source(s).model_dump() for s in project.sources], steps=steps,
+                    nodes=[np.asarray(a).tolist() for a in r.mesh_nodes])
+    return hashlib.sha256(json.dumps(identity, sort_keys=True).encode()).hexdigest()
+
+
+def check_frames(monitor, frames, frame_steps):
+    """Host copies of the device frames, after every run-control verdict has arrived.  The first
+    failure in step order raises: a divergence verdict, or a non-finite displayed frame (a verdict
+    at the same barrier comes first, as the verdict is evaluated before the frame is taken)."""
+    failure = None
+    if monitor is not None:
+        try:
+            monitor.finish()
+        except FloatingPointError as exc:
+            failure = exc
+    host = np.array([f.cpu().numpy() for f in frames])
+    for frame, step in zip(host, frame_steps):
+        if not np.isfinite(frame).all():
+            if failure is None or step < monitor.failed_step:
+                raise FloatingPointError('Fields diverged. Check mesh, sources and precision.')
+            break
+    if failure is not None:
+        raise failure
+    return host
+
+
+def run_summary(project, run, e, completed, termination, seconds, setup_seconds, kernel_steps=None, **extra):
     """Result.summary of one run (the keys of the reference's solver.py summary)."""
     r = project.region
     props = torch.cuda.get_device_properties(run.grid.device)
@@ -486,7 +531,9 @@ def run_summary(project, run, e, completed, termination, seconds, setup_seconds,
                    duration_fs=r.time_step*r.steps*1e15, backend='cuda', precision=r.precision,
                    gpu=props.name, gpu_arch=getattr(props, 'gcnArchName', None),
                    cuda_kernel=run.kernel, cuda_monitor_kernel='hip', engine='torchfdtd_amd HIP gfx950',
-                   steps_per_pass=2 if run.two_step else 1 if run.one_pass else 0,
+                   kernel_steps=kernel_steps,
+                   steps_per_pass=(0 if kernel_steps is None else 2 if kernel_steps['two_step'] else
+                                   1 if kernel_steps['one_pass'] else 0),
                    steps=completed, requested_steps=r.steps, cancelled=termination == 'cancelled',
                    termination_reason=termination, auto_shutoff=termination == 'decayed',
                    seconds=seconds, setup_seconds=setup_seconds,
