@@ -280,6 +280,16 @@ int tfdtd_norm_pack(const tfdtd_norm_grid* grids, int32_t count, void* host_tabl
 int tfdtd_state_norm(const void* device_table, int32_t count, int64_t tasks, int32_t dtype, double* partial, double* out,
                      void* stream);
 
+/* tfdtd_state_norm without the caller's partial buffer, and without any allocation: the partial
+ * sums live in a buffer of TFDTD_NORM_SCRATCH_TASKS tasks (6 MiB) that the library holds in each
+ * device's memory until the process ends.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  tasks beyond the limit return TFDTD_EINVAL.  Every call on one device shares that
+ * device's buffer: calls that overlap on one device (two streams, or two host threads) race and
+ * must be ordered by the caller.  The result equals tfdtd_state_norm's bit for bit. */
+#define TFDTD_NORM_SCRATCH_TASKS 262144
+int tfdtd_state_norm_noalloc(const void* device_table, int32_t count, int64_t tasks, int32_t dtype, double* out,
+                             void* stream);
 
 /* Run-control decision at a diagnostic barrier, on the device (replaces the host-side
  * DecayDecision.update of torchfdtd/run_control.py:58-85 and the host check of

@@ -154,6 +154,7 @@ PROTOTYPES = {
     'tfdtd_norm_table_bytes': (ctypes.c_size_t, [ctypes.POINTER(NormGrid), i32]),
     'tfdtd_norm_pack': (i32, [ctypes.POINTER(NormGrid), i32, vp, ctypes.c_size_t]),
     'tfdtd_state_norm': (i32, [vp, i32, i64, i32, vp, vp, vp]),
+    'tfdtd_state_norm_noalloc': (i32, [vp, i32, i64, i32, vp, vp]),
     'tfdtd_decay_update': (i32, [vp, vp, vp, i32, i64, vp, i32, vp]),
     'tfdtd_record_curl': (i32, [ctypes.POINTER(CurlRecord), vp]),
     'tfdtd_adjoint_step': (i32, [ctypes.POINTER(Adjoint), vp]),

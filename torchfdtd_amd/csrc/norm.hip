@@ -10,6 +10,8 @@
 
 #include "common.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr int kTask = 4096;
@@ -157,6 +159,33 @@ __global__ __launch_bounds__(64) void decay_update(const tfdtd_decay_rule* __res
     states[g] = s;
 }
 
+// Partial sums of tfdtd_state_norm_noalloc: one copy per device, held until the process ends.
+// Host code never passes `scratch` itself to a kernel (on the host it names a stand-in, not
+// device memory); it uses the per-device address from hipGetSymbolAddress below.
+static __device__ __align__(256) double scratch[3 * TFDTD_NORM_SCRATCH_TASKS];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+double* scratch_address[kMaxDevices] = {};
+
+// The address of the current device's copy of `scratch`, looked up once per device.
+int scratch_for_current_device(double** out) {
+    int device = -1;
+    hipError_t status = hipGetDevice(&device);
+    if (status != hipSuccess) return tfdtd_fail(TFDTD_EHIP, "norm: hipGetDevice: %s", hipGetErrorString(status));
+    if (device < 0 || device >= kMaxDevices) return tfdtd_fail(TFDTD_EINVAL, "norm: device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_address[device]) {
+        void* address = nullptr;
+        status = hipGetSymbolAddress(&address, HIP_SYMBOL(scratch));
+        if (status != hipSuccess || !address)
+            return tfdtd_fail(TFDTD_EHIP, "norm: hipGetSymbolAddress(scratch): %s", hipGetErrorString(status));
+        scratch_address[device] = static_cast<double*>(address);
+    }
+    *out = scratch_address[device];
+    return TFDTD_OK;
+}
+
 long long tasks_of(long long n) { return (n + kTask - 1) / kTask; }
 
 long long grid_tasks(const tfdtd_norm_grid& d) {
@@ -244,6 +273,18 @@ int tfdtd_decay_update(const tfdtd_decay_rule* rules, tfdtd_decay_state* states,
     hipLaunchKernelGGL(decay_update, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, static_cast<hipStream_t>(stream),
                        rules, states, norms, count, (long long)step, history, rows_per_grid);
     return tfdtd_check_launch("decay_update");
+}
+
+int tfdtd_state_norm_noalloc(const void* device_table, int32_t count, int64_t tasks, int32_t dtype, double* out,
+                             void* stream) {
+    if (tasks > TFDTD_NORM_SCRATCH_TASKS)
+        return tfdtd_fail(TFDTD_EINVAL, "norm: %lld tasks exceed the %d of the library's partial-sum buffer",
+                          (long long)tasks, TFDTD_NORM_SCRATCH_TASKS);
+    double* partial = nullptr;
+    const int status = scratch_for_current_device(&partial);
+    if (status) return status;
+    // Every task writes its own row of the buffer before norm_final reads it: nothing to clear.
+    return tfdtd_state_norm(device_table, count, tasks, dtype, partial, out, stream);
 }
 
 }  // extern "C"
