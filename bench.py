@@ -246,10 +246,37 @@ def surrogate_line(batch=128, reps=5):
     e1.record()
     torch.cuda.synchronize()
     seconds = e0.elapsed_time(e1)/1e3/reps
+    # The kernels on one large convolution (64 x 32x32x16 x 64 -> 128 channels, 3x3x3, stride 1): the
+    # implicit GEMM's own rate, forward and forward + backward (dgrad + wgrad), against the peak.
+    from torchfdtd_amd.surrogate import conv3d
+    gen = torch.Generator(device='cuda').manual_seed(5)
+    x = torch.randn((64, 32, 32, 16, 64), device='cuda', generator=gen).requires_grad_(True)
+    w = (torch.randn((128, 3, 3, 3, 64), device='cuda', generator=gen)*.02).requires_grad_(True)
+    b = torch.zeros(128, device='cuda', requires_grad=True)
+    gemm = 2*64*32*32*16*128*27*64
+    rates = {}
+    for label, backward in (('forward', False), ('forward_dgrad_wgrad', True)):
+        for _ in range(2):
+            y = conv3d(x, w, b, 1, 1, True)
+            if backward:
+                y.sum().backward()
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            y = conv3d(x, w, b, 1, 1, True)
+            if backward:
+                y.sum().backward()
+        e1.record()
+        torch.cuda.synchronize()
+        rates[label] = round(gemm*(3 if backward else 1)*reps/(e0.elapsed_time(e1)/1e3)/1e12, 1)
+    del x, w, b, y
+    torch.cuda.empty_cache()
     return dict(workload='SurrogateNet training step: 4 stride-2 3x3x3 convs 1->32->64->128->256 on 128x64x64x8, MLP head',
                 parameters=net.parameter_count(), ms_per_step=round(seconds*1e3, 3),
                 conv_tflops=round(flops/seconds/1e12, 2), peak_tflops=2500.0,
-                frac=round(flops/seconds/1e12/2500.0, 5), dtype='bf16 MFMA, fp32 accumulate')
+                frac=round(flops/seconds/1e12/2500.0, 5), dtype='bf16 MFMA, fp32 accumulate',
+                large_conv=dict(shape='64x32x32x16x64 -> 128, 3x3x3, stride 1', gemm_tflop=round(gemm/1e12, 3),
+                                tflops=rates, frac={k: round(v/2500.0, 4) for k, v in rates.items()}))
 
 
 def adjoint_line(steps=2000):
@@ -329,6 +356,7 @@ def main():
     ap.add_argument('--no-observed', action='store_true', help='skip the 2D/3D waveguide lines with their FP64 error')
     ap.add_argument('--no-surrogate', action='store_true')
     ap.add_argument('--repeats', type=int, default=3, help='further timed samples of --steps steps (median reported)')
+    ap.add_argument('--no-tune', action='store_true', help='keep the default launch shape of the cube step')
     ap.add_argument('--batch-grids', type=int, default=64)
     ap.add_argument('--batch-steps', type=int, default=2000)
     ap.add_argument('--batch-launch', choices=('tune', 'split', 'march'), default='tune')
@@ -370,6 +398,12 @@ def main():
     g = run.grid
     g.inverse_permittivity.copy_(cube_inverse_permittivity(mask)[..., None])
     torch.cuda.synchronize()
+    tuning = None
+    if not args.no_tune:
+        # The grid's launch shapes measured on this box (bitwise-identical candidates only).
+        from torchfdtd_amd.tuning import tune_launch
+        print('bench: tuning the launch shape', file=sys.stderr, flush=True)
+        tuning = tune_launch(run, sweeps=3, units=2)
     # Graphs of whole fours keep every step on the two-step pass (a graph of 50 steps ends in a pair
     # of one-step launches).
     width = args.graph_width if not run.two_step else args.graph_width // 4 * 4
@@ -467,7 +501,8 @@ def main():
             'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic (seeded smoothed-threshold eps noise)',
             'config': {'workload': f'{args.n}^3 cube, 16-cell CPML all faces, eps=1+11s per cell, Ez dipole, '
                                    f'{region.steps - 8} steps of which {args.steps} timed',
-                       'cells': cells, 'graph_steps': width, 'wall_s': round(wall, 3)},
+                       'cells': cells, 'graph_steps': width, 'wall_s': round(wall, 3),
+                       'launch': run.launch_shape(), 'launch_tuning': tuning},
             'samples_ms_per_step': [round(t*1e3/args.steps, 4) for t in samples],
             'median_ms_per_step': round(sorted(samples)[len(samples)//2]*1e3/args.steps, 4),
             'roofline': roofline, 'cpu_baseline': baseline}

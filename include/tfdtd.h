@@ -339,6 +339,44 @@ typedef struct tfdtd_decay_state {
 int tfdtd_decay_update(const tfdtd_decay_rule* rules, tfdtd_decay_state* states, const double* norms, int32_t count,
                        int64_t step, double* history, int32_t rows_per_grid, void* stream);
 
+/* The source boxes of a half step alone (torchfdtd/solver.py:778-783): dst[box][comp] +=
+ * values[*counter] in source order, without the step's curl update.  A grid whose E update is
+ * followed by a dispersive-material correction runs its E half step with nsrc = 0, the correction,
+ * then this call, in the reference's order. */
+int tfdtd_inject(const tfdtd_half_step* step, void* stream);
+
+/* Passive Drude/Lorentz materials (torchfdtd/materials.py:29-87, MaterialADE): the coupled
+ * trapezoidal update of one material's owned samples around the E half step.
+ *   tfdtd_ade_save     before the E update: old = E at the samples;
+ *   tfdtd_ade_correct  after it, before the E sources, per sample and pole j:
+ *     response_j = (Q_j - a_j P_j)/d_j
+ *     new = (eps_inf E - k_sum old - sum_j response_j)/(eps_inf + k_sum)
+ *     delta_j = response_j + k_j (new + old);  P_j += delta_j;  Q_j = -Q_j + 2 delta_j;  E = new
+ * with a_j = (w0 dt)^2/2, d_j = 1 + g dt/2 + (w0 dt)^2/4, k_j = s dt^2/(4 d_j) (computed by the
+ * caller in double, rounded to the field precision here).  One pole forms eps_inf + k in double
+ * and rounds it once; several poles sum k_j and add eps_inf in the field precision (the
+ * reference's scalar and array coefficients).  Samples are field elements: per_cell = 1 takes
+ * index[i/3] as a cell and component i % 3 ('cell' sampling), per_cell = 0 takes index[i] as an
+ * element index ((x*ny+y)*nz+z)*3+c ('yee' sampling).  P and Q are real[poles][samples]; old is
+ * real[samples], written by save and read by correct. */
+#define TFDTD_ADE_MAX_POLES 16
+typedef struct tfdtd_ade {
+    int32_t dtype;
+    int32_t poles;
+    int64_t samples;
+    int32_t per_cell;
+    const int64_t* index;
+    void* field;
+    void* old;
+    void* P;
+    void* Q;
+    double a[TFDTD_ADE_MAX_POLES], d[TFDTD_ADE_MAX_POLES], k[TFDTD_ADE_MAX_POLES];
+    double eps_inf;
+} tfdtd_ade;
+
+int tfdtd_ade_save(const tfdtd_ade* ade, void* stream);
+int tfdtd_ade_correct(const tfdtd_ade* ade, void* stream);
+
 /* Forward record for the adjoint: curl H of the E update (backward differences, periodic wrap
  * included) on the design box cells [box_lo, box_hi), written to
  * record[counter % modulus][box cell][3] -- read before the step's E update.  With the E-family
@@ -437,6 +475,11 @@ int tfdtd_gelu_backward(const float* dy, const float* pre, int64_t count, int32_
 size_t tfdtd_last_error(char* buffer, size_t size);
 
 int tfdtd_abi_version(void);
+
+/* SHA-256 (hex) of the sources the library was compiled from (csrc and this header), set by the
+ * build; a loader compares it with the sources beside it and refuses a stale library.  No
+ * reference counterpart (the reference compiles its kernels at run time). */
+const char* tfdtd_source_digest(void);
 
 #ifdef __cplusplus
 }

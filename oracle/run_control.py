@@ -9,14 +9,21 @@ import math
 import numpy as np
 
 
-def state_norm(E, H, inverse, memories=()):
-    """(energy, peak, finite): sum_c E_c^2/inv_eps_c + H_c^2 in FP64, max |E|,|H|, and whether every
-    CPML memory array is finite (run_control.py:131-159; unit cell volumes)."""
+def state_norm(E, H, inverse, memories=(), materials=(), dt=None):
+    """(energy, peak, finite): sum_c E_c^2/inv_eps_c + H_c^2 in FP64, plus the oscillator energies
+    (Q/(dt sqrt s))^2 + (P w0/sqrt s)^2 of dispersive materials, max |E|,|H|, and whether every CPML
+    memory array is finite (run_control.py:131-159; unit cell volumes)."""
     e, h, inv = (np.asarray(a, dtype=np.float64) for a in (E, H, inverse))
     energy = 0.
     for j in range(3):
         energy = energy+float((e[..., j]*e[..., j]*(1/inv[..., j if inv.shape[-1] == 3 else 0])).sum())
         energy = energy+float((h[..., j]*h[..., j]).sum())
+    for state in materials:
+        for j, (w0, strength, _) in enumerate(state.oscillators):
+            p = np.asarray(state.P[j] if state.multiple else state.P, dtype=np.float64)
+            q = np.asarray(state.Q[j] if state.multiple else state.Q, dtype=np.float64)
+            energy = energy+float(((q/(dt*np.sqrt(strength)))**2).sum())
+            energy = energy+float(((p*(w0/np.sqrt(strength)))**2).sum())
     peak = float(max(np.abs(e).max(), np.abs(h).max()))
     finite = all(bool(np.isfinite(m).all()) for m in memories)
     return energy, peak, finite

@@ -179,7 +179,9 @@ class Scene:
             xyz = np.meshgrid(*(a[s] for a, s in zip(axes, slices)), indexing='ij', sparse=True)
             mask = _contains(obj, *xyz)
             m = mats[obj.material]
-            eps[slices][mask] = m.get('index', 1.5)**2
+            # models.py:156-159: a dispersive material's instantaneous permittivity is epsilon_inf.
+            dispersive = m.get('model', 'dielectric') in ('drude', 'lorentz', 'multipole')
+            eps[slices][mask] = m.get('epsilon_inf', 1.) if dispersive else m.get('index', 1.5)**2
             owner[slices][mask] = index[obj.material]
             counts[obj.id] = int(np.count_nonzero(mask))
         return eps, owner, counts

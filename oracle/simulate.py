@@ -16,9 +16,11 @@ from .yee import Grid
 
 
 def prepare(project, native=False):
-    """(scene, eps, grid); native=True steps the grid with the C restatement (oracle/native.py)."""
+    """(scene, eps, grid); native=True steps the grid with the C restatement (oracle/native.py).
+    Dispersive materials get their ADE states in grid.material_states (oracle/materials.py)."""
+    from .materials import configure
     scene = Scene(project)
-    eps = scene.voxelize()
+    eps, ownership, _ = scene.voxelize(with_ownership=True)
     inverse = np.ones((*scene.shape, 3), scene.real)
     inverse[:] = 1/(eps if eps.ndim == 4 else eps[..., None])
     if native:
@@ -26,6 +28,7 @@ def prepare(project, native=False):
         grid = NativeGrid(scene, inverse)
     else:
         grid = Grid(scene, inverse)
+    grid.material_states = configure(grid, scene, ownership)
     return scene, eps, grid
 
 
@@ -59,8 +62,13 @@ def run(project, steps=None, grid=None, record_frames=True, native=False, rule=N
         for values, loc, c in sources[family]:
             field[loc+(c,)] += values[n]
 
+    states = getattr(g, 'material_states', [])
     for n in range(steps):
+        # materials.py / cuda_kernels.py:355-362: the ADE wraps the E update, before the E sources.
+        prepared = [state.prepare(g.E) for state in states]
         g.update_E()
+        for state, (old, response) in zip(states, prepared):
+            state.correct(g.E, old, response)
         inject('E', n)
         g.update_H()
         inject('H', n)
@@ -72,7 +80,7 @@ def run(project, steps=None, grid=None, record_frames=True, native=False, rule=N
         stop = False
         if rule is not None and (done % check_interval == 0 or done == steps):
             from .run_control import state_norm
-            energy, peak, finite = state_norm(g.E, g.H, g.inverse_permittivity, g.memory_states)
+            energy, peak, finite = state_norm(g.E, g.H, g.inverse_permittivity, g.memory_states, states, scene.dt)
             if not finite:
                 raise FloatingPointError('Non-finite CPML or incident-line memory detected.')
             stop = rule.update(done, energy, peak)

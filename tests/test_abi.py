@@ -28,6 +28,15 @@ def test_library_exports_every_declared_function():
     assert lib.tfdtd_abi_version() == 1
 
 
+def test_library_is_built_from_the_sources_beside_it():
+    """The build stamps the sources' digest into the library; the loader refuses a stale one."""
+    lib = N.load_library()
+    assert lib.tfdtd_source_digest().decode() == N.source_digest()
+    stale = ctypes.CDLL(str(N.LIBRARY))
+    stale.tfdtd_source_digest.restype = ctypes.c_char_p
+    assert len(stale.tfdtd_source_digest()) == 64
+
+
 def test_errors_are_reported_without_a_gpu():
     lib = N.load_library()
     d = N.HalfStep()
@@ -72,6 +81,8 @@ int main(void) {
   printf("tfdtd_decay_state %zu\n", sizeof(tfdtd_decay_state));
   F(tfdtd_decay_rule, field_limit); F(tfdtd_decay_rule, auto_shutoff);
   F(tfdtd_decay_state, verdict_step); F(tfdtd_decay_state, checks);
+  printf("tfdtd_ade %zu\n", sizeof(tfdtd_ade));
+  F(tfdtd_ade, samples); F(tfdtd_ade, index); F(tfdtd_ade, a); F(tfdtd_ade, k); F(tfdtd_ade, eps_inf);
   printf("tfdtd_adjoint %zu\n", sizeof(tfdtd_adjoint));
   F(tfdtd_adjoint, courant); F(tfdtd_adjoint, e); F(tfdtd_adjoint, h); F(tfdtd_adjoint, nseed); F(tfdtd_adjoint, seeds); F(tfdtd_adjoint, seed_rows);
   printf("tfdtd_conv %zu\n", sizeof(tfdtd_conv));
@@ -92,7 +103,7 @@ def test_ctypes_mirror_matches_the_c_layout(tmp_path):
               'tfdtd_cpml_segment': N.CpmlSegment, 'tfdtd_source': N.SourceBox, 'tfdtd_march': N.March,
               'tfdtd_march_face': N.MarchFace, 'tfdtd_march_family': N.MarchFamily, 'tfdtd_norm_grid': N.NormGrid,
               'tfdtd_curl_record': N.CurlRecord, 'tfdtd_adjoint': N.Adjoint,
-              'tfdtd_decay_rule': N.DecayRule, 'tfdtd_decay_state': N.DecayState,
+              'tfdtd_decay_rule': N.DecayRule, 'tfdtd_decay_state': N.DecayState, 'tfdtd_ade': N.Ade,
               'tfdtd_conv': __import__('torchfdtd_amd.surrogate', fromlist=['ConvShape']).ConvShape}
     for key, value in lines.items():
         if '.' in key:

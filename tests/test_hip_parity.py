@@ -77,7 +77,7 @@ def test_random_fields_and_every_cpml_state_bitwise(gpu, precision, layout):
 @pytest.mark.parametrize('layout,tile_y,chunk_x,nz,components', [
     ('pml3d', 8, 7, 131, 3), ('pml3d', 16, 64, 200, 3), ('pec', 8, 5, 131, 3), ('pml3d', 16, 0, 131, 3),
     ('pml3d', 8, 9, 200, 3), ('pec', 16, 6, 200, 3), ('pml3d', 8, 6, 200, 'index'), ('pec', 16, 4, 200, 1),
-    ('xpec', 8, 7, 131, 3), ('xpec2', 16, 6, 131, 1)])
+    ('xpec', 8, 7, 131, 3), ('xpec2', 16, 6, 131, 1), ('pml3d', 8, 2, 131, 3), ('pml3d', 16, 3, 200, 1)])
 def test_one_pass_step_matches_oracle_bitwise(gpu, precision, layout, tile_y, chunk_x, nz, components):
     """The one-pass E+H kernel (ping-pong buffers, LDS tiles with halo recompute, x chunks, CPML-free
     interior and CPML boundary launches) equals the reference update bit for bit, with 1/eps per

@@ -7,6 +7,7 @@ library or the GPU is missing.
 from __future__ import annotations
 
 import ctypes
+import hashlib
 import os
 import subprocess
 from pathlib import Path
@@ -14,7 +15,7 @@ from pathlib import Path
 PACKAGE = Path(__file__).resolve().parent
 ROOT = PACKAGE.parent
 LIBRARY = PACKAGE/'libtfdtd.so'
-SOURCES = [PACKAGE/'csrc'/name for name in ('yee.hip', 'march.hip', 'norm.hip', 'adjoint.hip', 'conv.hip')]
+SOURCES = [PACKAGE/'csrc'/name for name in ('yee.hip', 'march.hip', 'norm.hip', 'adjoint.hip', 'conv.hip', 'ade.hip')]
 HEADER = ROOT/'include'/'tfdtd.h'
 
 HIPCC_FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-shared', '-std=c++17',
@@ -23,12 +24,25 @@ HIPCC_FLAGS = ['--offload-arch=gfx950', '-O3', '-fPIC', '-shared', '-std=c++17',
                '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops']
 
 
+def source_digest():
+    """SHA-256 over the library's sources (every file under csrc/ and the C header), in name order;
+    None where the sources are not beside the package."""
+    files = sorted(p for p in (PACKAGE/'csrc').glob('*') if p.suffix in ('.hip', '.h')) + [HEADER]
+    if not all(f.exists() for f in files):
+        return None
+    digest = hashlib.sha256()
+    for f in files:
+        digest.update(f.name.encode()+b'\0'+f.read_bytes()+b'\0')
+    return digest.hexdigest()
+
+
 def build(verbose=False):
-    """Compile every HIP source into LIBRARY (always recompiles)."""
+    """Compile every HIP source into LIBRARY (always recompiles), stamped with source_digest()."""
     hipcc = os.environ.get('HIPCC', '/opt/rocm/bin/hipcc')
     sources = [str(s) for s in SOURCES if s.exists()]
     tmp = LIBRARY.with_name(LIBRARY.name+'.tmp')
-    cmd = [hipcc, *HIPCC_FLAGS, '-I', str(ROOT/'include'), *sources, '-o', str(tmp)]
+    cmd = [hipcc, *HIPCC_FLAGS, f'-DTFDTD_SOURCE_DIGEST="{source_digest()}"', '-I', str(ROOT/'include'), *sources,
+           '-o', str(tmp)]
     if verbose:
         print(' '.join(cmd), flush=True)
     subprocess.run(cmd, check=True, cwd=str(PACKAGE))
@@ -104,6 +118,15 @@ class NormGrid(ctypes.Structure):
                 ('inverse_permittivity', vp), ('eps_stride', i32), ('widths', vp*3), ('psi', vp*2), ('psi_count', i64*2)]
 
 
+ADE_MAX_POLES = 16
+
+
+class Ade(ctypes.Structure):
+    _fields_ = [('dtype', i32), ('poles', i32), ('samples', i64), ('per_cell', i32), ('index', vp), ('field', vp),
+                ('old', vp), ('P', vp), ('Q', vp), ('a', ctypes.c_double*ADE_MAX_POLES),
+                ('d', ctypes.c_double*ADE_MAX_POLES), ('k', ctypes.c_double*ADE_MAX_POLES), ('eps_inf', ctypes.c_double)]
+
+
 class DecayRule(ctypes.Structure):
     _fields_ = [('eligible_time', ctypes.c_double), ('time_step', ctypes.c_double), ('decay_threshold', ctypes.c_double),
                 ('growth_limit', ctypes.c_double), ('field_limit', ctypes.c_double), ('min_steps', i32),
@@ -156,6 +179,9 @@ PROTOTYPES = {
     'tfdtd_state_norm': (i32, [vp, i32, i64, i32, vp, vp, vp]),
     'tfdtd_state_norm_noalloc': (i32, [vp, i32, i64, i32, vp, vp]),
     'tfdtd_decay_update': (i32, [vp, vp, vp, i32, i64, vp, i32, vp]),
+    'tfdtd_inject': (i32, [ctypes.POINTER(HalfStep), vp]),
+    'tfdtd_ade_save': (i32, [ctypes.POINTER(Ade), vp]),
+    'tfdtd_ade_correct': (i32, [ctypes.POINTER(Ade), vp]),
     'tfdtd_record_curl': (i32, [ctypes.POINTER(CurlRecord), vp]),
     'tfdtd_adjoint_step': (i32, [ctypes.POINTER(Adjoint), vp]),
     'tfdtd_conv_output': (i32, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), ctypes.POINTER(i32)]),
@@ -164,6 +190,7 @@ PROTOTYPES = {
     'tfdtd_conv_wgrad': (i32, [vp, vp, vp, i32, vp, vp, vp]),
     'tfdtd_gelu_backward': (i32, [vp, vp, i64, i32, vp, vp, vp]),
     'tfdtd_abi_version': (i32, []),
+    'tfdtd_source_digest': (ctypes.c_char_p, []),
     'tfdtd_last_error': (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
     'tfdtd_yee_half_step': (i32, [ctypes.POINTER(HalfStep), vp]),
     'tfdtd_dft_update': (i32, [ctypes.POINTER(Plane), i32, vp]),
@@ -188,6 +215,13 @@ def load_library(path=None):
         fn = getattr(lib, name)
         fn.restype = restype
         fn.argtypes = argtypes
+    # A library compiled from other sources may lay its descriptors out differently from the
+    # ctypes mirror above: refuse it rather than run it.
+    expected = source_digest()
+    built = lib.tfdtd_source_digest().decode()
+    if path is None and expected is not None and built != expected:
+        raise RuntimeError(f'{target} was built from other sources than those beside it (digest {built[:12]}, '
+                           f'sources {expected[:12]}): rebuild with torchfdtd_amd._native.build()')
     if path is None:
         _lib = lib
     return lib

@@ -110,6 +110,18 @@ class DifferentiablePlaneResult:
         return self.poynting()@self.weights
 
 
+def _segment_widths(steps, interval, width):
+    """Graph widths that cover the sweep's segments without single-step replays: the checkpoint
+    interval (or the graph width) and the remainders the run leaves."""
+    widths = set()
+    for span in ((interval, steps-(steps-1)//interval*interval) if interval else (steps,)):
+        if span > 64:
+            widths.update({width, span % width})
+        else:
+            widths.add(span)
+    return tuple(sorted(w for w in widths if w > 0))
+
+
 def _even_interval(steps):
     k = max(2, math.ceil(math.sqrt(steps)))
     return k+(k % 2)
@@ -153,9 +165,10 @@ class _Engine:
             self.records.append(self._record_descriptor(run.H_b, run.psi_e_b, g.psi_buffers[False]))
         width = min(options.cuda_graph_steps, self.steps)
         self.width = width
+        self.widths = _segment_widths(self.steps, self.interval, width)
         self.forward_graphs = StepGraphs(self._step, run.states()+[self.record], self.steps, width,
                                          pair=self._pair if run.one_pass else None,
-                                         prepare=run.refresh_materials if run.one_pass else None)
+                                         prepare=run.refresh_materials if run.one_pass else None, widths=self.widths)
         self.report = dict(adjoint='discrete Yee/CPML (HIP transpose)', steps=self.steps, design_box=[list(b) for b in self.box],
                            checkpoint_interval=self.interval, record_bytes=self.record.numel()*item,
                            forward_kernel='one-pass' if run.one_pass else 'split', observers=len(observers))
@@ -197,12 +210,7 @@ class _Engine:
             N.check(lib.tfdtd_observe_step(ctypes.byref(observe), stream), 'tfdtd_observe_step')
 
     def _advance(self, count):
-        width = self.forward_graphs.width
-        while count >= width:
-            self.forward_graphs.replay(width)
-            count -= width
-        for _ in range(count):
-            self.forward_graphs.replay(1)
+        self.forward_graphs.advance(count)
 
     def _state(self):
         g = self.grid
@@ -279,12 +287,7 @@ class _Engine:
             N.check(self.lib.tfdtd_adjoint_step(ctypes.byref(d), stream), 'tfdtd_adjoint_step')
 
     def _reverse(self, count):
-        width = self.reverse_graphs.width
-        while count >= width:
-            self.reverse_graphs.replay(width)
-            count -= width
-        for _ in range(count):
-            self.reverse_graphs.replay(1)
+        self.reverse_graphs.advance(count)
 
     def backward(self, signal_bar):
         started = time.perf_counter()
@@ -299,7 +302,8 @@ class _Engine:
         seeds = (bar.to(torch.float64)*self.seed_scale).to(self.real).contiguous()
         self.adjoint = self._adjoint_descriptors(seeds)
         states = [self.e_bar, self.h_bar, *self.psi_bar[False], *self.psi_bar[True], self.gradient, self.reverse_counter]
-        self.reverse_graphs = StepGraphs(self._adjoint_step, states, self.steps, self.width, pair=self._adjoint_pair)
+        self.reverse_graphs = StepGraphs(self._adjoint_step, states, self.steps, self.width, pair=self._adjoint_pair,
+                                         widths=self.widths)
         replayed = 0
         if not self.interval:
             self.reverse_counter.fill_(self.steps-1)
@@ -372,6 +376,9 @@ def _admit(project):
         raise ValueError('The HIP adjoint runs real fields.')
     if r.run_control.auto_shutoff:
         raise ValueError('Differentiable simulations require a fixed number of timesteps.')
+    active = {s.material for s in project.structures if s.enabled}
+    if any(m.oscillators for m in project.materials if m.name in active):
+        raise ValueError('The HIP adjoint runs nondispersive materials; dispersive structures are not differentiated.')
     if not torch.cuda.is_available():
         raise RuntimeError('The HIP adjoint requires a ROCm GPU.')
 

@@ -184,6 +184,8 @@ class Cohort:
         self.runs = [Stepper(p, steps=self.steps, kernel='hip', one_pass=None if one_pass else False, tile_y=launch.tile_y,
                              chunk_x=launch.chunk_x, counter=self.counter, epsilon=eps, two_step=False)
                      for p, eps in zip(self.projects, epsilons)]
+        if any(run.dispersive for run in self.runs):
+            raise ValueError('Dispersive materials run through Simulation.run (one grid), not in cohorts.')
         if one_pass and not all(run.one_pass for run in self.runs):
             raise ValueError('The one-pass kernels need 3D grids without periodic axes and at most 8 sources per family.')
         if one_pass and len({run.grid.inverse_permittivity.shape[-1] for run in self.runs}) > 1:

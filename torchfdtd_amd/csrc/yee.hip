@@ -517,6 +517,45 @@ int make_plane(const tfdtd_plane* d, Plane<real, spec>* out) {
     return TFDTD_OK;
 }
 
+// The source boxes of a half step alone: field[cell][comp] += values[counter] in source order, over
+// the union box of the sources (the step's own update is not applied).
+template <typename real>
+__global__ __launch_bounds__(256) void inject_kernel(const HalfStep<real> a) {
+    const int bx = a.box_hi[0] - a.box_lo[0], by = a.box_hi[1] - a.box_lo[1], bz = a.box_hi[2] - a.box_lo[2];
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (long long)bx * by * bz) return;
+    const int z = a.box_lo[2] + (int)(j % bz), y = a.box_lo[1] + (int)((j / bz) % by), x = a.box_lo[0] + (int)(j / ((long long)by * bz));
+    const long long n = *global(a.counter);
+    if (n >= a.source_samples) return;
+    const long long i = ((long long)x * a.ny + y) * a.nz + z;
+    real out[3] = {a.dst[3 * i], a.dst[3 * i + 1], a.dst[3 * i + 2]};
+    bool hit = false;
+    for (int k = 0; k < a.nsrc; ++k) {
+        const Source<real>& box = a.srcs[k];
+        if (x >= box.lo[0] && x < box.hi[0] && y >= box.lo[1] && y < box.hi[1] && z >= box.lo[2] && z < box.hi[2]) {
+            const real v = global(box.values)[n];
+            out[box.comp] = out[box.comp] + v;
+            hit = true;
+        }
+    }
+    if (hit) {
+        a.dst[3 * i] = out[0];
+        a.dst[3 * i + 1] = out[1];
+        a.dst[3 * i + 2] = out[2];
+    }
+}
+
+template <typename real>
+int launch_inject(const tfdtd_half_step* d, hipStream_t stream) {
+    HalfStep<real> a;
+    int status = make_half_step(d, &a);
+    if (status) return status;
+    if (a.nsrc == 0) return TFDTD_OK;
+    const long long count = (long long)(a.box_hi[0] - a.box_lo[0]) * (a.box_hi[1] - a.box_lo[1]) * (a.box_hi[2] - a.box_lo[2]);
+    hipLaunchKernelGGL(inject_kernel<real>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, stream, a);
+    return check_launch("inject");
+}
+
 int half_step(const tfdtd_half_step* d, hipStream_t stream) {
     if (!d) return fail(TFDTD_EINVAL, "null half-step descriptor");
     if (d->family != TFDTD_FAMILY_E && d->family != TFDTD_FAMILY_H) return fail(TFDTD_EINVAL, "family must be E or H");
@@ -658,6 +697,11 @@ int tfdtd_batch_dft_update(const void* device_table, int32_t count, int32_t max_
 
 int tfdtd_abi_version(void) { return TFDTD_ABI_VERSION; }
 
+#ifndef TFDTD_SOURCE_DIGEST
+#define TFDTD_SOURCE_DIGEST "unknown"
+#endif
+const char* tfdtd_source_digest(void) { return TFDTD_SOURCE_DIGEST; }
+
 size_t tfdtd_last_error(char* buffer, size_t size) {
     const std::string& message = tfdtd_detail::last_error;
     if (buffer && size) {
@@ -669,6 +713,14 @@ size_t tfdtd_last_error(char* buffer, size_t size) {
 
 int tfdtd_yee_half_step(const tfdtd_half_step* step, void* stream) {
     return half_step(step, static_cast<hipStream_t>(stream));
+}
+
+int tfdtd_inject(const tfdtd_half_step* step, void* stream) {
+    if (!step) return fail(TFDTD_EINVAL, "null half-step descriptor");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (step->dtype == TFDTD_F32) return launch_inject<float>(step, s);
+    if (step->dtype == TFDTD_F64) return launch_inject<double>(step, s);
+    return fail(TFDTD_EINVAL, "dtype must be float32 or float64");
 }
 
 int tfdtd_dft_update(const tfdtd_plane* p, int32_t count, void* stream) {

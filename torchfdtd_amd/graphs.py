@@ -39,7 +39,9 @@ class StepGraphs:
     before every replay (e.g. Stepper.refresh_materials, which follows permittivity changes).
     """
 
-    def __init__(self, step, states, steps, width=1, warmup=3, pair=None, prepare=None, quad=None):
+    def __init__(self, step, states, steps, width=1, warmup=3, pair=None, prepare=None, quad=None, widths=()):
+        """`widths`: further step counts to capture a graph of each (<= 64), for callers that advance
+        by counts other than 1 and `width` (a checkpoint segment and its remainder)."""
         self.prepare = prepare
         self.width = min(validate_graph_steps(width), steps)
         states = list(states)
@@ -59,7 +61,7 @@ class StepGraphs:
         self.graphs = {}
         self.kernel_steps = dict(two_step=0, one_pass=0, split=0)     # steps replayed through each path
         self.composition = {}
-        counts = {1, self.width}
+        counts = {1, self.width} | {min(int(w), steps) for w in widths if 1 <= int(w) <= 64}
         if quad is not None and self.width > 4:
             counts.add(4)       # one four: replays that stop at any multiple of four stay on the quad path
         for count in sorted(counts):
@@ -90,6 +92,14 @@ class StepGraphs:
             for value in states:
                 value.zero_()
         torch.cuda.synchronize()
+
+    def advance(self, count):
+        """`count` steps by the widest captured graphs first (greedy over the captured widths)."""
+        widths = sorted(self.graphs, reverse=True)
+        while count > 0:
+            width = next(w for w in widths if w <= count)
+            self.replay(width)
+            count -= width
 
     def replay(self, width=1):
         if self.prepare is not None:

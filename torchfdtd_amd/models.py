@@ -49,6 +49,19 @@ class Material(Model):
             raise ValueError('Tensor permittivity is not run by this engine.')
         return self.index**2 if self.model == 'dielectric' else self.epsilon_inf
 
+    @property
+    def oscillators(self):
+        """(w0, strength, damping) in rad/s of each pole (models.py:161-174): Drude (0, wp^2, collision),
+        Lorentz (w0, delta_eps w0^2, 2 linewidth), multipole poles as listed; () for a dielectric."""
+        if self.model == 'drude':
+            return ((0., self.plasma_rad_s**2, self.collision_rad_s),)
+        if self.model == 'lorentz':
+            return ((self.resonance_rad_s, self.delta_epsilon*self.resonance_rad_s**2, 2*self.linewidth_rad_s),)
+        if self.model == 'multipole':
+            return tuple((float(p['resonance_rad_s']), float(p['strength_rad_s_squared']), float(p['damping_rad_s']))
+                         for p in self.poles)
+        return ()
+
 
 class BoundaryFace(Model):
     kind: Literal['pml', 'periodic', 'bloch', 'pec', 'antisymmetric', 'pmc', 'symmetric'] = 'pml'

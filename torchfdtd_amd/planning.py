@@ -311,9 +311,9 @@ def _voxelize_at(project, axes):
             continue
         mask = solid.contains(*np.meshgrid(*(axis[w] for axis, w in zip(axes, window)), indexing='ij', sparse=True))
         material = materials[obj.material]
-        if material.model != 'dielectric':
-            raise ValueError(f'{material.name}: dispersive and tensor materials are not run by this engine')
-        eps[window][mask] = material.instantaneous_epsilon
+        if material.model == 'tensor':
+            raise ValueError(f'{material.name}: tensor materials are not run by this engine')
+        eps[window][mask] = material.instantaneous_epsilon      # epsilon_inf for a dispersive material
         owner[window][mask] = number[obj.material]
         counts[obj.id] = int(np.count_nonzero(mask))
     return eps, owner, counts

@@ -81,7 +81,7 @@ class RunMonitor:
 
     RING = 8     # pinned verdict buffers in flight
 
-    def __init__(self, grids, projects, checks, labels=None):
+    def __init__(self, grids, projects, checks, labels=None, energies=None):
         from .diagnostics import StateNorms
         self.grids, self.projects = list(grids), list(projects)
         count = len(self.grids)
@@ -96,6 +96,8 @@ class RunMonitor:
         self.rows = max(1, int(checks))
         self.table = torch.zeros((count, self.rows, 6), dtype=torch.float64, device=device)
         self.labels = labels or ['']*count
+        # Per grid, None or a function giving further FP64 energy on the device (dispersive oscillators).
+        self.energies = energies or [None]*count
         self.ring = [torch.empty(self.state_bytes, dtype=torch.uint8, pin_memory=True) for _ in range(self.RING)]
         self.pending = deque()      # (step, ring slot, event), oldest first
         self.next_slot = 0
@@ -115,6 +117,9 @@ class RunMonitor:
         """Enqueue the norms, the rule and the verdict copy of barrier `step`."""
         stream = N.stream_handle()
         self.norms.launch()
+        for g, extra in enumerate(self.energies):
+            if extra is not None:
+                self.norms.out[g, 0] += extra()
         N.check(self.lib.tfdtd_decay_update(self.rule_table.data_ptr(), self.states.data_ptr(), self.norms.out.data_ptr(),
                                             len(self.grids), int(step), self.table.data_ptr(), self.rows, stream),
                 'tfdtd_decay_update')

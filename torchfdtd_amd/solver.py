@@ -136,20 +136,82 @@ def pack_table(count_bytes, pack, descriptors, prefix_rows, device):
     return table, torch.as_tensor(prefix_host, device=device), prefix_host
 
 
+class DispersiveMaterial:
+    """ADE state of the samples one Drude/Lorentz material owns (materials.py:29-87, 'ade' PML mode:
+    every owned sample, PML cells included): P and Q per pole, the saved E, and the tfdtd_ade
+    descriptor whose coefficients a, d, k are formed in double as the reference forms them."""
+
+    def __init__(self, grid, material, indices, per_cell):
+        poles = material.oscillators
+        if not 1 <= len(poles) <= N.ADE_MAX_POLES:
+            raise ValueError(f'{material.name}: 1 to {N.ADE_MAX_POLES} poles per material')
+        device, real = grid.E.device, grid.dtype
+        self.material, self.poles = material, poles
+        self.index = torch.as_tensor(np.asarray(indices, dtype=np.int64), device=device)
+        samples = len(indices)*(3 if per_cell else 1)
+        self.P = torch.zeros((len(poles), samples), device=device, dtype=real)
+        self.Q = torch.zeros_like(self.P)
+        self.old = torch.zeros(samples, device=device, dtype=real)
+        grid.memory_states.extend((self.P, self.Q))
+        dt = grid.time_step
+        d = N.Ade()
+        d.dtype = N.F64 if real == torch.float64 else N.F32
+        d.poles, d.samples, d.per_cell = len(poles), samples, int(per_cell)
+        d.index, d.field, d.old = self.index.data_ptr(), grid.E.data_ptr(), self.old.data_ptr()
+        d.P, d.Q = self.P.data_ptr(), self.Q.data_ptr()
+        for j, (w0, strength, gamma) in enumerate(poles):
+            denominator = 1+.5*gamma*dt+.25*(w0*dt)**2
+            d.a[j], d.d[j], d.k[j] = .5*(w0*dt)**2, denominator, strength*dt*dt/(4*denominator)
+        d.eps_inf = material.epsilon_inf
+        self.descriptor = d
+        self.dt = dt
+
+    def energy(self):
+        """FP64 oscillator energy (run_control.py:144-151): sum_j (Q_j/(dt sqrt s_j))^2 + (P_j w0_j/sqrt s_j)^2."""
+        total = torch.zeros((), device=self.P.device, dtype=torch.float64)
+        for j, (w0, strength, _) in enumerate(self.poles):
+            q = self.Q[j].to(torch.float64)/(self.dt*math.sqrt(strength))
+            p = self.P[j].to(torch.float64)*(w0/math.sqrt(strength))
+            total = total+(q*q).sum()+(p*p).sum()
+        return total
+
+
+def dispersive_materials(project, grid, ownership):
+    """One DispersiveMaterial per Drude/Lorentz material that owns samples."""
+    if ownership is None:
+        return []
+    states = []
+    for i, m in enumerate(project.materials):
+        if m.oscillators and np.any(ownership == i):
+            states.append(DispersiveMaterial(grid, m, np.flatnonzero(ownership.reshape(-1) == i), ownership.ndim == 3))
+    if states and project.region.pml_dispersion != 'ade':
+        raise ValueError(f'pml_dispersion="{project.region.pml_dispersion}" is not run by this engine; '
+                         'dispersive materials run with pml_dispersion="ade"')
+    return states
+
+
 class Stepper:
     """Device state of one resident run and its step (the reference's closure, solver.py:776-797)."""
 
     def __init__(self, project, steps=None, kernel=None, one_pass=None, tile_y=8, chunk_x=6, counter=None, epsilon=None,
-                 two_step=None, chunk_x2=20, waves2=12):
+                 two_step=None, chunk_x2=20, waves2=12, launch=None):
         p, r = project, project.region
         self.project = p
         self.steps = r.steps if steps is None else steps
         self.kernel = kernel or resolve_cuda_kernel(r)
         # An explicit relative permittivity (a design already mapped to the grid) replaces the voxelized structures.
-        self.eps = voxelize(p) if epsilon is None else _check_epsilon(epsilon, r)
+        ownership = None
+        if epsilon is None:
+            self.eps, ownership, _ = voxelize(p, with_ownership=True)
+        else:
+            self.eps = _check_epsilon(epsilon, r)
         self.grid = g = YeeGrid(r)
         g.set_permittivity(self.eps)
         self.lib = N.library()
+        # Drude/Lorentz materials: their ADE wraps the E update of the split step.
+        self.dispersive = dispersive_materials(p, g, ownership)
+        if self.dispersive and self.kernel != 'hip':
+            raise ValueError('dispersive materials run on the HIP kernel (cuda_kernel "hip" or "auto")')
         device, real = g.E.device, g.dtype
         # A batch shares one counter among its grids.
         self.counter = torch.zeros(1, device=device, dtype=torch.long) if counter is None else counter
@@ -167,10 +229,11 @@ class Stepper:
         self.planes = [FrequencyPlane(g, p.resolved_monitor(m)) for m in p.monitors if m.enabled and m.kind == 'field']
         if self.kernel == 'torch':
             configure_cuda_kernel(g, 'torch')
-        supported = self.kernel == 'hip' and march_supported(g, self.sources)
+        supported = self.kernel == 'hip' and march_supported(g, self.sources) and not self.dispersive
         self.one_pass = supported if one_pass is None else bool(one_pass)
         if self.one_pass and not supported:
-            raise ValueError('the one-pass kernel needs a 3D grid without periodic axes and at most 8 sources per family')
+            raise ValueError('the one-pass kernel needs a 3D grid without periodic axes, dispersive materials or '
+                             'more than 8 sources per family')
         self.tile_y, self.chunk_x = tile_y, chunk_x
         self.chunk_x2, self.waves2 = chunk_x2, waves2      # two-step pass: planes per chunk, wavefronts per tile
         # Two steps per pass (float32, one-pass grids without plane monitors or traces, whose
@@ -181,6 +244,14 @@ class Stepper:
         # One-byte material index of 1/eps for the one-pass step (grids with few distinct values).
         self.materials = MaterialIndex(g) if self.one_pass else None
         self._describe()
+        # launch='auto': measure the launch shapes of this grid's step path and keep the fastest
+        # (tuning.tune_launch; the record lands in self.launch_record).
+        self.launch_record = None
+        if launch == 'auto':
+            from .tuning import tune_launch
+            self.launch_record = tune_launch(self)
+        elif launch is not None:
+            raise ValueError("launch is None (default shapes) or 'auto' (measured)")
 
     def _two_step_allowed(self):
         return (self.one_pass and self.grid.dtype == torch.float32 and not self.planes and not self.trace_points)
@@ -217,7 +288,12 @@ class Stepper:
     def _describe(self):
         g = self.grid
         dtype = N.F64 if g.dtype == torch.float64 else N.F32
-        self.e_step = half_step_descriptor(g, False, self.sources['E'] if self.kernel == 'hip' else (), self.counter)
+        e_sources = self.sources['E'] if self.kernel == 'hip' else ()
+        if self.dispersive:
+            # The E sources follow the ADE correction: a source-only launch after it.
+            self.e_inject = half_step_descriptor(g, False, e_sources, self.counter)
+            e_sources = ()
+        self.e_step = half_step_descriptor(g, False, e_sources, self.counter)
         self.h_step = half_step_descriptor(g, True, self.sources['H'] if self.kernel == 'hip' else (), self.counter)
         self.plane_table = self._planes(g.E, g.H)
         self.trace_table = self._trace_table(g.E, g.H)
@@ -225,9 +301,11 @@ class Stepper:
         if self.one_pass:
             if self.materials is not None:
                 self.materials.refresh()     # the permittivity may have been replaced (a widened layout)
-            # Second buffer set of the one-pass step: time n is read from one set, n+1 written to the other.
-            self.E_b, self.H_b = torch.empty_like(g.E), torch.empty_like(g.H)
-            self.psi_e_b = torch.zeros_like(g.psi_buffers[False])
+            # Second buffer set of the one-pass step: time n is read from one set, n+1 written to the other
+            # (allocated once; a retile keeps it).
+            if getattr(self, 'E_b', None) is None:
+                self.E_b, self.H_b = torch.empty_like(g.E), torch.empty_like(g.H)
+                self.psi_e_b = torch.zeros_like(g.psi_buffers[False])
             psi_a = g.psi_buffers[False]
             self.march = [march_descriptor(g, g.E, g.H, self.E_b, self.H_b, psi_a, self.psi_e_b, self.sources, self.counter,
                                            self.tile_y, self.chunk_x, self.materials),
@@ -242,7 +320,8 @@ class Stepper:
                 self.march_tables.append((table, prefix, (ctypes.c_int32*3)(*host[:, 1].tolist())))
             if self.two_step:
                 # The two-step pass double-buffers the H-family CPML memory as well.
-                self.psi_h_b = torch.zeros_like(g.psi_buffers[True])
+                if getattr(self, 'psi_h_b', None) is None:
+                    self.psi_h_b = torch.zeros_like(g.psi_buffers[True])
                 psi_h = g.psi_buffers[True]
                 self.march2 = [march_descriptor(g, g.E, g.H, self.E_b, self.H_b, psi_a, self.psi_e_b, self.sources,
                                                 self.counter, self.waves2, self.chunk_x2, self.materials,
@@ -254,6 +333,25 @@ class Stepper:
             self.plane_table_b = self._planes(self.E_b, self.H_b)
             self.trace_table_b = self._trace_table(self.E_b, self.H_b)
             self.observe_b = self._observe(self.trace_table_b)
+
+    def oscillator_energy(self):
+        """FP64 energy of every dispersive material's oscillators (a device scalar)."""
+        return sum(m.energy() for m in self.dispersive)
+
+    def retile(self, tile_y=None, chunk_x=None, waves2=None, chunk_x2=None):
+        """Rebuild the step's descriptors for other launch shapes (rows per one-pass tile, planes per
+        chunk, wavefronts per two-step tile, planes per two-step chunk); the state is kept.  Graphs
+        captured before hold the old descriptors and must be captured again."""
+        if torch.cuda.is_current_stream_capturing():
+            raise RuntimeError('launch shapes change outside graph capture only')
+        self.tile_y = self.tile_y if tile_y is None else tile_y
+        self.chunk_x = self.chunk_x if chunk_x is None else chunk_x
+        self.waves2 = self.waves2 if waves2 is None else waves2
+        self.chunk_x2 = self.chunk_x2 if chunk_x2 is None else chunk_x2
+        self._describe()
+
+    def launch_shape(self):
+        return dict(tile_y=self.tile_y, chunk_x=self.chunk_x, waves2=self.waves2, chunk_x2=self.chunk_x2)
 
     def _march2_table(self, d):
         table, prefix, host = pack_table(lambda n: self.lib.tfdtd_batch_march_bytes(n, N.F32),
@@ -344,6 +442,9 @@ class Stepper:
 
     def step(self):
         stream = N.stream_handle(self.grid.device)
+        if self.dispersive:
+            self._dispersive_step(stream)
+            return
         if self.kernel == 'hip':
             N.check(self.lib.tfdtd_step_fused(ctypes.byref(self.e_step), ctypes.byref(self.h_step), self.plane_table,
                                               len(self.planes), ctypes.byref(self.observe), 1, stream), 'tfdtd_step_fused')
@@ -357,6 +458,21 @@ class Stepper:
             N.check(self.lib.tfdtd_dft_update(self.plane_table, len(self.planes), stream), 'tfdtd_dft_update')
         N.check(self.lib.tfdtd_observe_step(ctypes.byref(self.observe), stream), 'tfdtd_observe_step')
 
+    def _dispersive_step(self, stream):
+        """The reference's order with MaterialADE (cuda_kernels.py:355-362, solver.py:785-797): save E at
+        the owned samples, E update, ADE correction, E sources, H update and sources, planes, traces."""
+        lib = self.lib
+        for m in self.dispersive:
+            N.check(lib.tfdtd_ade_save(ctypes.byref(m.descriptor), stream), 'tfdtd_ade_save')
+        N.check(lib.tfdtd_yee_half_step(ctypes.byref(self.e_step), stream), 'tfdtd_yee_half_step')
+        for m in self.dispersive:
+            N.check(lib.tfdtd_ade_correct(ctypes.byref(m.descriptor), stream), 'tfdtd_ade_correct')
+        N.check(lib.tfdtd_inject(ctypes.byref(self.e_inject), stream), 'tfdtd_inject')
+        N.check(lib.tfdtd_yee_half_step(ctypes.byref(self.h_step), stream), 'tfdtd_yee_half_step')
+        if self.planes:
+            N.check(lib.tfdtd_dft_update(self.plane_table, len(self.planes), stream), 'tfdtd_dft_update')
+        N.check(lib.tfdtd_observe_step(ctypes.byref(self.observe), stream), 'tfdtd_observe_step')
+
     def _inject(self, field, family):
         for comp, box, values in self.sources[family]:
             field[box+(comp,)] += values.index_select(0, self.counter)[0]
@@ -369,11 +485,12 @@ class Simulation:
         self.project = Project.model_validate(project.model_dump())
         self.epsilon = epsilon
 
-    def run(self, progress=None, cancel=None, cuda_graph=True, cuda_graph_steps=1):
+    def run(self, progress=None, cancel=None, cuda_graph=True, cuda_graph_steps=1, *, launch=None):
+        """launch='auto' measures the step's launch shapes on this grid first (tuning.tune_launch)."""
         with ENGINE_LOCK, torch.no_grad():
-            return self._run(progress, cancel, cuda_graph, cuda_graph_steps)
+            return self._run(progress, cancel, cuda_graph, cuda_graph_steps, launch)
 
-    def _run(self, progress, cancel, cuda_graph, cuda_graph_steps):
+    def _run(self, progress, cancel, cuda_graph, cuda_graph_steps, launch=None):
         p, r = self.project, self.project.region
         refusal = r.unsupported()
         if refusal:
@@ -386,10 +503,11 @@ class Simulation:
             raise RuntimeError('The HIP engine requires a ROCm GPU; none is visible.')
         validate_graph_steps(cuda_graph_steps, cuda_graph)
         started = time.perf_counter()
-        run = Stepper(p, epsilon=self.epsilon)
+        run = Stepper(p, epsilon=self.epsilon, launch=launch)
         g = run.grid
         control = r.run_control
-        monitor = RunMonitor([g], [p], check_count(p)) if control.auto_shutoff or control.divergence_check else None
+        monitor = (RunMonitor([g], [p], check_count(p), energies=[run.oscillator_energy if run.dispersive else None])
+                   if control.auto_shutoff or control.divergence_check else None)
         graph = (StepGraphs(run.step, run.states(), r.steps, cuda_graph_steps, pair=run.step_pair if run.one_pass else None,
                             prepare=run.refresh_materials if run.one_pass else None,
                             quad=run.quad if run.two_step else None)
@@ -451,6 +569,7 @@ class Simulation:
             raise FloatingPointError('Non-finite fields detected.')
         kernel_steps = dict(graph.kernel_steps) if graph is not None else dict(two_step=0, one_pass=0, split=eager_steps)
         summary = run_summary(p, run, e, completed, termination, seconds, setup_seconds, kernel_steps=kernel_steps,
+                              launch=run.launch_shape() if run.one_pass else None, launch_tuning=run.launch_record,
                               cuda_graph=graph is not None, cuda_graph_steps=width, cuda_graph_replays=replays,
                               diagnostics=monitor.history(0) if monitor is not None else [],
                               diagnostic_seconds=diagnostic_seconds,
