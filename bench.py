@@ -109,7 +109,8 @@ def batched_line(steps, count, loop_grids, launch='tune', width=50):
     cells = sum(math.prod(c.project.region.shape) for c in cases)
     print(f'bench: uneven batch, {count} grids, {cells} cells', file=sys.stderr, flush=True)
     if launch == 'tune':
-        tuning = tune_tensor_batch(cases, candidates=(Launch('split'), Launch('march', 8, 6), Launch('march', 8, 8),
+        tuning = tune_tensor_batch(cases, candidates=(Launch('split'), Launch('march', 8, 2), Launch('march', 8, 3),
+                                                      Launch('march', 8, 4), Launch('march', 8, 6), Launch('march', 16, 3),
                                                       Launch('march', 16, 6)), repeats=2, trial_steps=100)
         chosen, timings = tuning.launch, {k: round(v['median_seconds'], 5) for k, v in tuning.timings.items()}
     else:

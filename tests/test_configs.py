@@ -41,7 +41,7 @@ def test_cube_128_two_step_graphs_match_oracle_bitwise(gpu):
     n, steps = 128, 44
     project = cube(n=n, steps=steps)
     mask = cube_noise_mask(n)
-    run = Stepper(project)
+    run = Stepper(project, two_step=True)
     assert run.two_step and run.one_pass
     run.grid.inverse_permittivity.copy_(cube_inverse_permittivity(mask)[..., None])
     graphs = StepGraphs(run.step, run.states(), steps, 20, warmup=1, pair=run.step_pair, prepare=run.refresh_materials,

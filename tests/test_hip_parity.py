@@ -194,7 +194,7 @@ def test_two_step_pass_with_sources_equals_one_step_pairs_under_graphs(gpu):
     mask = cube_noise_mask(96)
     out = []
     for two_step in (False, True):
-        run = Stepper(cube(n=96, steps=60), two_step=two_step)
+        run = Stepper(cube(n=96, steps=60), one_pass=True, two_step=two_step)
         assert run.two_step == two_step
         run.grid.inverse_permittivity.copy_(cube_inverse_permittivity(mask)[..., None])
         graphs = StepGraphs(run.step, run.states(), 60, 23, warmup=1, pair=run.step_pair,

@@ -22,16 +22,19 @@ def advance(run, fours):
     for _ in range(fours):
         if run.two_step:
             run.quad()
+        elif run.one_pass:
+            run.step_pair()
+            run.step_pair()
         else:
-            run.step_pair()
-            run.step_pair()
+            for _ in range(4):
+                run.step()
     torch.cuda.synchronize()
 
 
 @pytest.mark.parametrize('path', ['two_step', 'one_pass'])
 def test_tuned_launch_keeps_the_state_and_the_physics(gpu, path):
     from torchfdtd_amd import tuning
-    kw = {} if path == 'two_step' else dict(two_step=False)
+    kw = dict(one_pass=True) if path == 'two_step' else dict(one_pass=True, two_step=False)
     tuned, plain = cube_stepper(**kw), cube_stepper(**kw)
     assert tuned.two_step == (path == 'two_step')
     advance(tuned, 2)
@@ -55,14 +58,34 @@ def test_choice_is_cached_and_simulation_accepts_launch_auto(gpu):
     from torchfdtd_amd import Simulation, tuning
     from torchfdtd_amd.workloads import cube
     run = cube_stepper(n=64)
+    assert run.path == 'split' and run.paths() == ['split', 'one_pass', 'two_step']    # small grids: split by default
     first = tuning.tune_launch(run, sweeps=2, units=1)
-    assert not first['cached'] and len(first['median_ms']) == len(tuning.TWO_STEP)
+    assert not first['cached'] and len(first['median_ms']) == len(tuning.candidates_for(run))
+    assert len(first['median_ms']) == 1+len(tuning.ONE_PASS)+len(tuning.TWO_STEP) and not first['rejected']
+    assert run.path == first['path']
     again = tuning.tune_launch(cube_stepper(n=64))
-    assert again['cached'] and again['chosen'] == first['chosen']
+    assert again['cached'] and again['chosen'] == first['chosen'] and again['path'] == first['path']
     project = cube(n=48, steps=24)
     auto = Simulation(project).run(cuda_graph_steps=8, launch='auto')
     default = Simulation(project).run(cuda_graph_steps=8)
     np.testing.assert_array_equal(auto.electric, default.electric)
     np.testing.assert_array_equal(auto.magnetic, default.magnetic)
-    assert auto.summary['launch_tuning']['path'] == 'two_step' and auto.summary['steps_per_pass'] == 2
-    assert default.summary['launch_tuning'] is None
+    per_pass = {'split': 0, 'one_pass': 1, 'two_step': 2}
+    assert auto.summary['steps_per_pass'] == per_pass[auto.summary['launch_tuning']['path']]
+    assert default.summary['launch_tuning'] is None and default.summary['steps_per_pass'] == 0
+
+
+def test_every_path_continues_a_run_bitwise(gpu):
+    """set_path between whole units: a run that changes path every few steps equals one that never does."""
+    mixed, plain = cube_stepper(n=40, two_step=False), cube_stepper(n=40, two_step=False)
+    assert mixed.path == plain.path == 'split'
+    for path in ('one_pass', 'two_step', 'split', 'two_step', 'one_pass'):
+        mixed.set_path(path)
+        advance(mixed, 1)
+        for _ in range(4):
+            plain.step()
+    torch.cuda.synchronize()
+    assert int(mixed.counter.item()) == int(plain.counter.item()) == 20
+    for a, b in zip((mixed.grid.E, mixed.grid.H, *mixed.grid.memory_states),
+                    (plain.grid.E, plain.grid.H, *plain.grid.memory_states)):
+        assert torch.equal(a, b)

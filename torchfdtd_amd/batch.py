@@ -130,8 +130,9 @@ class Launch:
         return dict(kernel=self.kernel, tile_y=self.tile_y, chunk_x=self.chunk_x)
 
 
-DEFAULT_CANDIDATES = (Launch('split'), Launch('march', 8, 4), Launch('march', 8, 6), Launch('march', 8, 8),
-                      Launch('march', 8, 16), Launch('march', 8, 0), Launch('march', 16, 6), Launch('march', 16, 16))
+DEFAULT_CANDIDATES = (Launch('split'), Launch('march', 8, 2), Launch('march', 8, 3), Launch('march', 8, 4),
+                      Launch('march', 8, 6), Launch('march', 8, 8), Launch('march', 8, 16), Launch('march', 8, 0),
+                      Launch('march', 16, 3), Launch('march', 16, 6), Launch('march', 16, 16))
 
 
 def _cases(cases):
@@ -181,13 +182,17 @@ class Cohort:
         device = torch.device('cuda', torch.cuda.current_device())
         self.counter = torch.zeros(1, device=device, dtype=torch.long)
         one_pass = launch.kernel == 'march'
-        self.runs = [Stepper(p, steps=self.steps, kernel='hip', one_pass=None if one_pass else False, tile_y=launch.tile_y,
+        self.runs = [Stepper(p, steps=self.steps, kernel='hip', one_pass=False, tile_y=launch.tile_y,
                              chunk_x=launch.chunk_x, counter=self.counter, epsilon=eps, two_step=False)
                      for p, eps in zip(self.projects, epsilons)]
         if any(run.dispersive for run in self.runs):
             raise ValueError('Dispersive materials run through Simulation.run (one grid), not in cohorts.')
-        if one_pass and not all(run.one_pass for run in self.runs):
-            raise ValueError('The one-pass kernels need 3D grids without periodic axes and at most 8 sources per family.')
+        if one_pass:
+            # A cohort's many grids fill the GPU together: the one-pass kernels serve grids of any size.
+            if not all(run.one_pass_supported for run in self.runs):
+                raise ValueError('The one-pass kernels need 3D grids without periodic axes and at most 8 sources per family.')
+            for run in self.runs:
+                run.set_path('one_pass')
         if one_pass and len({run.grid.inverse_permittivity.shape[-1] for run in self.runs}) > 1:
             # One launch serves one permittivity layout: widen the per-cell grids to per component
             # (the same value three times, so every update is unchanged).

@@ -75,6 +75,34 @@ __device__ __forceinline__ real transpose_row(real d, const Family<real>& F, int
     return d * ik + c * m;
 }
 
+// The cell of this thread in the transpose launches: lanes along z, the block's four wavefronts
+// along y, blocks over (z tile of 64, y tile of 4, x) -- the block's coordinates come from uniform
+// 32-bit arithmetic, so no lane divides 64-bit indices.
+constexpr int kTileZ = 64, kTileY = 4;
+
+struct Cell3 {
+    int x, y, z;
+    long long j;
+    bool inside;
+};
+
+__device__ __forceinline__ Cell3 locate(const int n[3]) {
+    const int zt = (n[2] + kTileZ - 1) / kTileZ, yt = (n[1] + kTileY - 1) / kTileY;
+    const int b = blockIdx.x;
+    const int r = b / zt;
+    Cell3 c;
+    c.z = (b - r * zt) * kTileZ + (int)(threadIdx.x & (kTileZ - 1));
+    c.x = r / yt;
+    c.y = (r - c.x * yt) * kTileY + (int)(threadIdx.x / kTileZ);
+    c.inside = c.x < n[0] && c.y < n[1] && c.z < n[2];
+    c.j = ((long long)c.x * n[1] + c.y) * n[2] + c.z;
+    return c;
+}
+
+inline unsigned transpose_blocks(const int n[3]) {
+    return (unsigned)n[0] * (unsigned)((n[1] + kTileY - 1) / kTileY) * (unsigned)((n[2] + kTileZ - 1) / kTileZ);
+}
+
 template <typename real>
 __global__ __launch_bounds__(256) void seed_kernel(const Adjoint<real> a) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -87,38 +115,49 @@ __global__ __launch_bounds__(256) void seed_kernel(const Adjoint<real> a) {
     field[a.seed_targets[2 * k + 1]] += sum;
 }
 
+// Both transposes read every field value they need (the cell and its neighbour along each axis)
+// into registers before the first memory-array store: stores through psi_out would otherwise keep
+// the compiler from reusing or hoisting later field loads.
 template <typename real>
 __global__ __launch_bounds__(256) void h_transpose_kernel(const Adjoint<real> a) {
-    const long long cells = (long long)a.n[0] * a.n[1] * a.n[2];
-    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cells) return;
-    const int z = (int)(j % a.n[2]), y = (int)((j / a.n[2]) % a.n[1]), x = (int)(j / ((long long)a.n[1] * a.n[2]));
+    const Cell3 cell = locate(a.n);
+    if (!cell.inside) return;
+    const int x = cell.x, y = cell.y, z = cell.z;
+    const long long j = cell.j;
     const int p[3] = {x, y, z};
     const real hcoef = a.S * a.inv_mu[0];
-    real ob[3];
+    real ob[3], down[3][3], eb[3];
     for (int o = 0; o < 3; ++o) ob[o] = -hcoef * a.h_bar[3 * j + o];
+    for (int c = 0; c < 3; ++c) eb[c] = a.e_bar[3 * j + c];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        // Row i-1 of the axis, or row n-1 across the periodic wrap at i = 0.
+        const int na = a.n[ax], i = p[ax];
+        const long long stride = ax == 0 ? (long long)a.n[1] * a.n[2] : ax == 1 ? a.n[2] : 1;
+        const bool has = na > 1 && (i >= 1 || (a.periodic >> ax & 1));
+        const long long nb = i >= 1 ? j - stride : j + (long long)(na - 1) * stride;
+        for (int o = 0; o < 3; ++o) down[ax][o] = has ? -hcoef * a.h_bar[3 * nb + o] : real(0);
+    }
     real acc[3] = {real(0), real(0), real(0)};
 #pragma unroll
     for (int t = 0; t < 6; ++t) {
         const int ax = term_axis(t), c = term_in(t), o = term_out(t), s = term_sign(t);
         const int na = a.n[ax], i = p[ax];
         if (na == 1) continue;
-        const long long stride = ax == 0 ? (long long)a.n[1] * a.n[2] : ax == 1 ? a.n[2] : 1;
         // Forward differences: rows 0 .. n-2 carry a derivative; E_c[i+1] += d_D[i], E_c[i] -= d_D[i].
         if (i <= na - 2) {
             const real d = transpose_row(real(s) * ob[o], a.hf, ax, t & 1, i, a.n, x, y, z, true);
             acc[c] = acc[c] - d;
         }
         if (i >= 1) {
-            const real prev = -hcoef * a.h_bar[3 * (j - stride) + o];
             int q[3] = {x, y, z};
             q[ax] = i - 1;
-            const real d = transpose_row(real(s) * prev, a.hf, ax, t & 1, i - 1, a.n, q[0], q[1], q[2], false);
+            const real d = transpose_row(real(s) * down[ax][o], a.hf, ax, t & 1, i - 1, a.n, q[0], q[1], q[2], false);
             acc[c] = acc[c] + d;
         }
         if (a.periodic >> ax & 1) {
             // Wrap term at row n-1: E_c[0] += s*ob[n-1], E_c[n-1] -= s*ob[n-1].
-            if (i == 0) acc[c] = acc[c] + real(s) * (-hcoef * a.h_bar[3 * (j + (long long)(na - 1) * stride) + o]);
+            if (i == 0) acc[c] = acc[c] + real(s) * down[ax][o];
             if (i == na - 1) acc[c] = acc[c] - real(s) * ob[o];
         } else if ((a.pec >> ax & 1) && i == na - 1) {
             acc[c] = acc[c] - real(s) * a.pec_factor[ax] * ob[o];
@@ -126,7 +165,7 @@ __global__ __launch_bounds__(256) void h_transpose_kernel(const Adjoint<real> a)
     }
     real e[3];
     for (int c = 0; c < 3; ++c) {
-        e[c] = a.e_bar[3 * j + c] + acc[c];
+        e[c] = eb[c] + acc[c];
         a.e_bar[3 * j + c] = e[c];
     }
     const long long t = *a.counter;
@@ -147,41 +186,52 @@ __global__ __launch_bounds__(256) void h_transpose_kernel(const Adjoint<real> a)
 
 template <typename real>
 __global__ __launch_bounds__(256) void e_transpose_kernel(const Adjoint<real> a) {
-    const long long cells = (long long)a.n[0] * a.n[1] * a.n[2];
-    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) *a.counter = *a.counter - 1;
-    if (j >= cells) return;
-    const int z = (int)(j % a.n[2]), y = (int)((j / a.n[2]) % a.n[1]), x = (int)(j / ((long long)a.n[1] * a.n[2]));
+    const Cell3 cell = locate(a.n);
+    if (!cell.inside) return;
+    const int x = cell.x, y = cell.y, z = cell.z;
+    const long long j = cell.j;
     const int p[3] = {x, y, z};
-    auto obar = [&](long long cell, int o) {
-        const real inv = a.inv_eps[a.eps_stride == 3 ? 3 * cell + o : cell];
-        return (a.S * inv) * a.e_bar[3 * cell + o];
+    auto obar = [&](long long at, int o) {
+        const real inv = a.inv_eps[a.eps_stride == 3 ? 3 * at + o : at];
+        return (a.S * inv) * a.e_bar[3 * at + o];
     };
+    real own[3], up[3][3], hb[3];
+    for (int o = 0; o < 3; ++o) own[o] = obar(j, o);
+    for (int c = 0; c < 3; ++c) hb[c] = a.h_bar[3 * j + c];
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        // Row i+1 of the axis, or row 0 across the periodic wrap at i = n-1.
+        const int na = a.n[ax], i = p[ax];
+        const long long stride = ax == 0 ? (long long)a.n[1] * a.n[2] : ax == 1 ? a.n[2] : 1;
+        const bool has = na > 1 && (i + 1 <= na - 1 || (a.periodic >> ax & 1));
+        const long long nb = i + 1 <= na - 1 ? j + stride : j - (long long)(na - 1) * stride;
+        for (int o = 0; o < 3; ++o) up[ax][o] = has ? obar(nb, o) : real(0);
+    }
     real acc[3] = {real(0), real(0), real(0)};
 #pragma unroll
     for (int t = 0; t < 6; ++t) {
         const int ax = term_axis(t), c = term_in(t), o = term_out(t), s = term_sign(t);
         const int na = a.n[ax], i = p[ax];
         if (na == 1) continue;
-        const long long stride = ax == 0 ? (long long)a.n[1] * a.n[2] : ax == 1 ? a.n[2] : 1;
         // Backward differences: rows 1 .. n-1 carry a derivative; H_c[i] += d_D[i], H_c[i-1] -= d_D[i].
         if (i >= 1) {
-            const real d = transpose_row(real(s) * obar(j, o), a.ef, ax, t & 1, i, a.n, x, y, z, true);
+            const real d = transpose_row(real(s) * own[o], a.ef, ax, t & 1, i, a.n, x, y, z, true);
             acc[c] = acc[c] + d;
         }
         if (i + 1 <= na - 1) {
             int q[3] = {x, y, z};
             q[ax] = i + 1;
-            const real d = transpose_row(real(s) * obar(j + stride, o), a.ef, ax, t & 1, i + 1, a.n, q[0], q[1], q[2], false);
+            const real d = transpose_row(real(s) * up[ax][o], a.ef, ax, t & 1, i + 1, a.n, q[0], q[1], q[2], false);
             acc[c] = acc[c] - d;
         }
         if (a.periodic >> ax & 1) {
             // Wrap term at row 0: H_c[0] += s*ob[0], H_c[n-1] -= s*ob[0].
-            if (i == 0) acc[c] = acc[c] + real(s) * obar(j, o);
-            if (i == na - 1) acc[c] = acc[c] - real(s) * obar(j - (long long)(na - 1) * stride, o);
+            if (i == 0) acc[c] = acc[c] + real(s) * own[o];
+            if (i == na - 1) acc[c] = acc[c] - real(s) * up[ax][o];
         }
     }
-    for (int c = 0; c < 3; ++c) a.h_bar[3 * j + c] = a.h_bar[3 * j + c] + acc[c];
+    for (int c = 0; c < 3; ++c) a.h_bar[3 * j + c] = hb[c] + acc[c];
 }
 
 template <typename real>
@@ -345,8 +395,10 @@ static int launch_adjoint(const tfdtd_adjoint* d, hipStream_t s) {
         a.modulus = 1;
     }
     a.counter = reinterpret_cast<long long*>(d->counter);
-    const long long cells = (long long)dims[0] * dims[1] * dims[2];
-    const unsigned blocks = (unsigned)((cells + 255) / 256);
+    const unsigned long long tiles = (unsigned long long)dims[0] * ((dims[1] + kTileY - 1) / kTileY) *
+                                     ((dims[2] + kTileZ - 1) / kTileZ);
+    if (tiles > 0x7fffffffULL) return tfdtd_fail(TFDTD_EINVAL, "adjoint: grid too large for one launch");
+    const unsigned blocks = transpose_blocks(dims);
     if (a.nseed > 0) {
         hipLaunchKernelGGL(seed_kernel<real>, dim3((unsigned)((a.nseed + 255) / 256)), dim3(256), 0, s, a);
         status = tfdtd_check_launch("adjoint seeds");

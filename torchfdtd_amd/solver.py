@@ -32,6 +32,10 @@ from .run_control import RunMonitor, check_count
 
 C0 = 299792458.0
 ENGINE_LOCK = threading.Lock()
+# Grids of fewer cells take the split step by default: measured on MI355X, the split step beats the
+# best one-pass shape by 19-47% on 2.0-2.3 M-cell grids with 10-12 CPML cells, and loses by 10% at
+# 256^3 (16.8 M cells).  tuning.tune_launch measures every path on the grid itself.
+ONE_PASS_MIN_CELLS = 8_000_000
 
 
 @dataclass
@@ -229,8 +233,12 @@ class Stepper:
         self.planes = [FrequencyPlane(g, p.resolved_monitor(m)) for m in p.monitors if m.enabled and m.kind == 'field']
         if self.kernel == 'torch':
             configure_cuda_kernel(g, 'torch')
-        supported = self.kernel == 'hip' and march_supported(g, self.sources) and not self.dispersive
-        self.one_pass = supported if one_pass is None else bool(one_pass)
+        self.one_pass_supported = supported = (self.kernel == 'hip' and march_supported(g, self.sources)
+                                               and not self.dispersive)
+        # Default path by size: below ONE_PASS_MIN_CELLS the split step is the faster one (the one-pass
+        # tiles of a small grid are mostly CPML boundary tiles and too few to fill the GPU).
+        large = math.prod(r.shape) >= ONE_PASS_MIN_CELLS
+        self.one_pass = supported and (large or bool(two_step)) if one_pass is None else bool(one_pass)
         if self.one_pass and not supported:
             raise ValueError('the one-pass kernel needs a 3D grid without periodic axes, dispersive materials or '
                              'more than 8 sources per family')
@@ -255,6 +263,33 @@ class Stepper:
 
     def _two_step_allowed(self):
         return (self.one_pass and self.grid.dtype == torch.float32 and not self.planes and not self.trace_points)
+
+    def paths(self):
+        """The step paths this grid can take: 'split' always; 'one_pass' on one-pass grids; 'two_step'
+        on float32 one-pass grids without plane monitors or point traces."""
+        out = ['split']
+        if self.one_pass_supported:
+            out.append('one_pass')
+            if self.grid.dtype == torch.float32 and not self.planes and not self.trace_points:
+                out.append('two_step')
+        return out
+
+    @property
+    def path(self):
+        return 'two_step' if self.two_step else 'one_pass' if self.one_pass else 'split'
+
+    def set_path(self, path):
+        """Switch the step path between whole units (the state in the grid's own buffers, as after any
+        step_pair, quad or single step of the split path); graphs captured before must be captured again."""
+        if path not in self.paths():
+            raise ValueError(f'path {path!r} is not available on this grid (available: {self.paths()})')
+        if torch.cuda.is_current_stream_capturing():
+            raise RuntimeError('the step path changes outside graph capture only')
+        self.one_pass = path != 'split'
+        self.two_step = path == 'two_step'
+        if self.one_pass and self.materials is None:
+            self.materials = MaterialIndex(self.grid)
+        self._describe()
 
     def set_trace_points(self, points):
         """Replace the recorded traces by `points`: (magnetic, flat element index) pairs."""

@@ -1,20 +1,20 @@
-"""Measured launch shapes for one grid (the reference's select_launch / tune_yee,
+"""Measured step path and launch shape for one grid (the reference's select_launch / tune_yee,
 torchfdtd/reversible_cuda_tuning.py:84-229).
 
-`tune_launch(stepper)` times every candidate launch shape of the grid's step path -- the two-step
-pass (wavefronts per tile, planes per x chunk) for float32 grids without per-step observers, else
-the one-pass step (rows per tile, planes per x chunk) -- and installs the fastest.  As in the
-reference:
+`tune_launch(stepper)` times every candidate the grid can take -- the split step, the one-pass step
+(rows per tile, planes per x chunk) on one-pass grids, the two-step pass (wavefronts per tile, planes
+per x chunk) on float32 one-pass grids without per-step observers -- and installs the fastest.  As
+in the reference:
 
 * every candidate must give bitwise-identical physics: each advances the same saved state by one
-  unit (four steps of the two-step pass, two of the one-pass step) and its fields and CPML memory
-  must equal the first candidate's bit for bit, or it is rejected;
+  unit of four steps (four split steps, two one-pass pairs or one two-step quad) and its fields and
+  CPML memory must equal the first candidate's bit for bit, or it is rejected;
 * timing runs on the grid's own buffers from a saved copy of the state, which is restored
   afterwards, so the run continues from where it was; sweeps alternate the candidate order and the
   median of each candidate's samples decides;
-* tuning must precede graph capture (graphs hold the descriptors of one shape);
+* tuning must precede graph capture (graphs hold the descriptors of one path and shape);
 * results are cached (LRU, 128 entries) by device, grid shape, precision, CPML layout, permittivity
-  layout, source count and path, and a cached shape is installed without timing.
+  layout, source count and the paths available, and a cached choice is installed without timing.
 
 The z extent of a tile is not a candidate: a tile's z column is one wavefront of 64 lanes (61 owned
 cells in the two-step pass), the layout every variant's LDS staging is built around.
@@ -27,7 +27,9 @@ from collections import OrderedDict
 import torch
 
 TWO_STEP = tuple((w, c) for w in (12, 16) for c in (12, 16, 20, 28))
-ONE_PASS = tuple((t, c) for t in (8, 16) for c in (4, 6, 8))
+ONE_PASS = tuple((t, c) for t in (8, 16) for c in (2, 3, 4, 6, 8))
+NAMES = {'one_pass': ('tile_y', 'chunk_x'), 'two_step': ('waves2', 'chunk_x2'), 'split': ()}
+UNIT_STEPS = 4
 _CACHE: OrderedDict = OrderedDict()
 _CACHE_SIZE = 128
 
@@ -36,14 +38,49 @@ def _key(stepper):
     g = stepper.grid
     faces = tuple(sorted((f, a, c, e['lo'], e['hi']) for (f, a, c), entries in g.cpml.items() for e in entries))
     return (torch.cuda.get_device_name(g.device), tuple(g.E.shape), str(g.dtype), g.inverse_permittivity.shape[-1],
-            faces, len(stepper.sources['E']), len(stepper.sources['H']), stepper.two_step, stepper.one_pass,
-            stepper.materials is not None and stepper.materials.count <= 256)
+            faces, len(stepper.sources['E']), len(stepper.sources['H']), tuple(stepper.paths()),
+            len(stepper.planes), len(stepper.trace_points) > 0)
+
+
+def candidates_for(stepper):
+    """Every (path, shape) the grid can take, split first."""
+    paths = stepper.paths()
+    out = [('split', {})]
+    if 'one_pass' in paths:
+        out += [('one_pass', dict(zip(NAMES['one_pass'], c))) for c in ONE_PASS]
+    if 'two_step' in paths:
+        out += [('two_step', dict(zip(NAMES['two_step'], c))) for c in TWO_STEP]
+    return out
+
+
+def _normalise(stepper, candidates):
+    """Candidates as (path, shape) pairs: plain pairs of numbers are shapes of the stepper's current path."""
+    out = []
+    for c in candidates:
+        if isinstance(c[0], str):
+            out.append((c[0], dict(c[1]) if len(c) > 1 else {}))
+        else:
+            out.append((stepper.path, dict(zip(NAMES[stepper.path], c))))
+    return out
 
 
 def _unit(stepper):
-    """One timing unit: a quad of the two-step pass or a pair of one-pass steps (the state returns to
-    the grid's own buffers)."""
-    return (stepper.quad, 4) if stepper.two_step else (stepper.step_pair, 2)
+    """Four steps on the stepper's current path; the state returns to the grid's own buffers."""
+    if stepper.two_step:
+        stepper.quad()
+    elif stepper.one_pass:
+        stepper.step_pair()
+        stepper.step_pair()
+    else:
+        for _ in range(UNIT_STEPS):
+            stepper.step()
+
+
+def _install(stepper, path, shape):
+    if stepper.path != path:
+        stepper.set_path(path)
+    if shape:
+        stepper.retile(**shape)
 
 
 def _state(stepper):
@@ -52,28 +89,26 @@ def _state(stepper):
 
 
 def tune_launch(stepper, candidates=None, sweeps=5, units=2, report=None):
-    """Install the fastest bitwise-identical launch shape of `stepper`'s path and return its record:
+    """Install the fastest bitwise-identical (path, launch shape) of `stepper` and return its record:
     {'path', 'chosen', 'median_ms' per candidate (per step), 'rejected', 'cached'}."""
     if torch.cuda.is_current_stream_capturing():
         raise RuntimeError('launch tuning must precede graph capture')
-    if not stepper.one_pass:
-        record = dict(path='split', chosen=None, median_ms={}, rejected=[], cached=False)
+    if stepper.dispersive:
+        record = dict(path='split', chosen={}, median_ms={}, rejected=[], cached=False)
         if report is not None:
             report['launch_tuning'] = record
         return record
-    path = 'two_step' if stepper.two_step else 'one_pass'
-    names = ('waves2', 'chunk_x2') if stepper.two_step else ('tile_y', 'chunk_x')
     key = _key(stepper)
     if candidates is None and key in _CACHE:
         _CACHE.move_to_end(key)
         record = dict(_CACHE[key], cached=True)
-        stepper.retile(**record['chosen'])
+        _install(stepper, record['path'], record['chosen'])
         if report is not None:
             report['launch_tuning'] = record
         return record
-    shapes = [dict(zip(names, c)) for c in (candidates or (TWO_STEP if stepper.two_step else ONE_PASS))]
+    trials = _normalise(stepper, candidates) if candidates is not None else candidates_for(stepper)
     saved = [t.clone() for t in _state(stepper)]
-    original = stepper.launch_shape()
+    original_shape = stepper.launch_shape()
     # Candidates start from a seeded random state (fields and CPML memory), so that the identity check
     # sees every cell and slab row at work; the run's own state is restored at the end.
     gen = torch.Generator(device=stepper.grid.device).manual_seed(20260514)
@@ -90,22 +125,21 @@ def tune_launch(stepper, candidates=None, sweeps=5, units=2, report=None):
         for target, value in zip(_state(stepper), saved):
             target.copy_(value)
 
-    run, steps = _unit(stepper)
     reference, accepted, rejected = None, [], []
-    for shape in shapes:
-        stepper.retile(**shape)
+    for path, shape in trials:
+        _install(stepper, path, shape)
         restore()
         start()
-        run()
+        _unit(stepper)
         torch.cuda.synchronize()
         result = [t.clone() for t in (stepper.grid.E, stepper.grid.H, *stepper.grid.memory_states)]
         if reference is None:
             reference = result
-            accepted.append(shape)
+            accepted.append((path, shape))
         elif all(torch.equal(a, b) for a, b in zip(result, reference)):
-            accepted.append(shape)
+            accepted.append((path, shape))
         else:
-            rejected.append(shape)
+            rejected.append(dict(path=path, **shape))
         del result
     samples = {i: [] for i in range(len(accepted))}
     for sweep in range(sweeps):
@@ -113,27 +147,30 @@ def tune_launch(stepper, candidates=None, sweeps=5, units=2, report=None):
         if sweep % 2:
             order.reverse()
         for i in order:
-            stepper.retile(**accepted[i])
+            _install(stepper, *accepted[i])
             restore()
             start()
-            run()      # first launch of this shape outside the timed region
+            _unit(stepper)      # first launch of this path and shape outside the timed region
             begin, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             begin.record()
             for _ in range(units):
-                run()
+                _unit(stepper)
             end.record()
             end.synchronize()
-            samples[i].append(begin.elapsed_time(end)/(units*steps))
+            samples[i].append(begin.elapsed_time(end)/(units*UNIT_STEPS))
     medians = {i: statistics.median(v) for i, v in samples.items()}
     best = min(medians, key=medians.get)
-    chosen = {**original, **accepted[best]}
-    stepper.retile(**chosen)
+    path, shape = accepted[best]
+    stepper.retile(**original_shape)
+    _install(stepper, path, shape)
     restore()
     torch.cuda.synchronize()
     del saved, trial
-    record = dict(path=path, chosen=accepted[best], cached=False, rejected=rejected,
-                  median_ms={','.join(f'{k}={v}' for k, v in shape.items()): round(medians[i], 4)
-                             for i, shape in enumerate(accepted)})
+
+    def label(p, sh):
+        return ','.join([p, *(f'{k}={v}' for k, v in sh.items())])
+    record = dict(path=path, chosen=shape, cached=False, rejected=rejected,
+                  median_ms={label(*accepted[i]): round(medians[i], 4) for i in range(len(accepted))})
     if candidates is None:
         _CACHE[key] = {k: v for k, v in record.items() if k != 'cached'}
         while len(_CACHE) > _CACHE_SIZE:
