@@ -37,6 +37,7 @@ from .grid import march_family
 from .models import Project
 from .monitors import interpolation_map, plane_plan
 from .planning import index_at, periodic_axes
+from .plan import resolve_plan
 from .solver import Stepper
 
 COMPONENTS = ('Ex', 'Ey', 'Ez', 'Hx', 'Hy', 'Hz')
@@ -402,6 +403,17 @@ class DifferentiableSimulation(torch.nn.Module):
             flat = ((loc[0]*r.shape[1]+loc[1])*r.shape[2]+loc[2])*3+'xyz'.index(m.component[1].lower())
             self.observers.append((m.component[0] == 'H', flat))
 
+    @property
+    def plan(self):
+        """The resolved plan of the fixed project (plan.resolve_plan), shared with every entry point."""
+        if getattr(self, '_plan', None) is None:
+            self._plan = resolve_plan(self.project)
+        return self._plan
+
+    @property
+    def plan_hash(self):
+        return self.plan.plan_hash
+
     def _run(self, epsilon):
         _check_epsilon(epsilon, self.project.region)
         observers = sorted(set(self.observers))
@@ -453,6 +465,17 @@ class DifferentiablePlaneSimulation(torch.nn.Module):
             self.plans.append((monitor.id, monitor.normal, plan, maps))
         self.observers = observers
         self.components = tuple(('H' if m else 'E') for m, _ in observers)
+
+    @property
+    def plan(self):
+        """The resolved plan of the fixed project (plan.resolve_plan), shared with every entry point."""
+        if getattr(self, '_plan', None) is None:
+            self._plan = resolve_plan(self.project)
+        return self._plan
+
+    @property
+    def plan_hash(self):
+        return self.plan.plan_hash
 
     def forward(self, epsilon, frequency_hz):
         _check_epsilon(epsilon, self.project.region)

@@ -64,11 +64,12 @@ class CpmlSlab:
         return tuple(s)
 
 
-def cpml_slabs(region):
-    """Per (forward, axis, comp) the CPML slabs of every PML face, in face order."""
+def cpml_slabs(region, dtype=None):
+    """Per (forward, axis, comp) the CPML slabs of every PML face, in face order; b, c and inv_k in
+    `dtype` (default: the field precision)."""
     shape = region.shape
     courant = region.rectangular_courant
-    real = real_dtype(region)
+    real = real_dtype(region) if dtype is None else dtype
     table = {}
     for forward in (False, True):
         for axis, comp, _, _ in CURL_TERMS:
@@ -434,12 +435,13 @@ class SourceTerm:
     values: np.ndarray
 
 
-def source_terms(project, steps=None):
-    """Soft point and plane sources (injection.py:117-141, Cartesian or polarized, real fields)."""
+def source_terms(project, steps=None, dtype=None, only=None):
+    """Soft point and plane sources (injection.py:117-141, Cartesian or polarized, real fields);
+    values in `dtype` (default: the field precision); `only`: the terms of that one source."""
     r = project.region
     steps = r.steps if steps is None else steps
     out = []
-    for raw in project.sources:
+    for raw in project.sources if only is None else [only]:
         s = project.resolved_source(raw)
         if not s.enabled:
             continue
@@ -450,5 +452,5 @@ def source_terms(project, steps=None):
             scalar = s.model_copy(update={'component': component, 'theta': None})
             box = source_box(scalar, r)
             out.append(SourceTerm(component[0], 'xyz'.index(component[1].lower()), box,
-                                  (waveform*weight).astype(real_dtype(r))))
+                                  (waveform*weight).astype(real_dtype(r) if dtype is None else dtype)))
     return out

@@ -27,6 +27,7 @@ from .grid import (MaterialIndex, YeeGrid, configure_cuda_kernel, half_step_desc
                    resolve_cuda_kernel)
 from .models import Project
 from .monitors import FrequencyPlane, apodization_window, point_spectrum
+from .plan import resolve_plan
 from .planning import field_axes, index_at, source_terms, voxelize
 from .run_control import RunMonitor, check_count
 
@@ -694,6 +695,7 @@ def run_summary(project, run, e, completed, termination, seconds, setup_seconds,
                    mcells_per_second=math.prod(r.shape)*completed/max(seconds, 1e-9)/1e6,
                    field_peak=float(np.max(np.abs(e))),
                    material_sampling=r.material_sampling, boundaries=r.boundaries.model_dump(),
+                   plan_hash=resolve_plan(project).plan_hash,
                    units='geometry: um; time: s; E/H: reduced fields; Bloch phase: rad', warnings=[])
     summary.update(extra)
     return summary
