@@ -87,15 +87,6 @@ __device__ __forceinline__ void half_step_cell(const HalfStep<real>& a, const lo
     const int dims[3] = {a.nx, a.ny, a.nz};
     const long long stride[3] = {(long long)nyz, (long long)a.nz, 1};
     const Cell<real> c = cell_at(a.src, i);
-    // The updated field and its coefficients are read with the curl's inputs, ahead of the CPML
-    // memory stores below (which the compiler cannot order after these loads otherwise).
-    const Cell<real> old = cell_at(a.dst, i);
-    Cell<real> inv;
-    if (a.inverse_stride == 3) inv = cell_at(a.inverse, i);
-    else {
-        const real v = global(a.inverse)[a.inverse_stride == 1 ? i : 0];
-        inv = Cell<real>{{v, v, v}};
-    }
     Cell<real> nb[3];
     bool inside[3];
 #pragma unroll
@@ -142,6 +133,13 @@ __device__ __forceinline__ void half_step_cell(const HalfStep<real>& a, const lo
             d = -a.pec_factor[axis] * c.v[ic];
         }
         curl[oc] = term_sign(t) > 0 ? curl[oc] + d : curl[oc] - d;
+    }
+    const Cell<real> old = cell_at(a.dst, i);
+    Cell<real> inv;
+    if (a.inverse_stride == 3) inv = cell_at(a.inverse, i);
+    else {
+        const real v = global(a.inverse)[a.inverse_stride == 1 ? i : 0];
+        inv = Cell<real>{{v, v, v}};
     }
     real out[3];
 #pragma unroll
