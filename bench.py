@@ -402,9 +402,11 @@ def main():
     tuning = None
     if not args.no_tune:
         # The grid's launch shapes measured on this box (bitwise-identical candidates only).
-        from torchfdtd_amd.tuning import tune_launch
+        from torchfdtd_amd.tuning import ONE_PASS, tune_launch
         print('bench: tuning the launch shape', file=sys.stderr, flush=True)
-        tuning = tune_launch(run, sweeps=3, units=2)
+        # --one-step keeps the one-step kernel: only its shapes are candidates.
+        only = [('one_pass', dict(tile_y=t, chunk_x=c)) for t, c in ONE_PASS] if args.one_step else None
+        tuning = tune_launch(run, candidates=only, sweeps=3, units=2)
     # Graphs of whole fours keep every step on the two-step pass (a graph of 50 steps ends in a pair
     # of one-step launches).
     width = args.graph_width if not run.two_step else args.graph_width // 4 * 4
