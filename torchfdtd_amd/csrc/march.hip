@@ -578,8 +578,9 @@ __device__ __forceinline__ real cpml2(real d, const Family<real>& F, const Hit& 
     const real b = coef[hit.coef + hit.q];
     const real c = coef[hit.coef + hit.w + hit.q];
     const real ik = coef[hit.coef + 2 * hit.w + hit.q];
+    // STAGE 3: step n+1 with psi at time n already in the slot (loaded a plane ahead).
     const real m = (STAGE == 1 ? global(F.psi_in)[o] : slot) * b + c * d;
-    if (STAGE == 1) slot = m;
+    if (STAGE == 1 || STAGE == 3) slot = m;
     else if (write) global(F.psi_out)[o] = m;
     return d * ik + m;
 }
@@ -607,8 +608,9 @@ struct Consts2 {
     int ebl[3], ebh[3], hbl[3], hbh[3];
 };
 
-// e_cell with the CPML memory of term t in slot[t].
-template <typename real, int CPML, int STAGE, bool ZPRE = false>
+// e_cell with the CPML memory of term t in slot[t].  ZPRE / XPRE: at stage 1 the z / x terms find
+// their memory at time n already in the slot.
+template <typename real, int CPML, int STAGE, bool ZPRE = false, bool XPRE = false>
 __device__ __forceinline__ V3<real> e_cell2(const March<real>& a, const Consts2<real>& q, int x, int y, int z, bool write,
                                             const Hit& hx, const Hit& hy, const Hit& hz, const ZCoef<real>& zc, V3<real> h,
                                             V3<real> hxm, V3<real> hym, real hx_zm, real hy_zm, V3<real> e, V3<real> inv,
@@ -622,7 +624,7 @@ __device__ __forceinline__ V3<real> e_cell2(const March<real>& a, const Consts2<
         real d = real(0);
         if (x > 0) {
             d = h.z - hxm.z;
-            if ((CPML & 1) && hx.face >= 0) d = cpml2<real, STAGE>(d, a.ef, hx, 0, px, write, slot[0]);
+            if ((CPML & 1) && hx.face >= 0) d = cpml2<real, (XPRE && STAGE == 1) ? 3 : STAGE>(d, a.ef, hx, 0, px, write, slot[0]);
         }
         cy = cy - d;
     }
@@ -630,7 +632,7 @@ __device__ __forceinline__ V3<real> e_cell2(const March<real>& a, const Consts2<
         real d = real(0);
         if (x > 0) {
             d = h.y - hxm.y;
-            if ((CPML & 1) && hx.face >= 0) d = cpml2<real, STAGE>(d, a.ef, hx, 1, px, write, slot[1]);
+            if ((CPML & 1) && hx.face >= 0) d = cpml2<real, (XPRE && STAGE == 1) ? 3 : STAGE>(d, a.ef, hx, 1, px, write, slot[1]);
         }
         cz = cz + d;
     }
@@ -670,7 +672,7 @@ __device__ __forceinline__ V3<real> e_cell2(const March<real>& a, const Consts2<
 }
 
 // h_cell with the CPML memory of term t in slot[t].
-template <typename real, int CPML, int STAGE, bool ZPRE = false>
+template <typename real, int CPML, int STAGE, bool ZPRE = false, bool XPRE = false>
 __device__ __forceinline__ V3<real> h_cell2(const March<real>& a, const Consts2<real>& q, int x, int y, int z, bool write,
                                             real coefficient, const Hit& hx, const Hit& hy, const Hit& hz, const ZCoef<real>& zc,
                                             V3<real> e, V3<real> exn, V3<real> eyn, real ex_zp, real ey_zp, V3<real> h,
@@ -684,7 +686,7 @@ __device__ __forceinline__ V3<real> h_cell2(const March<real>& a, const Consts2<
         real d = real(0);
         if (x < nx - 1) {
             d = exn.z - e.z;
-            if ((CPML & 1) && hx.face >= 0) d = cpml2<real, STAGE>(d, a.hf, hx, 0, px, write, slot[0]);
+            if ((CPML & 1) && hx.face >= 0) d = cpml2<real, (XPRE && STAGE == 1) ? 3 : STAGE>(d, a.hf, hx, 0, px, write, slot[0]);
         } else if (q.pec_mask & 1) {
             d = -q.pec[0] * e.z;
         }
@@ -694,7 +696,7 @@ __device__ __forceinline__ V3<real> h_cell2(const March<real>& a, const Consts2<
         real d = real(0);
         if (x < nx - 1) {
             d = exn.y - e.y;
-            if ((CPML & 1) && hx.face >= 0) d = cpml2<real, STAGE>(d, a.hf, hx, 1, px, write, slot[1]);
+            if ((CPML & 1) && hx.face >= 0) d = cpml2<real, (XPRE && STAGE == 1) ? 3 : STAGE>(d, a.hf, hx, 1, px, write, slot[1]);
         } else if (q.pec_mask & 1) {
             d = -q.pec[0] * e.y;
         }
@@ -903,10 +905,29 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
         // The z-face tiles also load the z-slab CPML memory that step n+1 of the next plane reads (E at
         // plane x+1, H at plane x) with it.
         constexpr bool ZPRE = CPML == 4;
+        // The x-end tiles without y or z slabs load the x-slab CPML memory of step n+1 a plane ahead
+        // the same way (the x-slab rows of a plane share one slab row, so every lane reads one).
+        constexpr bool XPRE = CPML == 1;
         V3<real> eo_n[RW], inv3_n[RW];
         real inv1_n[RW];
         unsigned idx_n[RW];
         real pzE[RW][2] = {}, pzH[RW][2] = {};
+        real pxE[RW][2] = {}, pxH[RW][2] = {};
+        // H-family x-slab memory of plane x-1, which stage 1b of plane x reads (x <= nx).
+        auto load_xh = [&](int x) {
+            if (!XPRE || x < 1) return;
+            const Hit hx = find_hit(a.hf, 0, x - 1);
+            if (hx.face < 0) return;
+    #pragma unroll
+            for (int k = 0; k < RW; ++k) {
+                const int yk = yof(k);
+                if (zin && yk >= 0 && yk < ny) {
+                    const unsigned px = ((unsigned)hx.q * ny + yk) * nz + z;
+                    pxH[k][0] = global(a.hf.psi_in)[hx.psi0 + px];
+                    pxH[k][1] = global(a.hf.psi_in)[hx.psi1 + px];
+                }
+            }
+        };
         // H-family z-slab memory of plane x-1, which stage 1b of plane x reads (x <= nx).
         auto load_zh = [&](int x) {
     #pragma unroll
@@ -935,7 +956,22 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                     pzE[k][1] = global(a.ef.psi_in)[ez_hit.psi1 + pz];
                 }
             }
+            if (XPRE) {
+                const Hit hx = find_hit(a.ef, 0, x);
+                if (hx.face >= 0) {
+    #pragma unroll
+                    for (int k = 0; k < RW; ++k) {
+                        const int yk = yof(k);
+                        if (zin && yk >= 0 && yk < ny) {
+                            const unsigned px = ((unsigned)hx.q * ny + yk) * nz + z;
+                            pxE[k][0] = global(a.ef.psi_in)[hx.psi0 + px];
+                            pxE[k][1] = global(a.ef.psi_in)[hx.psi1 + px];
+                        }
+                    }
+                }
+            }
             load_zh(x);
+            load_xh(x);
         };
         if (PREFETCH && xs < nx) load_plane(xs);
         auto march_plane = [&](const int x, real* const sHx, real* const sHn) {
@@ -960,15 +996,18 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                 }
             }
             real zE[RW][2], zH[RW][2];   // this plane's z-slab CPML memory, before the next plane's loads
+            real xE[RW][2], xH[RW][2];   // and its x-slab memory
     #pragma unroll
             for (int k = 0; k < RW; ++k) {
                 zE[k][0] = pzE[k][0]; zE[k][1] = pzE[k][1];
                 zH[k][0] = pzH[k][0]; zH[k][1] = pzH[k][1];
+                xE[k][0] = pxE[k][0]; xE[k][1] = pxE[k][1];
+                xH[k][0] = pxH[k][0]; xH[k][1] = pxH[k][1];
             }
             if (PREFETCH && x + 1 <= xe && x + 1 < nx) load_plane(x + 1);
             // A chunk that ends at the grid's last plane (no high-x slab) still computes H1(nx-1) at
-            // x = nx: its z-slab memory is loaded without a plane nx of E to go with it.
-            else if (ZPRE && x + 1 <= xe && x + 1 == nx) load_zh(nx);
+            // x = nx: its z- and x-slab memory is loaded without a plane nx of E to go with it.
+            else if ((ZPRE || XPRE) && x + 1 <= xe && x + 1 == nx) { load_zh(nx); load_xh(nx); }
             V3<real> e1[RW], hn[RW], h1[RW], e2[RW];
             // Stage 1: E1(x).
             const Hit ex1 = find_hit(a.ef, 0, x);
@@ -984,7 +1023,8 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                     const real* pzm = p - 3;
                     const Hit ey = find_hit(a.ef, 1, y);
                     if (ZPRE) { pe1[k][4] = zE[k][0]; pe1[k][5] = zE[k][1]; }
-                    V3<real> v = e_cell2<real, CPML, 1, ZPRE>(a, q, x, y, z, false, ex1, ey, ez_hit, ez_coef, hn[k], Hn_c[k],
+                    if (XPRE) { pe1[k][0] = xE[k][0]; pe1[k][1] = xE[k][1]; }
+                    V3<real> v = e_cell2<real, CPML, 1, ZPRE, XPRE>(a, q, x, y, z, false, ex1, ey, ez_hit, ez_coef, hn[k], Hn_c[k],
                                                               lds3(pym), pzm[0], pzm[1], eo[k], inv[k], pe1[k]);
                     add_sources(a.se, q.nse, q.ebl, q.ebh, n, q.samples, x, y, z, v.x, v.y, v.z);
                     e1[k] = sel3(lanes, v);
@@ -1002,7 +1042,8 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                     const real* pzn = p + 3;
                     const Hit hy = find_hit(a.hf, 1, y);
                     if (ZPRE) { ph1[k][4] = zH[k][0]; ph1[k][5] = zH[k][1]; }
-                    V3<real> v = h_cell2<real, CPML, 1, ZPRE>(a, q, x - 1, y, z, false, hcoef, hx1, hy, hz_hit, hz_coef, E1_c[k],
+                    if (XPRE) { ph1[k][0] = xH[k][0]; ph1[k][1] = xH[k][1]; }
+                    V3<real> v = h_cell2<real, CPML, 1, ZPRE, XPRE>(a, q, x - 1, y, z, false, hcoef, hx1, hy, hz_hit, hz_coef, E1_c[k],
                                                               pE1 ? e1[k] : E1_c[k], lds3(p + 3 * 64), pzn[0], pzn[1], Hn_c[k],
                                                               ph1[k]);
                     add_sources(a.sh, q.nsh, q.hbl, q.hbh, n, q.samples, x - 1, y, z, v.x, v.y, v.z);
@@ -1083,6 +1124,9 @@ constexpr bool kConcurrentVariants = false;
 // Occupancy targets (waves per SIMD), measured at 512^3 with the H planes staged by direct loads:
 // FP32 tiles of 8 rows ask 8 for the interior (it compiles to 73 VGPRs, i.e. 6) and 6 for the z-face
 // and boundary tiles (80 VGPRs, a few spilled); FP64 and 16-row tiles hold twice the values per lane.
+// Loading the next plane's E and 1/eps ahead in the z-face and boundary tiles, with 4 waves per SIMD
+// so that it does not spill, measured slower: 2.44 against 2.25 ms per step at 512^3 and 0.173
+// against 0.139 s per 100 steps on the 64-grid uneven batch.
 #define TFDTD_TILE_BOUNDS                                                                          \
     __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(                                   \
         CPML == 0 ? 8 : sizeof(real) == 4 ? 6 : 3,                                                   \
