@@ -19,6 +19,7 @@ import math
 import os
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -277,7 +278,20 @@ def surrogate_line(batch=128, reps=5):
                 conv_tflops=round(flops/seconds/1e12, 2), peak_tflops=2500.0,
                 frac=round(flops/seconds/1e12/2500.0, 5), dtype='bf16 MFMA, fp32 accumulate',
                 large_conv=dict(shape='64x32x32x16x64 -> 128, 3x3x3, stride 1', gemm_tflop=round(gemm/1e12, 3),
-                                tflops=rates, frac={k: round(v/2500.0, 4) for k, v in rates.items()}))
+                                tflops=rates, frac={k: round(v/2500.0, 4) for k, v in rates.items()}),
+                study=surrogate_study())
+
+
+def surrogate_study():
+    """The design study's summary (training set size, held-out error, and per campaign the unscreened
+    solves needed to match the screened loop after 32, 96 and 192 solves), read from the committed
+    record of its last run: the study takes about 25 GPU-minutes and is not rerun here."""
+    path = Path(__file__).resolve().parent/'torchfdtd_amd'/'surrogate_study.json'
+    if not path.exists():
+        return None
+    record = json.loads(path.read_text())
+    return dict(record, source=f'read from {path.name} (python -m torchfdtd_amd.surrogate study, '
+                               f'{record.get("measured", "MI355X")}); not measured in this run')
 
 
 def adjoint_line(steps=2000):
@@ -354,6 +368,7 @@ def main():
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--no-batched', action='store_true')
     ap.add_argument('--no-adjoint', action='store_true')
+    ap.add_argument('--adjoint-steps', type=int, default=2000, help='steps of the adjoint configuration (2000; fewer for profiling)')
     ap.add_argument('--no-observed', action='store_true', help='skip the 2D/3D waveguide lines with their FP64 error')
     ap.add_argument('--no-surrogate', action='store_true')
     ap.add_argument('--repeats', type=int, default=3, help='further timed samples of --steps steps (median reported)')
@@ -524,7 +539,7 @@ def extra_lines(args, line):
         line['observed'] = observed_lines()
     if not args.no_adjoint:
         print('bench: adjoint gradient', file=sys.stderr, flush=True)
-        line['adjoint'] = adjoint_line()
+        line['adjoint'] = adjoint_line(args.adjoint_steps)
     if not args.no_surrogate:
         print('bench: surrogate training step', file=sys.stderr, flush=True)
         line['surrogate'] = surrogate_line()
