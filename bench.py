@@ -130,8 +130,18 @@ def batched_line(steps, count, loop_grids, launch='tune', width=50):
     plane_bytes = sum(plane_bytes_per_step(c.project) for c in cases)
     per_step = report.plan['loop_seconds']/steps
     achieved = (step_bytes+plane_bytes)/per_step/1e9
+    # Counted HBM bytes per step of this cohort and launch, from the committed counter summary.
+    traffic = None
+    path = os.path.join(ROOT, 'profiles', 'batch_uneven64_march_kernel_stats.md')
+    if count == 64 and (chosen.kernel, chosen.tile_y, chosen.chunk_x) == ('march', 8, 6) and os.path.exists(path):
+        for line in open(path):
+            if line.startswith('march_8_6_traffic_bytes_per_step:'):
+                traffic = int(line.split(':')[1])
     roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
-                    frac=round(achieved/HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved/HBM_PEAK_GBS, 4), traffic=traffic,
+                    traffic_source=('copied from profiles/batch_uneven64_march_kernel_stats.md (rocprofv3 --pmc FETCH_SIZE '
+                                    'and WRITE_SIZE per dispatch, scaled by a plain copy), not measured in this run'
+                                    if traffic else None),
                     kernel=f'batched {chosen.kernel} step of all {count} grids plus their DFT planes; duration = '
                            'per-step span of the timed cohort loop',
                     bytes_per_cell_update=round((step_bytes+plane_bytes)/cells, 2),
@@ -162,6 +172,19 @@ def plane_bytes_per_step(project):
         corners = 8 if r.dimension == '3d' else 4
         total += 2*spec*nf*points*comps/block+4*corners*points*comps
     return total
+
+
+def observed_cpu_baseline(project):
+    """The reference's CPU path (the NumPy oracle, one thread) on an observed project: the whole run for
+    a 2D guide, a bounded sample of its first 150 steps for a 3D one (about 15 s).  Returns (steps,
+    seconds, Gcell-updates/s)."""
+    from oracle import simulate
+    r = project.region
+    steps = r.steps if r.dimension == '2d' else min(r.steps, 150)
+    t0 = time.perf_counter()
+    simulate.run(project.model_dump(), steps=steps, record_frames=False)
+    seconds = time.perf_counter()-t0
+    return steps, seconds, math.prod(r.shape)*steps/seconds/1e9
 
 
 def observed_lines(width=50, repeats=3):
@@ -197,9 +220,14 @@ def observed_lines(width=50, repeats=3):
         spectrum = point_spectrum(got.times, got.signals[:, 0].astype(np.float64), settings)['value']
         exact_spectrum = point_spectrum(ref.times, ref.signals[:, 0], settings)['value']
         step_bytes = algorithmic_bytes(r)[0] if r.dimension == '3d' else 4*(6*cells+cells)+8*psi_tm(r)
+        cpu_steps, cpu_seconds, cpu_rate = observed_cpu_baseline(project)
         out[name] = dict(value=round(cells*r.steps/seconds/1e9, 3), unit='Gcell-updates/s', steps=r.steps, cells=cells,
                          seconds=round(seconds, 4), samples_seconds=[round(run.summary['seconds'], 4) for run in runs],
                          steps_per_pass=got.summary['steps_per_pass'],
+                         cpu_baseline=dict(value=round(cpu_rate, 5), unit='Gcell-updates/s', cores=1, kind='port',
+                                           sample=f'NumPy oracle, one thread, first {cpu_steps} of {r.steps} steps, '
+                                                  f'{cpu_seconds:.1f} s'),
+                         gpu_over_cpu=round(cells*r.steps/seconds/1e9/cpu_rate, 1),
                          effective_gbs=round(step_bytes*r.steps/seconds/1e9, 1),
                          fp32_vs_fp64_rel_l2=dict(E=rel(got.electric, ref.electric), H=rel(got.magnetic, ref.magnetic),
                                                   trace=rel(got.signals, ref.signals),
