@@ -1126,7 +1126,9 @@ constexpr bool kConcurrentVariants = false;
 // and boundary tiles (80 VGPRs, a few spilled); FP64 and 16-row tiles hold twice the values per lane.
 // Loading the next plane's E and 1/eps ahead in the z-face and boundary tiles, with 4 waves per SIMD
 // so that it does not spill, measured slower: 2.44 against 2.25 ms per step at 512^3 and 0.173
-// against 0.139 s per 100 steps on the 64-grid uneven batch.
+// against 0.139 s per 100 steps on the 64-grid uneven batch.  Loading only the z-slab CPML memory a
+// plane ahead at 6 waves (E after the plane's E, H after its H) spills more and was slower too: 2.29
+// ms and 0.158 s.
 #define TFDTD_TILE_BOUNDS                                                                          \
     __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(                                   \
         CPML == 0 ? 8 : sizeof(real) == 4 ? 6 : 3,                                                   \
