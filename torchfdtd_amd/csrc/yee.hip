@@ -101,31 +101,52 @@ __device__ __forceinline__ void half_step_cell(const HalfStep<real>& a, const lo
             nb[axis] = cell_at(a.src, forward ? i - span : i + span);
         }
     }
+    // CPML memory.  The two slabs of a term never overlap (tfdtd_assign_slot), so a cell meets at most
+    // one slab per term.  Every memory value and coefficient the cell needs is read here and the new
+    // memory is stored after the last load: a load placed after a store would wait for it (the
+    // compiler cannot tell the arrays apart), one round trip per term.
+    bool hit[6];
+    real* at[6];
+    real old_psi[6], cb[6], cc[6], cik[6], memory[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        const int axis = term_axis(t);
+        hit[t] = false;
+        at[t] = nullptr;
+        old_psi[t] = cb[t] = cc[t] = cik[t] = memory[t] = real(0);
+        if (dims[axis] == 1 || !inside[axis]) continue;
+        const int k = coord[axis];
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const Slab<real>& g = a.slab[2 * t + side];
+            if (k >= g.lo && k < g.hi) {
+                const int q = k - g.lo;
+                const int w = g.hi - g.lo;
+                long long p;
+                if (axis == 0) p = ((long long)q * a.ny + y) * a.nz + z;
+                else if (axis == 1) p = ((long long)x * w + q) * a.nz + z;
+                else p = ((long long)x * a.ny + y) * w + q;
+                hit[t] = true;
+                at[t] = g.psi + p;
+                old_psi[t] = *global(at[t]);
+                cb[t] = global(g.b)[q];
+                cc[t] = global(g.c)[q];
+                cik[t] = global(g.inv_k)[q];
+            }
+        }
+    }
     real curl[3] = {real(0), real(0), real(0)};
 #pragma unroll
     for (int t = 0; t < 6; ++t) {
         const int axis = term_axis(t), ic = term_in(t), oc = term_out(t);
         const int n = dims[axis];
         if (n == 1) continue;
-        const int k = coord[axis];
         real d = real(0);
         if (inside[axis]) {
             d = forward ? (nb[axis].v[ic] - c.v[ic]) : (c.v[ic] - nb[axis].v[ic]);
-#pragma unroll
-            for (int side = 0; side < 2; ++side) {
-                const Slab<real>& g = a.slab[2 * t + side];
-                if (k >= g.lo && k < g.hi) {
-                    const int q = k - g.lo;
-                    const int w = g.hi - g.lo;
-                    long long p;
-                    if (axis == 0) p = ((long long)q * a.ny + y) * a.nz + z;
-                    else if (axis == 1) p = ((long long)x * w + q) * a.nz + z;
-                    else p = ((long long)x * a.ny + y) * w + q;
-                    const auto psi = global(g.psi);
-                    const real memory = psi[p] * global(g.b)[q] + global(g.c)[q] * d;
-                    psi[p] = memory;
-                    d = d * global(g.inv_k)[q] + memory;
-                }
+            if (hit[t]) {
+                memory[t] = old_psi[t] * cb[t] + cc[t] * d;
+                d = d * cik[t] + memory[t];
             }
         } else if ((a.periodic_mask >> axis) & 1) {
             d = forward ? (nb[axis].v[ic] - c.v[ic]) : (c.v[ic] - nb[axis].v[ic]);
@@ -161,6 +182,9 @@ __device__ __forceinline__ void half_step_cell(const HalfStep<real>& a, const lo
             }
         }
     }
+#pragma unroll
+    for (int t = 0; t < 6; ++t)
+        if (hit[t]) *global(at[t]) = memory[t];
     const auto cell = global(a.dst + 3 * i);
     cell[0] = out[0];
     cell[1] = out[1];
