@@ -198,6 +198,11 @@ __global__ __launch_bounds__(256) void yee_half_step_kernel(const HalfStep<real>
     half_step_cell<real, forward>(a, i);
 }
 
+// Cells per thread of the batched half step: a workgroup covers 256 * kBatchSpan cells of one grid, so
+// the search for its grid and the descriptor loads are shared by more cells (2: 2% faster on the
+// 64-grid uneven batch than 1 or 4).
+constexpr int kBatchSpan = 2;
+
 // Many grids in one launch: block b belongs to the last grid g with prefix[g] <= b; the grids'
 // descriptors are a table in device memory, read with wave-uniform loads.
 template <typename real, bool forward>
@@ -211,9 +216,13 @@ __global__ __launch_bounds__(256) void batch_half_step_kernel(const HalfStep<rea
         else hi = mid;
     }
     const HalfStep<real>& a = table[lo];
-    const long long i = (long long)(b - prefix[lo]) * blockDim.x + threadIdx.x;
-    if (i >= (long long)a.nx * a.ny * a.nz) return;
-    half_step_cell<real, forward>(a, i);
+    const long long cells = (long long)a.nx * a.ny * a.nz;
+    const long long base = (long long)(b - prefix[lo]) * (256 * kBatchSpan) + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kBatchSpan; ++j) {
+        const long long i = base + 256 * j;
+        if (i < cells) half_step_cell<real, forward>(a, i);
+    }
 }
 
 template <typename real>
@@ -621,7 +630,7 @@ int pack_half_steps(const tfdtd_half_step* steps, int count, void* host_table, s
         const int status = make_half_step(steps + k, table + k);
         if (status) return status;
         prefix[k] = (int32_t)total;
-        total += ((long long)steps[k].nx * steps[k].ny * steps[k].nz + 255) / 256;
+        total += ((long long)steps[k].nx * steps[k].ny * steps[k].nz + 256 * kBatchSpan - 1) / (256 * kBatchSpan);
         if (total > 0x7fffffffLL) return fail(TFDTD_EINVAL, "batch: too many cells for one launch");
     }
     prefix[count] = (int32_t)total;

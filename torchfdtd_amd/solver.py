@@ -33,10 +33,24 @@ from .run_control import RunMonitor, check_count
 
 C0 = 299792458.0
 ENGINE_LOCK = threading.Lock()
-# Grids of fewer cells take the split step by default: measured on MI355X, the split step beats the
-# best one-pass shape by 19-47% on 2.0-2.3 M-cell grids with 10-12 CPML cells, and loses by 10% at
-# 256^3 (16.8 M cells).  tuning.tune_launch measures every path on the grid itself.
-ONE_PASS_MIN_CELLS = 8_000_000
+# Default step path by grid size (measured on MI355X, per step, tiles at their best shapes): the split
+# step beats the one-pass tiles by 11-48% on 0.8-7.1 M-cell grids, by 9% at 256^3 and ties them at
+# 320^3 (32.8 M cells); the two-step pass ties the split step at 256^3 and beats it by 21% at 320^3.
+# So float32 grids without plane monitors or traces take the two-step pass from TWO_STEP_MIN_CELLS,
+# other grids the one-pass step from ONE_PASS_MIN_CELLS, and smaller grids the split step.
+# tuning.tune_launch measures every path on the grid itself.
+TWO_STEP_MIN_CELLS = 20_000_000
+ONE_PASS_MIN_CELLS = 48_000_000
+
+
+def default_step_path(cells, two_step_eligible, one_pass_supported):
+    """The step path a grid of `cells` takes by default: 'two_step' (float32, no plane monitors or
+    traces: `two_step_eligible`), 'one_pass' or 'split'."""
+    if not one_pass_supported:
+        return 'split'
+    if two_step_eligible and cells >= TWO_STEP_MIN_CELLS:
+        return 'two_step'
+    return 'one_pass' if cells >= ONE_PASS_MIN_CELLS else 'split'
 
 
 @dataclass
@@ -236,9 +250,11 @@ class Stepper:
             configure_cuda_kernel(g, 'torch')
         self.one_pass_supported = supported = (self.kernel == 'hip' and march_supported(g, self.sources)
                                                and not self.dispersive)
-        # Default path by size: below ONE_PASS_MIN_CELLS the split step is the faster one (the one-pass
-        # tiles of a small grid are mostly CPML boundary tiles and too few to fill the GPU).
-        large = math.prod(r.shape) >= ONE_PASS_MIN_CELLS
+        # Default path by size (the one-pass tiles of a small grid are mostly CPML boundary tiles and
+        # too few to fill the GPU): the two-step pass for large grids that allow it, the one-pass step
+        # for larger grids that do not, the split step otherwise.
+        eligible = g.dtype == torch.float32 and not self.planes and not self.trace_points
+        large = default_step_path(math.prod(r.shape), eligible, supported) != 'split'
         self.one_pass = supported and (large or bool(two_step)) if one_pass is None else bool(one_pass)
         if self.one_pass and not supported:
             raise ValueError('the one-pass kernel needs a 3D grid without periodic axes, dispersive materials or '
