@@ -130,18 +130,21 @@ def batched_line(steps, count, loop_grids, launch='tune', width=50):
     plane_bytes = sum(plane_bytes_per_step(c.project) for c in cases)
     per_step = report.plan['loop_seconds']/steps
     achieved = (step_bytes+plane_bytes)/per_step/1e9
-    # Counted HBM bytes per step of this cohort and launch, from the committed counter summary.
-    traffic = None
-    path = os.path.join(ROOT, 'profiles', 'batch_uneven64_march_kernel_stats.md')
-    if count == 64 and (chosen.kernel, chosen.tile_y, chosen.chunk_x) == ('march', 8, 6) and os.path.exists(path):
+    # Counted HBM bytes per step of this cohort and launch, from the committed counter summaries.
+    traffic, source = None, None
+    summaries = {('split',): ('batch_uneven64_kernel_stats.md', 'split_traffic_bytes_per_step:'),
+                 ('march', 8, 6): ('batch_uneven64_march_kernel_stats.md', 'march_8_6_traffic_bytes_per_step:')}
+    name, key = summaries.get((chosen.kernel,) if chosen.kernel == 'split' else (chosen.kernel, chosen.tile_y, chosen.chunk_x),
+                              (None, None))
+    path = os.path.join(ROOT, 'profiles', name) if name else None
+    if count == 64 and path and os.path.exists(path):
         for line in open(path):
-            if line.startswith('march_8_6_traffic_bytes_per_step:'):
-                traffic = int(line.split(':')[1])
+            if line.startswith(key):
+                traffic, source = int(line.split(':')[1]), f'profiles/{name}'
     roofline = dict(bound='hbm', achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit='GB/s',
                     frac=round(achieved/HBM_PEAK_GBS, 4), traffic=traffic,
-                    traffic_source=('copied from profiles/batch_uneven64_march_kernel_stats.md (rocprofv3 --pmc FETCH_SIZE '
-                                    'and WRITE_SIZE per dispatch, scaled by a plain copy), not measured in this run'
-                                    if traffic else None),
+                    traffic_source=(f'copied from {source} (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch, '
+                                    'scaled by a plain copy), not measured in this run' if traffic else None),
                     kernel=f'batched {chosen.kernel} step of all {count} grids plus their DFT planes; duration = '
                            'per-step span of the timed cohort loop',
                     bytes_per_cell_update=round((step_bytes+plane_bytes)/cells, 2),
