@@ -190,6 +190,50 @@ def observed_cpu_baseline(project):
     return steps, seconds, math.prod(r.shape)*steps/seconds/1e9
 
 
+def observed_cube_line(steps=96, width=48):
+    """The 512^3 cube's workload (16-cell CPML, seeded permittivity, Ez dipole) with an x-normal
+    16-frequency float64 DFT plane (E and H, 400 x 400 points) and an Ez and an Hy trace near high x:
+    per-step time on the default path, the two-step pass (its observers sample step n+1 from the
+    snapshot of the three planes they read), and on the one-step kernel, both in graphs of `width`
+    steps after a warm-up graph."""
+    import torch
+
+    from torchfdtd_amd import FieldMonitor, Monitor
+    from torchfdtd_amd import workloads as W
+    from torchfdtd_amd.graphs import StepGraphs
+    from torchfdtd_amd.models import SpectrumSettings
+    from torchfdtd_amd.solver import Stepper
+    project = W.cube(512, steps=steps+width)
+    spectrum = SpectrumSettings(sampling='wavelength', frequency_points=16, wavelength_start=1.45, wavelength_stop=1.65,
+                                apodization='none')
+    project.monitors = [FieldMonitor(id='plane', center=(8., 0, 0), normal='x', size=(0, 16., 16.), dft_precision='float64',
+                                     spectrum=spectrum),
+                        Monitor(id='ez', component='Ez', center=(8., 1., 0)), Monitor(id='hy', component='Hy', center=(8.04, 0, 1.))]
+    project = type(project).model_validate(project.model_dump())
+    inverse = W.cube_inverse_permittivity(W.cube_noise_mask(512))[..., None]
+    cells = math.prod(project.region.shape)
+    out = dict(workload='512^3 cube, 16-cell CPML, Ez dipole, x-normal 16-frequency float64 DFT plane (400 x 400 points, '
+                        'E and H) and two traces near high x', steps=steps)
+    for name, kw in (('default', {}), ('one_step', dict(two_step=False))):
+        print(f'bench: observed cube, {name} path', file=sys.stderr, flush=True)
+        run = Stepper(project, **kw)
+        run.grid.inverse_permittivity.copy_(inverse)
+        graphs = StepGraphs(run.step, run.states(), project.region.steps, width, pair=run.step_pair,
+                            prepare=run.refresh_materials, quad=run.quad if run.two_step else None)
+        graphs.advance(width)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        graphs.advance(steps)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter()-t0)/steps*1e3
+        out[name] = dict(path=run.path, ms_per_step=round(ms, 4), value=round(cells/ms/1e6, 3), unit='Gcell-updates/s',
+                         snapshot_planes=(run.snapshot_span[1]-run.snapshot_span[0]) if run.two_step else None)
+        del run, graphs
+        torch.cuda.empty_cache()
+    out['speedup'] = round(out['one_step']['ms_per_step']/out['default']['ms_per_step'], 3)
+    return out
+
+
 def observed_lines(width=50, repeats=3):
     """The 160x120 2D TM waveguide (point trace, 32-frequency spectrum) and the 160x120x40 3D
     waveguide (point trace), 1000 steps each through Simulation.run (graphs of `width` steps):
@@ -568,6 +612,7 @@ def extra_lines(args, line):
         line['batched'] = batched_line(args.batch_steps, args.batch_grids, args.loop_grids, args.batch_launch)
     if not args.no_observed:
         line['observed'] = observed_lines()
+        line['observed']['cube_512_plane_and_traces'] = observed_cube_line()
     if not args.no_adjoint:
         print('bench: adjoint gradient', file=sys.stderr, flush=True)
         line['adjoint'] = adjoint_line(args.adjoint_steps)

@@ -159,6 +159,13 @@ typedef struct tfdtd_march {
     const uint8_t* material;
     const void* material_table;
     int32_t materials;
+    /* Two-step pass only (the one-step kernel refuses them): when snap_hi > snap_lo, E and H of the
+     * intermediate step n+1 on the planes [snap_lo, snap_hi) are also written to snap_e and snap_h,
+     * (snap_hi - snap_lo) * ny * nz * 3 values each in the layout of the fields, so that plane
+     * monitors and traces on those planes can sample step n+1.  Zero when unused. */
+    void* snap_e;
+    void* snap_h;
+    int32_t snap_lo, snap_hi;
 } tfdtd_march;
 
 /* torchfdtd/reversible_cuda_fused.py:259-325 (FusedEHMarching) generalised to the

@@ -1,0 +1,81 @@
+"""Plane monitors and traces on the two-step pass: the pass also writes step n+1 on the x planes the
+observers read to a snapshot, and after each pass the DFT and trace launches sample step n+1 from it
+and step n+2 from the fields (the launches of two one-pass steps, in their order).  A grid with an
+x-normal plane monitor (E and H, float64 accumulation) and E/H traces runs the same steps on the two-step
+pass (eager and in graphs), the one-pass step and the split step: fields, CPML memory, traces and
+spectra are equal bit for bit (the split and one-pass steps equal the oracle bit for bit in
+test_hip_parity.py and test_configs.py)."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def observed_project(steps=80, trace_x=1.0):
+    from torchfdtd_amd import FieldMonitor, Monitor, Project, Region, Source, Structure
+    from torchfdtd_amd.models import SpectrumSettings
+    region = Region(dimension='3d', size=(3.2, 1.6, 1.2), mesh=.04, pml_cells=8, steps=steps, backend='cuda')
+    spectrum = SpectrumSettings(sampling='wavelength', frequency_points=8, wavelength_start=1.45, wavelength_stop=1.65,
+                                apodization='none')
+    return Project(region=region,
+                   structures=[Structure(center=(0, 0, 0), size=(2.4, .4, .3), material='Si (constant n)')],
+                   sources=[Source(center=(.8, 0, 0), component='Ez', wavelength=1, pulse_cycles=1)],
+                   monitors=[FieldMonitor(id='plane', center=(1.0, 0, 0), normal='x', size=(0, .8, .5),
+                                          dft_precision='float64', spectrum=spectrum),
+                             Monitor(id='ez', component='Ez', center=(trace_x, .1, 0)),
+                             Monitor(id='hy', component='Hy', center=(trace_x + .04, 0, .05))])
+
+
+def state(run):
+    for plane in run.planes:
+        plane.flush()
+    torch.cuda.synchronize()
+    g = run.grid
+    return dict(E=g.E.cpu(), H=g.H.cpu(), traces=run.traces.cpu(), planes=[p.value.cpu() for p in run.planes],
+                memory=[m.cpu() for m in g.psi_buffers.values()])
+
+
+def assert_same(got, expected):
+    for key in ('E', 'H', 'traces'):
+        assert torch.equal(got[key], expected[key]), key
+    for a, b in zip(got['planes'], expected['planes']):
+        assert torch.equal(a, b)
+    for a, b in zip(got['memory'], expected['memory']):
+        assert torch.equal(a, b)
+
+
+def test_two_step_observers_equal_the_one_step_paths(gpu):
+    from torchfdtd_amd.graphs import StepGraphs
+    from torchfdtd_amd.solver import Stepper
+    project = observed_project()
+    results = {}
+    for path in ('split', 'one_pass', 'two_step'):
+        run = Stepper(project, one_pass=path != 'split', two_step=path == 'two_step')
+        assert run.path == path and 'two_step' in run.paths()
+        run.advance(80)
+        results[path] = state(run)
+        if path == 'two_step':
+            lo, hi = run.snapshot_span
+            assert 0 < hi - lo <= run.grid.Nx // 8 and run.snap_e.shape[0] == hi - lo
+    reference = results['split']
+    assert np.abs(reference['traces'].numpy()).max() > 0 and reference['planes'][0].abs().max() > 0
+    assert_same(results['one_pass'], reference)
+    assert_same(results['two_step'], reference)
+    # The same in graphs of eight steps (two passes of two steps each per graph).
+    run = Stepper(project, one_pass=True, two_step=True)
+    graphs = StepGraphs(run.step, run.states(), 80, 8, pair=run.step_pair, prepare=run.refresh_materials, quad=run.quad)
+    graphs.advance(80)
+    assert graphs.kernel_steps['two_step'] == 80, graphs.kernel_steps
+    assert_same(state(run), reference)
+
+
+def test_observers_spread_over_many_planes_keep_the_one_step_paths(gpu):
+    from torchfdtd_amd.solver import Stepper
+    project = observed_project(trace_x=-1.2)      # the Ez trace 55 planes before the monitor plane
+    run = Stepper(project, one_pass=True)
+    span = run.observed_planes()
+    assert span[1] - span[0] > run.grid.Nx // 8
+    assert run.paths() == ['split', 'one_pass'] and not run.two_step
+    with pytest.raises(ValueError, match='two-step'):
+        Stepper(project, one_pass=True, two_step=True)

@@ -110,7 +110,8 @@ class March(ctypes.Structure):
                 ('nsrc_e', i32), ('srcs_e', SourceBox*MAX_SOURCES),
                 ('nsrc_h', i32), ('srcs_h', SourceBox*MAX_SOURCES),
                 ('source_samples', i64), ('counter', vp), ('tile_y', i32), ('chunk_x', i32),
-                ('material', vp), ('material_table', vp), ('materials', i32)]
+                ('material', vp), ('material_table', vp), ('materials', i32),
+                ('snap_e', vp), ('snap_h', vp), ('snap_lo', i32), ('snap_hi', i32)]
 
 
 class NormGrid(ctypes.Structure):

@@ -89,6 +89,10 @@ struct March {
     int yshift;                 // y tile ty starts at ty*9 - yshift
     int iy0, iy1, iytiles;      // tall interior tiles: rows [iy0, iy1) in iytiles tiles of 13 rows
     int nxend;                  // boundary tiles in the x end chunks (the rest lie in x-middle chunks)
+    // Two-step pass: E and H of step n+1 on planes [snap_lo, snap_hi) are also stored here.
+    real* snap_e;
+    real* snap_h;
+    int snap_lo, snap_hi;
 };
 
 // psi <- psi*b + c*d (stored when `write`), d <- d*inv_k + psi, for slab row q of term k.
@@ -606,6 +610,7 @@ struct Consts2 {
     int nse, nsh;
     long long samples;
     int ebl[3], ebh[3], hbl[3], hbh[3];
+    int snap_lo, snap_hi;
 };
 
 // e_cell with the CPML memory of term t in slot[t].  ZPRE / XPRE: at stage 1 the z / x terms find
@@ -829,6 +834,7 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
     q.nx = nx; q.ny = ny; q.nz = nz; q.pec_mask = a.pec_mask;
     q.courant = a.courant; q.hcoef = hcoef;
     q.nse = a.nsrc_e; q.nsh = a.nsrc_h; q.samples = a.source_samples;
+    q.snap_lo = a.snap_lo; q.snap_hi = a.snap_hi;
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
         q.pec[ax] = a.pec_factor[ax];
@@ -1028,6 +1034,10 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                                                               lds3(pym), pzm[0], pzm[1], eo[k], inv[k], pe1[k]);
                     add_sources(a.se, q.nse, q.ebl, q.ebh, n, q.samples, x, y, z, v.x, v.y, v.z);
                     e1[k] = sel3(lanes, v);
+                    // Snapshot of step n+1: the cells this tile owns (those whose E2 it stores).
+                    if (x >= q.snap_lo && x < q.snap_hi && x >= xa && x < xb && r >= 1 && r < NR - 2 && lane >= 1 &&
+                        lane < 62 && zin && y < ylim)
+                        store3(a.snap_e, 3u * ((unsigned)(x - q.snap_lo) * plane + (unsigned)y * nz + (unsigned)z), v);
                 }
             }
             // Stage 1b: H1(x-1) from E1(x-1) (LDS, with the row and column above) and E1(x).
@@ -1048,6 +1058,9 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                                                               ph1[k]);
                     add_sources(a.sh, q.nsh, q.hbl, q.hbh, n, q.samples, x - 1, y, z, v.x, v.y, v.z);
                     h1[k] = sel3(lanes, v);
+                    if (x - 1 >= q.snap_lo && x - 1 < q.snap_hi && x - 1 >= xa && x - 1 < xb && r >= 1 && r < NR - 2 &&
+                        lane >= 1 && lane < 62 && zin && y < ylim)
+                        store3(a.snap_h, 3u * ((unsigned)(x - 1 - q.snap_lo) * plane + (unsigned)y * nz + (unsigned)z), v);
                 }
                 put3(sH1 + 3 * (r * 64 + lane), h1[k]);
             }
@@ -1242,6 +1255,7 @@ inline void free_range(int tiles, int size, int e_extra, int n, const Face* ef, 
 // Tile layout of one grid: chunking, z shift, the slab-free boxes and the workgroups of each variant.
 template <typename real, int W>
 int layout_march(const tfdtd_march* d, March<real>* a) {
+    if (d->snap_hi > d->snap_lo) return tfdtd_fail(TFDTD_EINVAL, "march: the step n+1 snapshot is an option of the two-step pass");
     const int status = make_march(d, a);
     if (status) return status;
     constexpr int TY = 2 * W;
@@ -1443,6 +1457,14 @@ int layout_march2(const tfdtd_march* d, March<float>* a) {
     if (h_slabs && (!d->h.psi_out || d->h.psi_out == d->h.psi_in))
         return tfdtd_fail(TFDTD_EINVAL, "march2: the H-family CPML memory needs separate input and output buffers");
     a->hf.psi_out = static_cast<float*>(h_slabs ? d->h.psi_out : const_cast<void*>(d->h.psi_in));
+    if (d->snap_hi > d->snap_lo) {
+        if (d->snap_lo < 0 || d->snap_hi > d->nx || !d->snap_e || !d->snap_h)
+            return tfdtd_fail(TFDTD_EINVAL, "march2: the step n+1 snapshot needs planes inside [0, nx) and an E and an H buffer");
+        a->snap_e = static_cast<float*>(d->snap_e);
+        a->snap_h = static_cast<float*>(d->snap_h);
+        a->snap_lo = d->snap_lo;
+        a->snap_hi = d->snap_hi;
+    }
     const int chunk = d->chunk_x > 0 ? d->chunk_x : d->nx;
     a->chunk = chunk;
     // Shift the y tiles to the cheapest of the nine phases (slab-free tiles weigh 10, the others 17).

@@ -233,10 +233,11 @@ class MaterialIndex:
 
 
 def march_descriptor(grid, e_in, h_in, e_out, h_out, psi_e_in, psi_e_out, sources, counter, tile_y=8, chunk_x=8,
-                     materials=None, psi_h=None):
+                     materials=None, psi_h=None, snapshot=None):
     """tfdtd_march: one whole step reading (e_in, h_in, psi_e_in) and writing (e_out, h_out, psi_e_out);
     `materials` (a MaterialIndex) lets the step read 1/eps through the one-byte index.  The H-family
-    CPML memory updates in place unless `psi_h` = (input, output) is given (the two-step march)."""
+    CPML memory updates in place unless `psi_h` = (input, output) is given (the two-step march).
+    `snapshot` = (E buffer, H buffer, lo, hi): the two-step pass also writes step n+1 on planes [lo, hi)."""
     d = N.March()
     d.dtype = N.F64 if grid.dtype == torch.float64 else N.F32
     d.nx, d.ny, d.nz = grid.Nx, grid.Ny, grid.Nz
@@ -272,6 +273,9 @@ def march_descriptor(grid, e_in, h_in, e_out, h_out, psi_e_in, psi_e_out, source
     d.tile_y, d.chunk_x = tile_y, chunk_x
     if materials is not None:
         materials.describe(d)
+    if snapshot is not None:
+        snap_e, snap_h, d.snap_lo, d.snap_hi = snapshot
+        d.snap_e, d.snap_h = _ptr(snap_e), _ptr(snap_h)
     return d
 
 

@@ -41,6 +41,22 @@ ENGINE_LOCK = threading.Lock()
 # tuning.tune_launch measures every path on the grid itself.
 TWO_STEP_MIN_CELLS = 20_000_000
 ONE_PASS_MIN_CELLS = 48_000_000
+# The two-step pass serves plane monitors and traces by writing step n+1 on the planes they read to a
+# snapshot; grids whose observers spread over more than an eighth of the x planes keep the one-step
+# paths (the snapshot would cost more than the pass saves).
+SNAPSHOT_MAX_FRACTION = 8
+
+
+class _Shifted:
+    """A buffer viewed at an element offset: data_ptr() of plane `lo` of a snapshot that starts at
+    plane lo is that of plane 0 of a full field, so the observers' flat indices need no change."""
+
+    def __init__(self, tensor, elements):
+        self.tensor, self.elements = tensor, elements
+        self.device = tensor.device
+
+    def data_ptr(self):
+        return self.tensor.data_ptr() - self.elements*self.tensor.element_size()
 
 
 def default_step_path(cells, two_step_eligible, one_pass_supported):
@@ -253,7 +269,7 @@ class Stepper:
         # Default path by size (the one-pass tiles of a small grid are mostly CPML boundary tiles and
         # too few to fill the GPU): the two-step pass for large grids that allow it, the one-pass step
         # for larger grids that do not, the split step otherwise.
-        eligible = g.dtype == torch.float32 and not self.planes and not self.trace_points
+        eligible = self._two_step_eligible()
         large = default_step_path(math.prod(r.shape), eligible, supported) != 'split'
         self.one_pass = supported and (large or bool(two_step)) if one_pass is None else bool(one_pass)
         if self.one_pass and not supported:
@@ -261,11 +277,12 @@ class Stepper:
                              'more than 8 sources per family')
         self.tile_y, self.chunk_x = tile_y, chunk_x
         self.chunk_x2, self.waves2 = chunk_x2, waves2      # two-step pass: planes per chunk, wavefronts per tile
-        # Two steps per pass (float32, one-pass grids without plane monitors or traces, whose
-        # intermediate steps then never need to reach memory).
+        # Two steps per pass (float32 one-pass grids; plane monitors and traces sample step n+1 from a
+        # snapshot of the planes they read).
         self.two_step = self._two_step_allowed() if two_step is None else bool(two_step)
         if self.two_step and not self._two_step_allowed():
-            raise ValueError('the two-step kernel needs a float32 one-pass grid without plane monitors or point traces')
+            raise ValueError('the two-step kernel needs a float32 one-pass grid whose plane monitors and point traces '
+                             f'read at most 1/{SNAPSHOT_MAX_FRACTION} of the x planes')
         # One-byte material index of 1/eps for the one-pass step (grids with few distinct values).
         self.materials = MaterialIndex(g) if self.one_pass else None
         self._describe()
@@ -278,16 +295,32 @@ class Stepper:
         elif launch is not None:
             raise ValueError("launch is None (default shapes) or 'auto' (measured)")
 
+    def observed_planes(self):
+        """[lo, hi): the x planes that the plane monitors and traces read (None without observers)."""
+        g = self.grid
+        per_plane = 3*g.Ny*g.Nz
+        xs = [flat // per_plane for _, flat in self.trace_points]
+        for plane in self.planes:
+            for indices, _ in plane.maps:
+                xs += [int(indices.min()) // per_plane, int(indices.max()) // per_plane]
+        return (min(xs), max(xs)+1) if xs else None
+
+    def _two_step_eligible(self):
+        if self.grid.dtype != torch.float32:
+            return False
+        span = self.observed_planes()
+        return span is None or span[1]-span[0] <= max(2, self.grid.Nx // SNAPSHOT_MAX_FRACTION)
+
     def _two_step_allowed(self):
-        return (self.one_pass and self.grid.dtype == torch.float32 and not self.planes and not self.trace_points)
+        return self.one_pass and self._two_step_eligible()
 
     def paths(self):
         """The step paths this grid can take: 'split' always; 'one_pass' on one-pass grids; 'two_step'
-        on float32 one-pass grids without plane monitors or point traces."""
+        on float32 one-pass grids whose observers read at most 1/SNAPSHOT_MAX_FRACTION of the x planes."""
         out = ['split']
         if self.one_pass_supported:
             out.append('one_pass')
-            if self.grid.dtype == torch.float32 and not self.planes and not self.trace_points:
+            if self._two_step_eligible():
                 out.append('two_step')
         return out
 
@@ -312,7 +345,7 @@ class Stepper:
         """Replace the recorded traces by `points`: (magnetic, flat element index) pairs."""
         self.trace_points = [(bool(m), int(flat)) for m, flat in points]
         self.traces = torch.zeros((self.steps, len(self.trace_points)), device=self.grid.device, dtype=self.grid.dtype)
-        self.two_step = self.two_step and self._two_step_allowed()   # traces need every step in memory
+        self.two_step = self.two_step and self._two_step_allowed()   # the traces may widen the snapshot
         self._describe()
 
     def _trace_table(self, E, H):
@@ -375,12 +408,27 @@ class Stepper:
                 if getattr(self, 'psi_h_b', None) is None:
                     self.psi_h_b = torch.zeros_like(g.psi_buffers[True])
                 psi_h = g.psi_buffers[True]
+                # Observers sample step n+1 from a snapshot of the planes they read, which the pass
+                # writes; a plane table and a trace table point into it as into a full field.
+                self.snapshot_span = span = self.observed_planes()
+                snapshot = None
+                if span is not None:
+                    shape = (span[1]-span[0], g.Ny, g.Nz, 3)
+                    if getattr(self, 'snap_e', None) is None or tuple(self.snap_e.shape) != shape:
+                        self.snap_e = torch.zeros(shape, device=g.E.device, dtype=g.dtype)
+                        self.snap_h = torch.zeros(shape, device=g.E.device, dtype=g.dtype)
+                    snapshot = (self.snap_e, self.snap_h, span[0], span[1])
+                    first = span[0]*g.Ny*g.Nz*3
+                    E_s, H_s = _Shifted(self.snap_e, first), _Shifted(self.snap_h, first)
+                    self.plane_table_s = self._planes(E_s, H_s)
+                    self.trace_table_s = self._trace_table(E_s, H_s)
+                    self.observe_s = self._observe(self.trace_table_s)
                 self.march2 = [march_descriptor(g, g.E, g.H, self.E_b, self.H_b, psi_a, self.psi_e_b, self.sources,
                                                 self.counter, self.waves2, self.chunk_x2, self.materials,
-                                                psi_h=(psi_h, self.psi_h_b)),
+                                                psi_h=(psi_h, self.psi_h_b), snapshot=snapshot),
                                march_descriptor(g, self.E_b, self.H_b, g.E, g.H, self.psi_e_b, psi_a, self.sources,
                                                 self.counter, self.waves2, self.chunk_x2, self.materials,
-                                                psi_h=(self.psi_h_b, psi_h))]
+                                                psi_h=(self.psi_h_b, psi_h), snapshot=snapshot)]
                 self.march2_tables = [self._march2_table(d) for d in self.march2]
             self.plane_table_b = self._planes(self.E_b, self.H_b)
             self.trace_table_b = self._trace_table(self.E_b, self.H_b)
@@ -459,12 +507,19 @@ class Stepper:
                 'tfdtd_batch_march2')
 
     def quad(self):
-        """Four steps with the two-step kernel: grid buffers -> second set -> grid buffers."""
+        """Four steps with the two-step kernel: grid buffers -> second set -> grid buffers.  After each
+        pass the observers sample step n+1 from the snapshot, then step n+2 from the output buffers (the
+        launches of two one-pass steps, in their order); without observers the observe launches only
+        advance the counter."""
         stream = N.stream_handle(self.grid.device)
-        for k in (0, 1):
+        lib = self.lib
+        observed = self.snapshot_span is not None
+        for k, planes, observe in ((0, self.plane_table_b, self.observe_b), (1, self.plane_table, self.observe)):
             self.march2_step(k, stream)
-            for _ in range(2):     # no traces: the observe launch only advances the counter
-                N.check(self.lib.tfdtd_observe_step(ctypes.byref(self.observe), stream), 'tfdtd_observe_step')
+            for table, obs in (((self.plane_table_s, self.observe_s) if observed else (planes, observe)), (planes, observe)):
+                if self.planes:
+                    N.check(lib.tfdtd_dft_update(table, len(self.planes), stream), 'tfdtd_dft_update')
+                N.check(lib.tfdtd_observe_step(ctypes.byref(obs), stream), 'tfdtd_observe_step')
 
     def step_pair(self):
         """Two steps with the one-pass kernel: grid buffers -> second set -> grid buffers."""
