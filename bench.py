@@ -444,7 +444,9 @@ def main():
     ap.add_argument('--no-batched', action='store_true')
     ap.add_argument('--no-adjoint', action='store_true')
     ap.add_argument('--adjoint-steps', type=int, default=2000, help='steps of the adjoint configuration (2000; fewer for profiling)')
-    ap.add_argument('--no-observed', action='store_true', help='skip the 2D/3D waveguide lines with their FP64 error')
+    ap.add_argument('--no-observed', action='store_true', help='skip the observed lines (waveguides with their FP64 error, '
+                    'the 512^3 cube with a plane monitor and traces)')
+    ap.add_argument('--no-waveguides', action='store_true', help='of the observed lines, run only the 512^3 cube')
     ap.add_argument('--no-surrogate', action='store_true')
     ap.add_argument('--repeats', type=int, default=3, help='further timed samples of --steps steps (median reported)')
     ap.add_argument('--no-tune', action='store_true', help='keep the default launch shape of the cube step')
@@ -611,7 +613,7 @@ def extra_lines(args, line):
     if not args.no_batched:
         line['batched'] = batched_line(args.batch_steps, args.batch_grids, args.loop_grids, args.batch_launch)
     if not args.no_observed:
-        line['observed'] = observed_lines()
+        line['observed'] = {} if args.no_waveguides else observed_lines()
         line['observed']['cube_512_plane_and_traces'] = observed_cube_line()
     if not args.no_adjoint:
         print('bench: adjoint gradient', file=sys.stderr, flush=True)
