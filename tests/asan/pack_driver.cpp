@@ -139,6 +139,12 @@ int main() {
         bad[0].counter = nullptr;
         expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,
                "a missing counter is refused");
+        bad = grids;
+        bad[1].snap_e = bad[1].snap_h = address(910);
+        bad[1].snap_lo = 2;
+        bad[1].snap_hi = 4;
+        expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,
+               "a step n+1 snapshot is refused by the one-step kernel");
     }
     // Two-step tables (float32, both families double-buffered).
     {
@@ -162,6 +168,21 @@ int main() {
         bad[1].dtype = TFDTD_F64;
         expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march2_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,
                "float64 is refused by the two-step pass");
+        std::vector<tfdtd_march> observed = grids;
+        observed[0].snap_e = address(920);
+        observed[0].snap_h = address(930);
+        observed[0].snap_lo = 30;
+        observed[0].snap_hi = 33;
+        expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march2_pack(observed.data(), count, t, b, prefix.data()); }) == TFDTD_OK,
+               "two-step pack with a step n+1 snapshot");
+        bad = observed;
+        bad[0].snap_hi = 41;
+        expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march2_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,
+               "a snapshot past the last plane is refused");
+        bad = observed;
+        bad[0].snap_h = nullptr;
+        expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march2_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,
+               "a snapshot without its H buffer is refused");
     }
     // Split half-step tables.
     {
