@@ -4,7 +4,7 @@ and step n+2 from the fields (the launches of two one-pass steps, in their order
 x-normal plane monitor (E and H, float64 accumulation) and E/H traces runs the same steps on the two-step
 pass (eager and in graphs), the one-pass step and the split step: fields, CPML memory, traces and
 spectra are equal bit for bit (the split and one-pass steps equal the oracle bit for bit in
-test_hip_parity.py and test_configs.py)."""
+test_hip_parity.py and test_configs.py), and the two-step run equals the oracle directly."""
 import numpy as np
 import pytest
 import torch
@@ -79,3 +79,24 @@ def test_observers_spread_over_many_planes_keep_the_one_step_paths(gpu):
     assert run.paths() == ['split', 'one_pass'] and not run.two_step
     with pytest.raises(ValueError, match='two-step'):
         Stepper(project, one_pass=True, two_step=True)
+
+
+def test_two_step_observers_match_the_oracle(gpu):
+    """The same grid for 80 steps on the two-step pass against the oracle of the reference's CPU path:
+    fields and traces bit for bit, the float64 plane to 1e-12 of its largest entry."""
+    from oracle import simulate
+    from torchfdtd_amd.solver import Stepper
+    project = observed_project()
+    run = Stepper(project, one_pass=True, two_step=True)
+    run.advance(80)
+    got = state(run)
+    ref = simulate.run(project.model_dump(), native=True, record_frames=False)
+    np.testing.assert_array_equal(got['E'].numpy(), ref['E'])
+    np.testing.assert_array_equal(got['H'].numpy(), ref['H'])
+    np.testing.assert_array_equal(got['traces'].numpy(), ref['signals'])
+    (expected,), plane = ref['planes'], run.planes[0]
+    names = list(expected.components)
+    values = expected.value[..., [names.index(c) for c in plane.components]]
+    scale = np.abs(values).max()
+    assert scale > 0 and np.abs(ref['signals']).max() > 0
+    np.testing.assert_allclose(got['planes'][0].numpy()/scale, values/scale, rtol=1e-12, atol=1e-12)
