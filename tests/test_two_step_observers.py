@@ -100,3 +100,15 @@ def test_two_step_observers_match_the_oracle(gpu):
     scale = np.abs(values).max()
     assert scale > 0 and np.abs(ref['signals']).max() > 0
     np.testing.assert_allclose(got['planes'][0].numpy()/scale, values/scale, rtol=1e-12, atol=1e-12)
+
+
+def test_launch_tuner_accepts_the_two_step_pass_with_observers(gpu):
+    """tuning.tune_launch times the two-step pass on an observed grid and finds its fields, CPML
+    memory, traces and spectra identical to the split step's."""
+    from torchfdtd_amd.solver import Stepper
+    from torchfdtd_amd.tuning import tune_launch
+    run = Stepper(observed_project())
+    assert run.path == 'split' and 'two_step' in run.paths()
+    record = tune_launch(run, candidates=[('split', {}), ('one_pass', dict(tile_y=8, chunk_x=6)),
+                                          ('two_step', dict(waves2=12, chunk_x2=12))])
+    assert record['rejected'] == [] and len(record['median_ms']) == 3, record

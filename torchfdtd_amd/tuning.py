@@ -132,7 +132,7 @@ def tune_launch(stepper, candidates=None, sweeps=5, units=2, report=None):
         start()
         _unit(stepper)
         torch.cuda.synchronize()
-        result = [t.clone() for t in (stepper.grid.E, stepper.grid.H, *stepper.grid.memory_states)]
+        result = [t.clone() for t in (stepper.grid.E, stepper.grid.H, stepper.traces, *stepper.grid.memory_states)]
         if reference is None:
             reference = result
             accepted.append((path, shape))
