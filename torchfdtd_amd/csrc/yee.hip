@@ -305,7 +305,26 @@ __device__ __forceinline__ real plane_sample(const Plane<real, spec>& a, long lo
     const auto idx = global(a.indices[c]);
     const auto w = global(a.weights[c]);
     real v = real(0);
-    for (int j = 0; j < a.corners; ++j) v = v + field[idx[j * a.points + p]] * w[j * a.points + p];
+    if (a.corners == 8 || a.corners == 4) {
+        // Every corner's index and weight first, then every corner's field value, then the sum in
+        // corner order: the loads overlap instead of forming one round trip per corner.
+        long long at[8];
+        real wt[8], f[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (j < a.corners) {
+                at[j] = idx[j * a.points + p];
+                wt[j] = w[j * a.points + p];
+            }
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (j < a.corners) f[j] = field[at[j]];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (j < a.corners) v = v + f[j] * wt[j];
+    } else {
+        for (int j = 0; j < a.corners; ++j) v = v + field[idx[j * a.points + p]] * w[j * a.points + p];
+    }
     return v * global(electric ? a.window_e : a.window_h)[n];
 }
 
