@@ -26,7 +26,7 @@ from collections import OrderedDict
 
 import torch
 
-TWO_STEP = tuple((w, c) for w in (12, 16) for c in (12, 16, 20, 28))
+TWO_STEP = tuple((w, c) for w in (12, 16) for c in (12, 16, 20, 24, 28))
 ONE_PASS = tuple((t, c) for t in (8, 16) for c in (2, 3, 4, 6, 8))
 NAMES = {'one_pass': ('tile_y', 'chunk_x'), 'two_step': ('waves2', 'chunk_x2'), 'split': ()}
 UNIT_STEPS = 4
