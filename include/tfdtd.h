@@ -184,7 +184,8 @@ int tfdtd_yee_step_march(const tfdtd_march* step, void* stream);
 
 /* Two whole Yee steps, n -> n+2, in one pass over HBM (float32): the step closure of
  * torchfdtd/solver.py:785-797 twice without the observers, with the intermediate step kept
- * on chip.  The result -- E, H and both CPML memories -- equals two calls of
+ * on chip (observers of step n+1 read the snapshot of tfdtd_march.snap_*, launched by the caller
+ * after the pass, then those of step n+2).  The result -- E, H and both CPML memories -- equals two calls of
  * tfdtd_yee_step_march (*_in -> *_out -> *_in) bit for bit, but lands in the *_out set.
  * Time n is read from the *_in buffers and n+2 written to the *_out buffers; unlike the one-step
  * call, the H family's CPML memory is double-buffered too (h.psi_out must differ from h.psi_in
