@@ -7,7 +7,7 @@ Every convolution and linear layer runs on the library's implicit-GEMM kernels
 (``tfdtd_conv_forward`` / ``tfdtd_conv_dgrad`` / ``tfdtd_conv_wgrad``, bf16 operands, FP32
 accumulation); weights are kept in FP32 (the optimiser's master copy) and rounded to bf16
 per call.  The reference has no neural surrogate: this is the learned screen of the
-inverse-design loop, with the FDTD solve as the judge (``screen``).
+inverse-design loop, with the FDTD solve as the arbiter (``screen``).
 """
 from __future__ import annotations
 
@@ -402,7 +402,7 @@ def design_study(data, *, seed=2026, epochs=150, batch=128, lr=1e-3, candidates=
                  baseline_iterations=48, refine_epochs=4, steps=2000, frequencies=32, band=(1.50, 1.60), objectives=None,
                  log=print):
     """Train the surrogate on `data` (transmission [n, 32] of the first n seeded designs), then run one
-    screened and one unscreened design campaign per objective with the FDTD solve as the judge.
+    screened and one unscreened design campaign per objective with the FDTD solve as the arbiter.
     The surrogate is trained once; each campaign starts from that network."""
     import copy
     import time

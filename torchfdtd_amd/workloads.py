@@ -18,7 +18,7 @@
   2.4 um left of centre, a 50x50x6-cell design box at the centre with seeded uniform density
   rho (eps = 2.085 + 10.025 rho) and an x-normal float64 DFT plane 2.4 um right of centre over
   the CPML-free cross-section, sampled at 8 wavelengths 1.50-1.60 um.
-* ``design_slab``: the surrogate's data and judge.  A 64x64x8-cell density slab
+* ``design_slab``: the surrogate's data and arbiter.  A 64x64x8-cell density slab
   (eps = 2.085 + 10.025 rho) in a 96x96x52-cell grid at 0.04 um with 10-cell CPML, lit by an
   Ex plane source below it (1.45-1.65 um broadband) and measured by a 32-wavelength float64
   DFT plane above it; the transmission is the plane's flux over that of the empty slab.
