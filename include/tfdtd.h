@@ -159,13 +159,15 @@ typedef struct tfdtd_march {
     const uint8_t* material;
     const void* material_table;
     int32_t materials;
-    /* Two-step pass only (the one-step kernel refuses them): when snap_hi > snap_lo, E and H of the
-     * intermediate step n+1 on the planes [snap_lo, snap_hi) are also written to snap_e and snap_h,
-     * (snap_hi - snap_lo) * ny * nz * 3 values each in the layout of the fields, so that plane
-     * monitors and traces on those planes can sample step n+1.  Zero when unused. */
+    /* Two-step pass only (the one-step kernel refuses them): E and H of the intermediate step n+1
+     * on snap_count (at most 4) x ranges [snap_lo[k], snap_hi[k]), ascending and disjoint, are also
+     * written to snap_e and snap_h, the ranges' planes one after another in the layout of the fields
+     * (sum of (snap_hi[k] - snap_lo[k]) * ny * nz * 3 values each), so that plane monitors and traces
+     * on those planes can sample step n+1.  snap_count 0 when unused. */
     void* snap_e;
     void* snap_h;
-    int32_t snap_lo, snap_hi;
+    int32_t snap_count;
+    int32_t snap_lo[4], snap_hi[4];
 } tfdtd_march;
 
 /* torchfdtd/reversible_cuda_fused.py:259-325 (FusedEHMarching) generalised to the

@@ -141,8 +141,9 @@ int main() {
                "a missing counter is refused");
         bad = grids;
         bad[1].snap_e = bad[1].snap_h = address(910);
-        bad[1].snap_lo = 2;
-        bad[1].snap_hi = 4;
+        bad[1].snap_count = 1;
+        bad[1].snap_lo[0] = 2;
+        bad[1].snap_hi[0] = 4;
         expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,
                "a step n+1 snapshot is refused by the one-step kernel");
     }
@@ -171,14 +172,25 @@ int main() {
         std::vector<tfdtd_march> observed = grids;
         observed[0].snap_e = address(920);
         observed[0].snap_h = address(930);
-        observed[0].snap_lo = 30;
-        observed[0].snap_hi = 33;
+        observed[0].snap_count = 2;
+        observed[0].snap_lo[0] = 3;
+        observed[0].snap_hi[0] = 5;
+        observed[0].snap_lo[1] = 30;
+        observed[0].snap_hi[1] = 33;
         expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march2_pack(observed.data(), count, t, b, prefix.data()); }) == TFDTD_OK,
-               "two-step pack with a step n+1 snapshot");
+               "two-step pack with a step n+1 snapshot of two ranges");
         bad = observed;
-        bad[0].snap_hi = 41;
+        bad[0].snap_hi[1] = 41;
         expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march2_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,
                "a snapshot past the last plane is refused");
+        bad = observed;
+        bad[0].snap_lo[1] = 4;
+        expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march2_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,
+               "overlapping snapshot ranges are refused");
+        bad = observed;
+        bad[0].snap_count = 5;
+        expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march2_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,
+               "more than four snapshot ranges are refused");
         bad = observed;
         bad[0].snap_h = nullptr;
         expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march2_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,

@@ -12,19 +12,26 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
-def observed_project(steps=80, trace_x=1.0):
+def observed_project(steps=80, trace_x=1.0, reflection=False, wide=False):
+    """x-normal plane near high x and two traces; `reflection` adds an x-normal plane near low x (two
+    snapshot ranges), `wide` a y-normal plane across most of x (more planes than a snapshot takes)."""
     from torchfdtd_amd import FieldMonitor, Monitor, Project, Region, Source, Structure
     from torchfdtd_amd.models import SpectrumSettings
     region = Region(dimension='3d', size=(3.2, 1.6, 1.2), mesh=.04, pml_cells=8, steps=steps, backend='cuda')
     spectrum = SpectrumSettings(sampling='wavelength', frequency_points=8, wavelength_start=1.45, wavelength_stop=1.65,
                                 apodization='none')
+    monitors = [FieldMonitor(id='plane', center=(1.0, 0, 0), normal='x', size=(0, .8, .5), dft_precision='float64',
+                             spectrum=spectrum),
+                Monitor(id='ez', component='Ez', center=(trace_x, .1, 0)),
+                Monitor(id='hy', component='Hy', center=(trace_x + .04, 0, .05))]
+    if reflection:
+        monitors.append(FieldMonitor(id='reflection', center=(-.9, 0, 0), normal='x', size=(0, .8, .5), spectrum=spectrum))
+    if wide:
+        monitors.append(FieldMonitor(id='side', center=(0, .2, 0), normal='y', size=(2.0, 0, .5), spectrum=spectrum))
     return Project(region=region,
                    structures=[Structure(center=(0, 0, 0), size=(2.4, .4, .3), material='Si (constant n)')],
                    sources=[Source(center=(.8, 0, 0), component='Ez', wavelength=1, pulse_cycles=1)],
-                   monitors=[FieldMonitor(id='plane', center=(1.0, 0, 0), normal='x', size=(0, .8, .5),
-                                          dft_precision='float64', spectrum=spectrum),
-                             Monitor(id='ez', component='Ez', center=(trace_x, .1, 0)),
-                             Monitor(id='hy', component='Hy', center=(trace_x + .04, 0, .05))])
+                   monitors=monitors)
 
 
 def state(run):
@@ -45,10 +52,13 @@ def assert_same(got, expected):
         assert torch.equal(a, b)
 
 
-def test_two_step_observers_equal_the_one_step_paths(gpu):
+@pytest.mark.parametrize('reflection', [False, True])
+def test_two_step_observers_equal_the_one_step_paths(gpu, reflection):
+    """One snapshot range (a plane and two traces near high x), and two (a reflection plane near low x
+    as well)."""
     from torchfdtd_amd.graphs import StepGraphs
     from torchfdtd_amd.solver import Stepper
-    project = observed_project()
+    project = observed_project(reflection=reflection)
     results = {}
     for path in ('split', 'one_pass', 'two_step'):
         run = Stepper(project, one_pass=path != 'split', two_step=path == 'two_step')
@@ -56,8 +66,10 @@ def test_two_step_observers_equal_the_one_step_paths(gpu):
         run.advance(80)
         results[path] = state(run)
         if path == 'two_step':
-            lo, hi = run.snapshot_span
-            assert 0 < hi - lo <= run.grid.Nx // 8 and run.snap_e.shape[0] == hi - lo
+            ranges = run.snapshot_ranges
+            total = sum(hi - lo for lo, hi in ranges)
+            assert len(ranges) == (2 if reflection else 1) and 0 < total <= run.grid.Nx // 8
+            assert run.snap_e.shape[0] == total
     reference = results['split']
     assert np.abs(reference['traces'].numpy()).max() > 0 and reference['planes'][0].abs().max() > 0
     assert_same(results['one_pass'], reference)
@@ -72,10 +84,9 @@ def test_two_step_observers_equal_the_one_step_paths(gpu):
 
 def test_observers_spread_over_many_planes_keep_the_one_step_paths(gpu):
     from torchfdtd_amd.solver import Stepper
-    project = observed_project(trace_x=-1.2)      # the Ez trace 55 planes before the monitor plane
+    project = observed_project(wide=True)      # a y-normal plane across 50 x planes
     run = Stepper(project, one_pass=True)
-    span = run.observed_planes()
-    assert span[1] - span[0] > run.grid.Nx // 8
+    assert sum(hi - lo for lo, hi in run.observed_planes()) > run.grid.Nx // 8
     assert run.paths() == ['split', 'one_pass'] and not run.two_step
     with pytest.raises(ValueError, match='two-step'):
         Stepper(project, one_pass=True, two_step=True)

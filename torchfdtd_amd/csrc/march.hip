@@ -89,11 +89,23 @@ struct March {
     int yshift;                 // y tile ty starts at ty*9 - yshift
     int iy0, iy1, iytiles;      // tall interior tiles: rows [iy0, iy1) in iytiles tiles of 13 rows
     int nxend;                  // boundary tiles in the x end chunks (the rest lie in x-middle chunks)
-    // Two-step pass: E and H of step n+1 on planes [snap_lo, snap_hi) are also stored here.
+    // Two-step pass: E and H of step n+1 on the planes of snap_count ranges [seg_lo[k], seg_hi[k])
+    // are also stored here, range k from snapshot plane seg_base[k] on; [snap_lo, snap_hi) is their hull.
     real* snap_e;
     real* snap_h;
     int snap_lo, snap_hi;
+    int snap_count;
+    int seg_lo[4], seg_hi[4], seg_base[4];
 };
+
+// The snapshot plane that holds grid plane x, or -1 (read from the descriptor: only planes inside the
+// ranges' hull ask).
+template <typename real>
+__device__ __forceinline__ int snapshot_plane(const March<real>& a, int x) {
+    for (int k = 0; k < a.snap_count; ++k)
+        if (x >= a.seg_lo[k] && x < a.seg_hi[k]) return a.seg_base[k] + x - a.seg_lo[k];
+    return -1;
+}
 
 // psi <- psi*b + c*d (stored when `write`), d <- d*inv_k + psi, for slab row q of term k.
 template <typename real>
@@ -1036,8 +1048,10 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                     e1[k] = sel3(lanes, v);
                     // Snapshot of step n+1: the cells this tile owns (those whose E2 it stores).
                     if (x >= q.snap_lo && x < q.snap_hi && x >= xa && x < xb && r >= 1 && r < NR - 2 && lane >= 1 &&
-                        lane < 62 && zin && y < ylim)
-                        store3(a.snap_e, 3u * ((unsigned)(x - q.snap_lo) * plane + (unsigned)y * nz + (unsigned)z), v);
+                        lane < 62 && zin && y < ylim) {
+                        const int sp = snapshot_plane(a, x);
+                        if (sp >= 0) store3(a.snap_e, 3u * ((unsigned)sp * plane + (unsigned)y * nz + (unsigned)z), v);
+                    }
                 }
             }
             // Stage 1b: H1(x-1) from E1(x-1) (LDS, with the row and column above) and E1(x).
@@ -1059,8 +1073,10 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                     add_sources(a.sh, q.nsh, q.hbl, q.hbh, n, q.samples, x - 1, y, z, v.x, v.y, v.z);
                     h1[k] = sel3(lanes, v);
                     if (x - 1 >= q.snap_lo && x - 1 < q.snap_hi && x - 1 >= xa && x - 1 < xb && r >= 1 && r < NR - 2 &&
-                        lane >= 1 && lane < 62 && zin && y < ylim)
-                        store3(a.snap_h, 3u * ((unsigned)(x - 1 - q.snap_lo) * plane + (unsigned)y * nz + (unsigned)z), v);
+                        lane >= 1 && lane < 62 && zin && y < ylim) {
+                        const int sp = snapshot_plane(a, x - 1);
+                        if (sp >= 0) store3(a.snap_h, 3u * ((unsigned)sp * plane + (unsigned)y * nz + (unsigned)z), v);
+                    }
                 }
                 put3(sH1 + 3 * (r * 64 + lane), h1[k]);
             }
@@ -1255,7 +1271,7 @@ inline void free_range(int tiles, int size, int e_extra, int n, const Face* ef, 
 // Tile layout of one grid: chunking, z shift, the slab-free boxes and the workgroups of each variant.
 template <typename real, int W>
 int layout_march(const tfdtd_march* d, March<real>* a) {
-    if (d->snap_hi > d->snap_lo) return tfdtd_fail(TFDTD_EINVAL, "march: the step n+1 snapshot is an option of the two-step pass");
+    if (d->snap_count != 0) return tfdtd_fail(TFDTD_EINVAL, "march: the step n+1 snapshot is an option of the two-step pass");
     const int status = make_march(d, a);
     if (status) return status;
     constexpr int TY = 2 * W;
@@ -1457,13 +1473,24 @@ int layout_march2(const tfdtd_march* d, March<float>* a) {
     if (h_slabs && (!d->h.psi_out || d->h.psi_out == d->h.psi_in))
         return tfdtd_fail(TFDTD_EINVAL, "march2: the H-family CPML memory needs separate input and output buffers");
     a->hf.psi_out = static_cast<float*>(h_slabs ? d->h.psi_out : const_cast<void*>(d->h.psi_in));
-    if (d->snap_hi > d->snap_lo) {
-        if (d->snap_lo < 0 || d->snap_hi > d->nx || !d->snap_e || !d->snap_h)
-            return tfdtd_fail(TFDTD_EINVAL, "march2: the step n+1 snapshot needs planes inside [0, nx) and an E and an H buffer");
+    if (d->snap_count != 0) {
+        if (d->snap_count < 0 || d->snap_count > 4 || !d->snap_e || !d->snap_h)
+            return tfdtd_fail(TFDTD_EINVAL, "march2: the step n+1 snapshot takes 1 to 4 ranges and an E and an H buffer");
+        int base = 0;
+        for (int k = 0; k < d->snap_count; ++k) {
+            const int lo = d->snap_lo[k], hi = d->snap_hi[k];
+            if (lo < 0 || hi > d->nx || hi <= lo || (k > 0 && lo < d->snap_hi[k - 1]))
+                return tfdtd_fail(TFDTD_EINVAL, "march2: snapshot range %d is empty, outside [0, nx) or not after the one before", k);
+            a->seg_lo[k] = lo;
+            a->seg_hi[k] = hi;
+            a->seg_base[k] = base;
+            base += hi - lo;
+        }
         a->snap_e = static_cast<float*>(d->snap_e);
         a->snap_h = static_cast<float*>(d->snap_h);
-        a->snap_lo = d->snap_lo;
-        a->snap_hi = d->snap_hi;
+        a->snap_count = d->snap_count;
+        a->snap_lo = d->snap_lo[0];
+        a->snap_hi = d->snap_hi[d->snap_count - 1];
     }
     const int chunk = d->chunk_x > 0 ? d->chunk_x : d->nx;
     a->chunk = chunk;

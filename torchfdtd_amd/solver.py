@@ -42,9 +42,25 @@ ENGINE_LOCK = threading.Lock()
 TWO_STEP_MIN_CELLS = 20_000_000
 ONE_PASS_MIN_CELLS = 48_000_000
 # The two-step pass serves plane monitors and traces by writing step n+1 on the planes they read to a
-# snapshot; grids whose observers spread over more than an eighth of the x planes keep the one-step
-# paths (the snapshot would cost more than the pass saves).
+# snapshot of at most SNAPSHOT_RANGES x ranges; grids whose observers need more than an eighth of the x
+# planes keep the one-step paths (the snapshot would cost more than the pass saves).
 SNAPSHOT_MAX_FRACTION = 8
+SNAPSHOT_RANGES = 4
+
+
+def merge_ranges(spans, most=SNAPSHOT_RANGES):
+    """Ascending disjoint half-open ranges covering `spans`: overlapping or touching ranges joined, then
+    the two nearest neighbours joined until at most `most` remain."""
+    out = []
+    for lo, hi in sorted(spans):
+        if out and lo <= out[-1][1]:
+            out[-1] = (out[-1][0], max(out[-1][1], hi))
+        else:
+            out.append((lo, hi))
+    while len(out) > most:
+        k = min(range(len(out)-1), key=lambda i: out[i+1][0]-out[i][1])
+        out[k:k+2] = [(out[k][0], out[k+1][1])]
+    return out
 
 
 class _Shifted:
@@ -282,7 +298,7 @@ class Stepper:
         self.two_step = self._two_step_allowed() if two_step is None else bool(two_step)
         if self.two_step and not self._two_step_allowed():
             raise ValueError('the two-step kernel needs a float32 one-pass grid whose plane monitors and point traces '
-                             f'read at most 1/{SNAPSHOT_MAX_FRACTION} of the x planes')
+                             f'read at most 1/{SNAPSHOT_MAX_FRACTION} of the x planes in at most {SNAPSHOT_RANGES} ranges')
         # One-byte material index of 1/eps for the one-pass step (grids with few distinct values).
         self.materials = MaterialIndex(g) if self.one_pass else None
         self._describe()
@@ -295,28 +311,33 @@ class Stepper:
         elif launch is not None:
             raise ValueError("launch is None (default shapes) or 'auto' (measured)")
 
-    def observed_planes(self):
-        """[lo, hi): the x planes that the plane monitors and traces read (None without observers)."""
-        g = self.grid
-        per_plane = 3*g.Ny*g.Nz
-        xs = [flat // per_plane for _, flat in self.trace_points]
+    def _observer_spans(self):
+        """x planes [lo, hi) each trace and each plane monitor reads (traces first, in order)."""
+        per_plane = 3*self.grid.Ny*self.grid.Nz
+        spans = [(flat // per_plane, flat // per_plane + 1) for _, flat in self.trace_points]
         for plane in self.planes:
-            for indices, _ in plane.maps:
-                xs += [int(indices.min()) // per_plane, int(indices.max()) // per_plane]
-        return (min(xs), max(xs)+1) if xs else None
+            spans.append((min(int(indices.min()) for indices, _ in plane.maps) // per_plane,
+                          max(int(indices.max()) for indices, _ in plane.maps) // per_plane + 1))
+        return spans
+
+    def observed_planes(self):
+        """The x ranges a step n+1 snapshot covers: every plane the observers read, merged into at most
+        SNAPSHOT_RANGES ranges ([] without observers)."""
+        return merge_ranges(self._observer_spans())
 
     def _two_step_eligible(self):
         if self.grid.dtype != torch.float32:
             return False
-        span = self.observed_planes()
-        return span is None or span[1]-span[0] <= max(2, self.grid.Nx // SNAPSHOT_MAX_FRACTION)
+        planes = sum(hi-lo for lo, hi in self.observed_planes())
+        return planes <= max(2, self.grid.Nx // SNAPSHOT_MAX_FRACTION)
 
     def _two_step_allowed(self):
         return self.one_pass and self._two_step_eligible()
 
     def paths(self):
         """The step paths this grid can take: 'split' always; 'one_pass' on one-pass grids; 'two_step'
-        on float32 one-pass grids whose observers read at most 1/SNAPSHOT_MAX_FRACTION of the x planes."""
+        on float32 one-pass grids whose observers read at most 1/SNAPSHOT_MAX_FRACTION of the x planes
+        (in at most SNAPSHOT_RANGES merged ranges)."""
         out = ['split']
         if self.one_pass_supported:
             out.append('one_pass')
@@ -409,19 +430,34 @@ class Stepper:
                     self.psi_h_b = torch.zeros_like(g.psi_buffers[True])
                 psi_h = g.psi_buffers[True]
                 # Observers sample step n+1 from a snapshot of the planes they read, which the pass
-                # writes; a plane table and a trace table point into it as into a full field.
-                self.snapshot_span = span = self.observed_planes()
+                # writes, its ranges one after another; each plane and trace table entry points into
+                # it as into a full field.
+                self.snapshot_ranges = ranges = self.observed_planes()
                 snapshot = None
-                if span is not None:
-                    shape = (span[1]-span[0], g.Ny, g.Nz, 3)
+                if ranges:
+                    shape = (sum(hi-lo for lo, hi in ranges), g.Ny, g.Nz, 3)
                     if getattr(self, 'snap_e', None) is None or tuple(self.snap_e.shape) != shape:
                         self.snap_e = torch.zeros(shape, device=g.E.device, dtype=g.dtype)
                         self.snap_h = torch.zeros(shape, device=g.E.device, dtype=g.dtype)
-                    snapshot = (self.snap_e, self.snap_h, span[0], span[1])
-                    first = span[0]*g.Ny*g.Nz*3
-                    E_s, H_s = _Shifted(self.snap_e, first), _Shifted(self.snap_h, first)
-                    self.plane_table_s = self._planes(E_s, H_s)
-                    self.trace_table_s = self._trace_table(E_s, H_s)
+                    snapshot = (self.snap_e, self.snap_h, ranges)
+                    per_plane, starts, base = g.Ny*g.Nz*3, [], 0
+                    for lo, hi in ranges:
+                        starts.append((lo, hi, lo-base))   # snapshot plane of grid plane x: x - (lo - base)
+                        base += hi-lo
+
+                    def shift(span):
+                        (offset,) = [o for lo, hi, o in starts if lo <= span[0] and span[1] <= hi]
+                        return offset*per_plane
+
+                    spans = self._observer_spans()
+                    trace_spans, plane_spans = spans[:len(self.trace_points)], spans[len(self.trace_points):]
+                    self.plane_table_s = (N.Plane*max(1, len(self.planes)))()
+                    for k, (plane, span) in enumerate(zip(self.planes, plane_spans)):
+                        self.plane_table_s[k] = plane.descriptor(self.counter, _Shifted(self.snap_e, shift(span)),
+                                                                 _Shifted(self.snap_h, shift(span)))
+                    rows = [(_Shifted(self.snap_h if magnetic else self.snap_e, shift(span)).data_ptr(), flat)
+                            for (magnetic, flat), span in zip(self.trace_points, trace_spans)]
+                    self.trace_table_s = torch.as_tensor(np.array(rows, dtype=np.int64).reshape(-1, 2), device=g.E.device)
                     self.observe_s = self._observe(self.trace_table_s)
                 self.march2 = [march_descriptor(g, g.E, g.H, self.E_b, self.H_b, psi_a, self.psi_e_b, self.sources,
                                                 self.counter, self.waves2, self.chunk_x2, self.materials,
@@ -513,7 +549,7 @@ class Stepper:
         advance the counter."""
         stream = N.stream_handle(self.grid.device)
         lib = self.lib
-        observed = self.snapshot_span is not None
+        observed = bool(self.snapshot_ranges)
         for k, planes, observe in ((0, self.plane_table_b, self.observe_b), (1, self.plane_table, self.observe)):
             self.march2_step(k, stream)
             for table, obs in (((self.plane_table_s, self.observe_s) if observed else (planes, observe)), (planes, observe)):
