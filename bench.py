@@ -193,8 +193,8 @@ def observed_cpu_baseline(project):
 def observed_cube_line(steps=96, width=48):
     """The 512^3 cube's workload (16-cell CPML, seeded permittivity, Ez dipole) with an x-normal
     16-frequency float64 DFT plane (E and H, 400 x 400 points) and an Ez and an Hy trace near high x:
-    per-step time on the default path, the two-step pass (its observers sample step n+1 from the
-    snapshot of the three planes they read), and on the one-step kernel, both in graphs of `width`
+    per-step time on the default path, the two-step pass (its observers sample step n+1 from a
+    snapshot of the cells they read), and on the one-step kernel, both in graphs of `width`
     steps after a warm-up graph."""
     import torch
 
@@ -227,7 +227,7 @@ def observed_cube_line(steps=96, width=48):
         torch.cuda.synchronize()
         ms = (time.perf_counter()-t0)/steps*1e3
         out[name] = dict(path=run.path, ms_per_step=round(ms, 4), value=round(cells/ms/1e6, 3), unit='Gcell-updates/s',
-                         snapshot_planes=int(run.snap_e.shape[0]) if run.two_step and run.snapshot_ranges else None)
+                         snapshot_cells=int(run.snap_e.shape[0]) if run.two_step and run.snapshot_ranges else None)
         del run, graphs
         torch.cuda.empty_cache()
     out['speedup'] = round(out['one_step']['ms_per_step']/out['default']['ms_per_step'], 3)

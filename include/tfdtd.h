@@ -160,14 +160,14 @@ typedef struct tfdtd_march {
     const void* material_table;
     int32_t materials;
     /* Two-step pass only (the one-step kernel refuses them): E and H of the intermediate step n+1
-     * on snap_count (at most 4) x ranges [snap_lo[k], snap_hi[k]), ascending and disjoint, are also
-     * written to snap_e and snap_h, the ranges' planes one after another in the layout of the fields
-     * (sum of (snap_hi[k] - snap_lo[k]) * ny * nz * 3 values each), so that plane monitors and traces
-     * on those planes can sample step n+1.  snap_count 0 when unused. */
+     * in snap_count (at most 4) disjoint boxes of cells [snap_lo[k][a], snap_hi[k][a]) are also
+     * written to snap_e and snap_h: box k's cells after box k-1's, each box x-major like the
+     * fields, 3 values per cell, so that plane monitors and traces inside the boxes can sample step
+     * n+1 through indices into these buffers.  snap_count 0 when unused. */
     void* snap_e;
     void* snap_h;
     int32_t snap_count;
-    int32_t snap_lo[4], snap_hi[4];
+    int32_t snap_lo[4][3], snap_hi[4][3];
 } tfdtd_march;
 
 /* torchfdtd/reversible_cuda_fused.py:259-325 (FusedEHMarching) generalised to the

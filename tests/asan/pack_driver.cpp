@@ -142,8 +142,7 @@ int main() {
         bad = grids;
         bad[1].snap_e = bad[1].snap_h = address(910);
         bad[1].snap_count = 1;
-        bad[1].snap_lo[0] = 2;
-        bad[1].snap_hi[0] = 4;
+        for (int ax = 0; ax < 3; ++ax) { bad[1].snap_lo[0][ax] = 2; bad[1].snap_hi[0][ax] = 4; }
         expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,
                "a step n+1 snapshot is refused by the one-step kernel");
     }
@@ -173,24 +172,27 @@ int main() {
         observed[0].snap_e = address(920);
         observed[0].snap_h = address(930);
         observed[0].snap_count = 2;
-        observed[0].snap_lo[0] = 3;
-        observed[0].snap_hi[0] = 5;
-        observed[0].snap_lo[1] = 30;
-        observed[0].snap_hi[1] = 33;
+        const int box[2][2][3] = {{{3, 0, 0}, {5, 30, 70}}, {{30, 10, 5}, {33, 20, 60}}};
+        for (int k = 0; k < 2; ++k)
+            for (int ax = 0; ax < 3; ++ax) { observed[0].snap_lo[k][ax] = box[k][0][ax]; observed[0].snap_hi[k][ax] = box[k][1][ax]; }
         expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march2_pack(observed.data(), count, t, b, prefix.data()); }) == TFDTD_OK,
-               "two-step pack with a step n+1 snapshot of two ranges");
+               "two-step pack with a step n+1 snapshot of two boxes");
         bad = observed;
-        bad[0].snap_hi[1] = 41;
+        bad[0].snap_hi[1][0] = 41;
         expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march2_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,
                "a snapshot past the last plane is refused");
         bad = observed;
-        bad[0].snap_lo[1] = 4;
+        bad[0].snap_lo[1][0] = 4;
         expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march2_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,
-               "overlapping snapshot ranges are refused");
+               "overlapping snapshot boxes are refused");
+        bad = observed;
+        bad[0].snap_hi[0][2] = 71;
+        expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march2_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,
+               "a snapshot box past the grid's z extent is refused");
         bad = observed;
         bad[0].snap_count = 5;
         expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march2_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,
-               "more than four snapshot ranges are refused");
+               "more than four snapshot boxes are refused");
         bad = observed;
         bad[0].snap_h = nullptr;
         expect(pack_exact(bytes, [&](void* t, size_t b) { return tfdtd_batch_march2_pack(bad.data(), count, t, b, prefix.data()); }) != TFDTD_OK,

@@ -1,6 +1,6 @@
 """The default step path by grid size (solver.default_step_path), CPU: the thresholds sit at the
 measured crossovers of the split step, the one-pass step and the two-step pass."""
-from torchfdtd_amd.solver import ONE_PASS_MIN_CELLS, TWO_STEP_MIN_CELLS, default_step_path, merge_ranges
+from torchfdtd_amd.solver import ONE_PASS_MIN_CELLS, TWO_STEP_MIN_CELLS, default_step_path, merge_boxes
 
 
 def test_default_step_path_follows_the_measured_crossovers():
@@ -15,16 +15,18 @@ def test_default_step_path_follows_the_measured_crossovers():
     assert default_step_path(512**3, True, False) == 'split'       # no one-pass kernel for this grid
 
 
-def test_snapshot_ranges_merge_to_at_most_four():
-    """The x ranges of a step n+1 snapshot: overlapping and touching spans joined, then the nearest
-    neighbours until four remain; every span lies inside one range."""
-    assert merge_ranges([]) == []
-    assert merge_ranges([(64, 67), (65, 66), (66, 67)]) == [(64, 67)]
-    assert merge_ranges([(10, 12), (12, 13), (64, 67)]) == [(10, 13), (64, 67)]
-    spans = [(0, 1), (10, 11), (20, 22), (30, 31), (33, 34), (60, 62)]
-    merged = merge_ranges(spans)      # gaps 9, 9, 8, 2, 26: the 2-plane gap closes, then the 8-plane one
-    assert merged == [(0, 1), (10, 11), (20, 34), (60, 62)]
-    assert merge_ranges(spans, most=5) == [(0, 1), (10, 11), (20, 22), (30, 34), (60, 62)]
-    for lo, hi in spans:
-        assert any(a <= lo and hi <= b for a, b in merged)
-    assert all(a[1] < b[0] for a, b in zip(merged, merged[1:]))
+def test_snapshot_boxes_merge_to_at_most_four():
+    """The boxes of a step n+1 snapshot: overlapping and touching boxes replaced by their bounding box,
+    then the pair that adds the fewest cells, until four remain; every observer box lies inside one."""
+    assert merge_boxes([]) == []
+    plane = ((64, 10, 9), (67, 31, 22))
+    traces = [((65, 22, 15), (66, 23, 16)), ((66, 20, 16), (67, 21, 17))]
+    assert merge_boxes([plane, *traces]) == [plane]                     # traces inside the plane's box
+    far = ((17, 10, 9), (19, 31, 22))
+    assert merge_boxes([plane, far]) == [far, plane]                    # two disjoint boxes stay two
+    assert merge_boxes([((0, 0, 0), (1, 1, 1)), ((1, 0, 0), (2, 1, 1))]) == [((0, 0, 0), (2, 1, 1))]   # touching
+    points = [((x, y, z), (x + 1, y + 1, z + 1)) for x, y, z in ((0, 0, 0), (0, 0, 9), (50, 0, 0), (50, 30, 0), (50, 30, 9))]
+    merged = merge_boxes(points)
+    assert len(merged) == 4 and ((0, 0, 0), (1, 1, 10)) in merged      # the pair 9 cells apart joins first
+    for lo, hi in points:
+        assert any(all(a[k] <= lo[k] and hi[k] <= b[k] for k in range(3)) for a, b in merged)

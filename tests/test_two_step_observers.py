@@ -5,6 +5,8 @@ x-normal plane monitor (E and H, float64 accumulation) and E/H traces runs the s
 pass (eager and in graphs), the one-pass step and the split step: fields, CPML memory, traces and
 spectra are equal bit for bit (the split and one-pass steps equal the oracle bit for bit in
 test_hip_parity.py and test_configs.py), and the two-step run equals the oracle directly."""
+import math
+
 import numpy as np
 import pytest
 import torch
@@ -12,22 +14,26 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
-def observed_project(steps=80, trace_x=1.0, reflection=False, wide=False):
+def observed_project(steps=80, trace_x=1.0, reflection=False, wide=False, scattered=False):
     """x-normal plane near high x and two traces; `reflection` adds an x-normal plane near low x (two
-    snapshot ranges), `wide` a y-normal plane across most of x (more planes than a snapshot takes)."""
+    snapshot boxes), `wide` a y-normal plane along x beside it (two boxes), `scattered` a z-normal and a
+    y-normal plane across each other (one box of more cells than a snapshot takes)."""
     from torchfdtd_amd import FieldMonitor, Monitor, Project, Region, Source, Structure
     from torchfdtd_amd.models import SpectrumSettings
     region = Region(dimension='3d', size=(3.2, 1.6, 1.2), mesh=.04, pml_cells=8, steps=steps, backend='cuda')
     spectrum = SpectrumSettings(sampling='wavelength', frequency_points=8, wavelength_start=1.45, wavelength_stop=1.65,
                                 apodization='none')
-    monitors = [FieldMonitor(id='plane', center=(1.0, 0, 0), normal='x', size=(0, .8, .5), dft_precision='float64',
+    monitors = [FieldMonitor(id='plane', center=(1.0, 0, 0), normal='x', size=(0, .4, .5), dft_precision='float64',
                              spectrum=spectrum),
                 Monitor(id='ez', component='Ez', center=(trace_x, .1, 0)),
                 Monitor(id='hy', component='Hy', center=(trace_x + .04, 0, .05))]
     if reflection:
-        monitors.append(FieldMonitor(id='reflection', center=(-.9, 0, 0), normal='x', size=(0, .8, .5), spectrum=spectrum))
+        monitors.append(FieldMonitor(id='reflection', center=(-.9, 0, 0), normal='x', size=(0, .4, .5), spectrum=spectrum))
     if wide:
-        monitors.append(FieldMonitor(id='side', center=(0, .2, 0), normal='y', size=(2.0, 0, .5), spectrum=spectrum))
+        monitors.append(FieldMonitor(id='side', center=(0, -.3, 0), normal='y', size=(2.0, 0, .5), spectrum=spectrum))
+    if scattered:
+        monitors += [FieldMonitor(id='floor', center=(0, 0, 0), normal='z', size=(2.0, .8, 0), spectrum=spectrum),
+                     FieldMonitor(id='wall', center=(0, 0, 0), normal='y', size=(2.0, 0, .5), spectrum=spectrum)]
     return Project(region=region,
                    structures=[Structure(center=(0, 0, 0), size=(2.4, .4, .3), material='Si (constant n)')],
                    sources=[Source(center=(.8, 0, 0), component='Ez', wavelength=1, pulse_cycles=1)],
@@ -52,13 +58,14 @@ def assert_same(got, expected):
         assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize('reflection', [False, True])
-def test_two_step_observers_equal_the_one_step_paths(gpu, reflection):
-    """One snapshot range (a plane and two traces near high x), and two (a reflection plane near low x
-    as well)."""
+@pytest.mark.parametrize('layout', ['one box', 'reflection', 'wide'])
+def test_two_step_observers_equal_the_one_step_paths(gpu, layout):
+    """One snapshot box (a plane and two traces near high x), two (a reflection plane near low x as
+    well, or a y-normal plane along x beside the x-normal one)."""
+    reflection = layout == 'reflection'
     from torchfdtd_amd.graphs import StepGraphs
     from torchfdtd_amd.solver import Stepper
-    project = observed_project(reflection=reflection)
+    project = observed_project(reflection=reflection, wide=layout == 'wide')
     results = {}
     for path in ('split', 'one_pass', 'two_step'):
         run = Stepper(project, one_pass=path != 'split', two_step=path == 'two_step')
@@ -66,9 +73,9 @@ def test_two_step_observers_equal_the_one_step_paths(gpu, reflection):
         run.advance(80)
         results[path] = state(run)
         if path == 'two_step':
-            ranges = run.snapshot_ranges
-            total = sum(hi - lo for lo, hi in ranges)
-            assert len(ranges) == (2 if reflection else 1) and 0 < total <= run.grid.Nx // 8
+            boxes = run.snapshot_ranges
+            total = sum(math.prod(b - a for a, b in zip(lo, hi)) for lo, hi in boxes)
+            assert len(boxes) == (1 if layout == 'one box' else 2) and 0 < total <= run.grid.E.numel() // 3 // 8, boxes
             assert run.snap_e.shape[0] == total
     reference = results['split']
     assert np.abs(reference['traces'].numpy()).max() > 0 and reference['planes'][0].abs().max() > 0
@@ -82,11 +89,12 @@ def test_two_step_observers_equal_the_one_step_paths(gpu, reflection):
     assert_same(state(run), reference)
 
 
-def test_observers_spread_over_many_planes_keep_the_one_step_paths(gpu):
+def test_observers_spread_over_many_cells_keep_the_one_step_paths(gpu):
     from torchfdtd_amd.solver import Stepper
-    project = observed_project(wide=True)      # a y-normal plane across 50 x planes
+    project = observed_project(scattered=True)
     run = Stepper(project, one_pass=True)
-    assert sum(hi - lo for lo, hi in run.observed_planes()) > run.grid.Nx // 8
+    boxes = run.observed_planes()
+    assert sum(math.prod(b - a for a, b in zip(lo, hi)) for lo, hi in boxes) > run.grid.E.numel() // 3 // 8
     assert run.paths() == ['split', 'one_pass'] and not run.two_step
     with pytest.raises(ValueError, match='two-step'):
         Stepper(project, one_pass=True, two_step=True)

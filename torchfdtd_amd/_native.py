@@ -111,7 +111,7 @@ class March(ctypes.Structure):
                 ('nsrc_h', i32), ('srcs_h', SourceBox*MAX_SOURCES),
                 ('source_samples', i64), ('counter', vp), ('tile_y', i32), ('chunk_x', i32),
                 ('material', vp), ('material_table', vp), ('materials', i32),
-                ('snap_e', vp), ('snap_h', vp), ('snap_count', i32), ('snap_lo', i32*4), ('snap_hi', i32*4)]
+                ('snap_e', vp), ('snap_h', vp), ('snap_count', i32), ('snap_lo', (i32*3)*4), ('snap_hi', (i32*3)*4)]
 
 
 class NormGrid(ctypes.Structure):

@@ -89,21 +89,27 @@ struct March {
     int yshift;                 // y tile ty starts at ty*9 - yshift
     int iy0, iy1, iytiles;      // tall interior tiles: rows [iy0, iy1) in iytiles tiles of 13 rows
     int nxend;                  // boundary tiles in the x end chunks (the rest lie in x-middle chunks)
-    // Two-step pass: E and H of step n+1 on the planes of snap_count ranges [seg_lo[k], seg_hi[k])
-    // are also stored here, range k from snapshot plane seg_base[k] on; [snap_lo, snap_hi) is their hull.
+    // Two-step pass: E and H of step n+1 in snap_count boxes [sbox_lo[k], sbox_hi[k]) are also stored
+    // here, box k from snapshot cell sbox_base[k] on; [snap_lo, snap_hi) is the boxes' x hull.
     real* snap_e;
     real* snap_h;
     int snap_lo, snap_hi;
     int snap_count;
-    int seg_lo[4], seg_hi[4], seg_base[4];
+    int sbox_lo[4][3], sbox_hi[4][3];
+    unsigned sbox_base[4];
 };
 
-// The snapshot plane that holds grid plane x, or -1 (read from the descriptor: only planes inside the
-// ranges' hull ask).
+// The snapshot cell that holds cell (x, y, z), or -1 (read from the descriptor: only planes inside the
+// boxes' x hull ask).
 template <typename real>
-__device__ __forceinline__ int snapshot_plane(const March<real>& a, int x) {
-    for (int k = 0; k < a.snap_count; ++k)
-        if (x >= a.seg_lo[k] && x < a.seg_hi[k]) return a.seg_base[k] + x - a.seg_lo[k];
+__device__ __forceinline__ int snapshot_cell(const March<real>& a, int x, int y, int z) {
+    for (int k = 0; k < a.snap_count; ++k) {
+        const int* lo = a.sbox_lo[k];
+        const int* hi = a.sbox_hi[k];
+        if (x >= lo[0] && x < hi[0] && y >= lo[1] && y < hi[1] && z >= lo[2] && z < hi[2])
+            return (int)(a.sbox_base[k] + ((unsigned)(x - lo[0]) * (unsigned)(hi[1] - lo[1]) + (unsigned)(y - lo[1])) *
+                                              (unsigned)(hi[2] - lo[2]) + (unsigned)(z - lo[2]));
+    }
     return -1;
 }
 
@@ -1049,8 +1055,8 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                     // Snapshot of step n+1: the cells this tile owns (those whose E2 it stores).
                     if (x >= q.snap_lo && x < q.snap_hi && x >= xa && x < xb && r >= 1 && r < NR - 2 && lane >= 1 &&
                         lane < 62 && zin && y < ylim) {
-                        const int sp = snapshot_plane(a, x);
-                        if (sp >= 0) store3(a.snap_e, 3u * ((unsigned)sp * plane + (unsigned)y * nz + (unsigned)z), v);
+                        const int sc = snapshot_cell(a, x, y, z);
+                        if (sc >= 0) store3(a.snap_e, 3u * (unsigned)sc, v);
                     }
                 }
             }
@@ -1074,8 +1080,8 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                     h1[k] = sel3(lanes, v);
                     if (x - 1 >= q.snap_lo && x - 1 < q.snap_hi && x - 1 >= xa && x - 1 < xb && r >= 1 && r < NR - 2 &&
                         lane >= 1 && lane < 62 && zin && y < ylim) {
-                        const int sp = snapshot_plane(a, x - 1);
-                        if (sp >= 0) store3(a.snap_h, 3u * ((unsigned)sp * plane + (unsigned)y * nz + (unsigned)z), v);
+                        const int sc = snapshot_cell(a, x - 1, y, z);
+                        if (sc >= 0) store3(a.snap_h, 3u * (unsigned)sc, v);
                     }
                 }
                 put3(sH1 + 3 * (r * 64 + lane), h1[k]);
@@ -1475,22 +1481,35 @@ int layout_march2(const tfdtd_march* d, March<float>* a) {
     a->hf.psi_out = static_cast<float*>(h_slabs ? d->h.psi_out : const_cast<void*>(d->h.psi_in));
     if (d->snap_count != 0) {
         if (d->snap_count < 0 || d->snap_count > 4 || !d->snap_e || !d->snap_h)
-            return tfdtd_fail(TFDTD_EINVAL, "march2: the step n+1 snapshot takes 1 to 4 ranges and an E and an H buffer");
-        int base = 0;
+            return tfdtd_fail(TFDTD_EINVAL, "march2: the step n+1 snapshot takes 1 to 4 boxes and an E and an H buffer");
+        const int n[3] = {d->nx, d->ny, d->nz};
+        long long base = 0;
+        a->snap_lo = d->nx;
+        a->snap_hi = 0;
         for (int k = 0; k < d->snap_count; ++k) {
-            const int lo = d->snap_lo[k], hi = d->snap_hi[k];
-            if (lo < 0 || hi > d->nx || hi <= lo || (k > 0 && lo < d->snap_hi[k - 1]))
-                return tfdtd_fail(TFDTD_EINVAL, "march2: snapshot range %d is empty, outside [0, nx) or not after the one before", k);
-            a->seg_lo[k] = lo;
-            a->seg_hi[k] = hi;
-            a->seg_base[k] = base;
-            base += hi - lo;
+            for (int ax = 0; ax < 3; ++ax) {
+                const int lo = d->snap_lo[k][ax], hi = d->snap_hi[k][ax];
+                if (lo < 0 || hi > n[ax] || hi <= lo)
+                    return tfdtd_fail(TFDTD_EINVAL, "march2: snapshot box %d is empty or outside the grid along axis %d", k, ax);
+                a->sbox_lo[k][ax] = lo;
+                a->sbox_hi[k][ax] = hi;
+            }
+            for (int j = 0; j < k; ++j) {
+                bool overlap = true;
+                for (int ax = 0; ax < 3; ++ax)
+                    overlap = overlap && d->snap_lo[k][ax] < d->snap_hi[j][ax] && d->snap_lo[j][ax] < d->snap_hi[k][ax];
+                if (overlap) return tfdtd_fail(TFDTD_EINVAL, "march2: snapshot boxes %d and %d overlap", j, k);
+            }
+            a->sbox_base[k] = (unsigned)base;
+            base += (long long)(d->snap_hi[k][0] - d->snap_lo[k][0]) * (d->snap_hi[k][1] - d->snap_lo[k][1]) *
+                    (d->snap_hi[k][2] - d->snap_lo[k][2]);
+            a->snap_lo = std::min(a->snap_lo, d->snap_lo[k][0]);
+            a->snap_hi = std::max(a->snap_hi, d->snap_hi[k][0]);
         }
+        if (3 * base > 0x7fffffffLL) return tfdtd_fail(TFDTD_EINVAL, "march2: the snapshot exceeds 31-bit offsets");
         a->snap_e = static_cast<float*>(d->snap_e);
         a->snap_h = static_cast<float*>(d->snap_h);
         a->snap_count = d->snap_count;
-        a->snap_lo = d->snap_lo[0];
-        a->snap_hi = d->snap_hi[d->snap_count - 1];
     }
     const int chunk = d->chunk_x > 0 ? d->chunk_x : d->nx;
     a->chunk = chunk;

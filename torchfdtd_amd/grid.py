@@ -237,8 +237,8 @@ def march_descriptor(grid, e_in, h_in, e_out, h_out, psi_e_in, psi_e_out, source
     """tfdtd_march: one whole step reading (e_in, h_in, psi_e_in) and writing (e_out, h_out, psi_e_out);
     `materials` (a MaterialIndex) lets the step read 1/eps through the one-byte index.  The H-family
     CPML memory updates in place unless `psi_h` = (input, output) is given (the two-step march).
-    `snapshot` = (E buffer, H buffer, [(lo, hi), ...]): the two-step pass also writes step n+1 on those
-    planes, the ranges one after another in the buffers."""
+    `snapshot` = (E buffer, H buffer, [(lo, hi), ...]) with lo, hi cell triples: the two-step pass also
+    writes step n+1 in those boxes, one after another in the buffers."""
     d = N.March()
     d.dtype = N.F64 if grid.dtype == torch.float64 else N.F32
     d.nx, d.ny, d.nz = grid.Nx, grid.Ny, grid.Nz
@@ -278,7 +278,8 @@ def march_descriptor(grid, e_in, h_in, e_out, h_out, psi_e_in, psi_e_out, source
         snap_e, snap_h, ranges = snapshot
         d.snap_e, d.snap_h, d.snap_count = _ptr(snap_e), _ptr(snap_h), len(ranges)
         for k, (lo, hi) in enumerate(ranges):
-            d.snap_lo[k], d.snap_hi[k] = lo, hi
+            for a in range(3):
+                d.snap_lo[k][a], d.snap_hi[k][a] = lo[a], hi[a]
     return d
 
 

@@ -196,8 +196,9 @@ class FrequencyPlane:
         grid.memory_states.append(self.flushed)
         self.counter = None
 
-    def descriptor(self, counter, E=None, H=None):
-        """tfdtd_plane sampling `E`/`H` (the grid's fields by default)."""
+    def descriptor(self, counter, E=None, H=None, indices=None):
+        """tfdtd_plane sampling `E`/`H` (the grid's fields by default) at `indices` (one index tensor per
+        component, the maps' own by default)."""
         d = N.Plane()
         g = self.grid
         d.dtype = N.F64 if g.dtype == torch.float64 else N.F32
@@ -209,8 +210,8 @@ class FrequencyPlane:
         d.corners = self.maps[0][0].shape[0]
         d.time_downsample = self.monitor.time_downsample
         d.E, d.H = (g.E if E is None else E).data_ptr(), (g.H if H is None else H).data_ptr()
-        for k, (indices, weights) in enumerate(self.maps):
-            d.indices[k] = indices.data_ptr()
+        for k, (own, weights) in enumerate(self.maps):
+            d.indices[k] = (own if indices is None else indices[k]).data_ptr()
             d.weights[k] = weights.data_ptr()
         d.window_e, d.window_h = self.windows[0].data_ptr(), self.windows[1].data_ptr()
         d.omega, d.half, d.value = self.omega.data_ptr(), self.half_phase.data_ptr(), self.value.data_ptr()

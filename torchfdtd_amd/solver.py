@@ -41,38 +41,47 @@ ENGINE_LOCK = threading.Lock()
 # tuning.tune_launch measures every path on the grid itself.
 TWO_STEP_MIN_CELLS = 20_000_000
 ONE_PASS_MIN_CELLS = 48_000_000
-# The two-step pass serves plane monitors and traces by writing step n+1 on the planes they read to a
-# snapshot of at most SNAPSHOT_RANGES x ranges; grids whose observers need more than an eighth of the x
-# planes keep the one-step paths (the snapshot would cost more than the pass saves).
+# The two-step pass serves plane monitors and traces by writing step n+1 in the cells they read to a
+# snapshot of at most SNAPSHOT_BOXES boxes; grids whose observers need more than an eighth of the
+# cells keep the one-step paths (the snapshot would cost more than the pass saves).
 SNAPSHOT_MAX_FRACTION = 8
-SNAPSHOT_RANGES = 4
+SNAPSHOT_BOXES = 4
 
 
-def merge_ranges(spans, most=SNAPSHOT_RANGES):
-    """Ascending disjoint half-open ranges covering `spans`: overlapping or touching ranges joined, then
-    the two nearest neighbours joined until at most `most` remain."""
-    out = []
-    for lo, hi in sorted(spans):
-        if out and lo <= out[-1][1]:
-            out[-1] = (out[-1][0], max(out[-1][1], hi))
-        else:
-            out.append((lo, hi))
+def _cells(box):
+    lo, hi = box
+    return math.prod(b-a for a, b in zip(lo, hi))
+
+
+def _union(a, b):
+    return tuple(map(min, a[0], b[0])), tuple(map(max, a[1], b[1]))
+
+
+def merge_boxes(boxes, most=SNAPSHOT_BOXES):
+    """Disjoint boxes (lo, hi cell triples, half-open) covering `boxes`: boxes that overlap or touch are
+    replaced by their bounding box, then the pair whose bounding box adds the fewest cells, until at
+    most `most` remain; sorted by lo."""
+    out = [(tuple(lo), tuple(hi)) for lo, hi in boxes]
+    merged = True
+    while merged:
+        merged = False
+        for i in range(len(out)):
+            for j in range(i+1, len(out)):
+                a, b = out[i], out[j]
+                if all(a[0][k] <= b[1][k] and b[0][k] <= a[1][k] for k in range(3)):
+                    out[i] = _union(a, b)
+                    del out[j]
+                    merged = True
+                    break
+            if merged:
+                break
     while len(out) > most:
-        k = min(range(len(out)-1), key=lambda i: out[i+1][0]-out[i][1])
-        out[k:k+2] = [(out[k][0], out[k+1][1])]
-    return out
-
-
-class _Shifted:
-    """A buffer viewed at an element offset: data_ptr() of plane `lo` of a snapshot that starts at
-    plane lo is that of plane 0 of a full field, so the observers' flat indices need no change."""
-
-    def __init__(self, tensor, elements):
-        self.tensor, self.elements = tensor, elements
-        self.device = tensor.device
-
-    def data_ptr(self):
-        return self.tensor.data_ptr() - self.elements*self.tensor.element_size()
+        pairs = [(i, j) for i in range(len(out)) for j in range(i+1, len(out))]
+        i, j = min(pairs, key=lambda ij: _cells(_union(out[ij[0]], out[ij[1]])) - _cells(out[ij[0]]) - _cells(out[ij[1]]))
+        out[i] = _union(out[i], out[j])
+        del out[j]
+        out = merge_boxes(out, most=len(out))
+    return sorted(out)
 
 
 def default_step_path(cells, two_step_eligible, one_pass_supported):
@@ -298,7 +307,7 @@ class Stepper:
         self.two_step = self._two_step_allowed() if two_step is None else bool(two_step)
         if self.two_step and not self._two_step_allowed():
             raise ValueError('the two-step kernel needs a float32 one-pass grid whose plane monitors and point traces '
-                             f'read at most 1/{SNAPSHOT_MAX_FRACTION} of the x planes in at most {SNAPSHOT_RANGES} ranges')
+                             f'read at most 1/{SNAPSHOT_MAX_FRACTION} of the cells in at most {SNAPSHOT_BOXES} boxes')
         # One-byte material index of 1/eps for the one-pass step (grids with few distinct values).
         self.materials = MaterialIndex(g) if self.one_pass else None
         self._describe()
@@ -311,33 +320,55 @@ class Stepper:
         elif launch is not None:
             raise ValueError("launch is None (default shapes) or 'auto' (measured)")
 
-    def _observer_spans(self):
-        """x planes [lo, hi) each trace and each plane monitor reads (traces first, in order)."""
-        per_plane = 3*self.grid.Ny*self.grid.Nz
-        spans = [(flat // per_plane, flat // per_plane + 1) for _, flat in self.trace_points]
+    def _observer_boxes(self):
+        """The cell box each trace and each plane monitor reads (traces first, in order)."""
+        g = self.grid
+        shape = (g.Nx, g.Ny, g.Nz)
+        boxes = []
+        for _, flat in self.trace_points:
+            cell = np.unravel_index(flat // 3, shape)
+            boxes.append((tuple(int(c) for c in cell), tuple(int(c)+1 for c in cell)))
         for plane in self.planes:
-            spans.append((min(int(indices.min()) for indices, _ in plane.maps) // per_plane,
-                          max(int(indices.max()) for indices, _ in plane.maps) // per_plane + 1))
-        return spans
+            cells = torch.cat([(indices // 3).reshape(-1) for indices, _ in plane.maps])
+            coords = [cells // (g.Ny*g.Nz), (cells // g.Nz) % g.Ny, cells % g.Nz]
+            boxes.append((tuple(int(c.min()) for c in coords), tuple(int(c.max())+1 for c in coords)))
+        return boxes
 
     def observed_planes(self):
-        """The x ranges a step n+1 snapshot covers: every plane the observers read, merged into at most
-        SNAPSHOT_RANGES ranges ([] without observers)."""
-        return merge_ranges(self._observer_spans())
+        """The boxes of cells a step n+1 snapshot covers: every cell the observers read, merged into at
+        most SNAPSHOT_BOXES disjoint boxes ([] without observers)."""
+        return merge_boxes(self._observer_boxes())
 
     def _two_step_eligible(self):
         if self.grid.dtype != torch.float32:
             return False
-        planes = sum(hi-lo for lo, hi in self.observed_planes())
-        return planes <= max(2, self.grid.Nx // SNAPSHOT_MAX_FRACTION)
+        g = self.grid
+        cells = sum(_cells(b) for b in self.observed_planes())
+        return cells <= max(2*g.Ny*g.Nz, g.Nx*g.Ny*g.Nz // SNAPSHOT_MAX_FRACTION)
+
+    def _snapshot_index(self, indices, boxes):
+        """Element indices of the fields -> the same elements in the snapshot (box after box)."""
+        g = self.grid
+        cells, comp = indices // 3, indices % 3
+        x, y, z = cells // (g.Ny*g.Nz), (cells // g.Nz) % g.Ny, cells % g.Nz
+        out = torch.full_like(indices, -1)
+        base = 0
+        for lo, hi in boxes:
+            inside = ((x >= lo[0]) & (x < hi[0]) & (y >= lo[1]) & (y < hi[1]) & (z >= lo[2]) & (z < hi[2]))
+            local = ((x-lo[0])*(hi[1]-lo[1]) + (y-lo[1]))*(hi[2]-lo[2]) + (z-lo[2])
+            out = torch.where(inside, (base+local)*3 + comp, out)
+            base += _cells((lo, hi))
+        if bool((out < 0).any()):
+            raise RuntimeError('an observer reads a cell outside the step n+1 snapshot')
+        return out.contiguous()
 
     def _two_step_allowed(self):
         return self.one_pass and self._two_step_eligible()
 
     def paths(self):
         """The step paths this grid can take: 'split' always; 'one_pass' on one-pass grids; 'two_step'
-        on float32 one-pass grids whose observers read at most 1/SNAPSHOT_MAX_FRACTION of the x planes
-        (in at most SNAPSHOT_RANGES merged ranges)."""
+        on float32 one-pass grids whose observers read at most 1/SNAPSHOT_MAX_FRACTION of the cells (in
+        at most SNAPSHOT_BOXES merged boxes)."""
         out = ['split']
         if self.one_pass_supported:
             out.append('one_pass')
@@ -429,34 +460,26 @@ class Stepper:
                 if getattr(self, 'psi_h_b', None) is None:
                     self.psi_h_b = torch.zeros_like(g.psi_buffers[True])
                 psi_h = g.psi_buffers[True]
-                # Observers sample step n+1 from a snapshot of the planes they read, which the pass
-                # writes, its ranges one after another; each plane and trace table entry points into
-                # it as into a full field.
-                self.snapshot_ranges = ranges = self.observed_planes()
+                # Observers sample step n+1 from a snapshot of the cells they read, which the pass
+                # writes box after box; the snapshot's plane and trace tables index into it.
+                self.snapshot_ranges = boxes = self.observed_planes()
                 snapshot = None
-                if ranges:
-                    shape = (sum(hi-lo for lo, hi in ranges), g.Ny, g.Nz, 3)
-                    if getattr(self, 'snap_e', None) is None or tuple(self.snap_e.shape) != shape:
-                        self.snap_e = torch.zeros(shape, device=g.E.device, dtype=g.dtype)
-                        self.snap_h = torch.zeros(shape, device=g.E.device, dtype=g.dtype)
-                    snapshot = (self.snap_e, self.snap_h, ranges)
-                    per_plane, starts, base = g.Ny*g.Nz*3, [], 0
-                    for lo, hi in ranges:
-                        starts.append((lo, hi, lo-base))   # snapshot plane of grid plane x: x - (lo - base)
-                        base += hi-lo
-
-                    def shift(span):
-                        (offset,) = [o for lo, hi, o in starts if lo <= span[0] and span[1] <= hi]
-                        return offset*per_plane
-
-                    spans = self._observer_spans()
-                    trace_spans, plane_spans = spans[:len(self.trace_points)], spans[len(self.trace_points):]
+                if boxes:
+                    total = sum(_cells(b) for b in boxes)
+                    if getattr(self, 'snap_e', None) is None or self.snap_e.shape[0] != total:
+                        self.snap_e = torch.zeros((total, 3), device=g.E.device, dtype=g.dtype)
+                        self.snap_h = torch.zeros((total, 3), device=g.E.device, dtype=g.dtype)
+                    snapshot = (self.snap_e, self.snap_h, boxes)
                     self.plane_table_s = (N.Plane*max(1, len(self.planes)))()
-                    for k, (plane, span) in enumerate(zip(self.planes, plane_spans)):
-                        self.plane_table_s[k] = plane.descriptor(self.counter, _Shifted(self.snap_e, shift(span)),
-                                                                 _Shifted(self.snap_h, shift(span)))
-                    rows = [(_Shifted(self.snap_h if magnetic else self.snap_e, shift(span)).data_ptr(), flat)
-                            for (magnetic, flat), span in zip(self.trace_points, trace_spans)]
+                    self.snap_indices = [[self._snapshot_index(indices, boxes) for indices, _ in plane.maps]
+                                         for plane in self.planes]
+                    for k, plane in enumerate(self.planes):
+                        self.plane_table_s[k] = plane.descriptor(self.counter, self.snap_e, self.snap_h,
+                                                                 indices=self.snap_indices[k])
+                    flats = self._snapshot_index(torch.as_tensor([flat for _, flat in self.trace_points], dtype=torch.long,
+                                                                 device=g.E.device), boxes).tolist()
+                    rows = [((self.snap_h if magnetic else self.snap_e).data_ptr(), f)
+                            for (magnetic, _), f in zip(self.trace_points, flats)]
                     self.trace_table_s = torch.as_tensor(np.array(rows, dtype=np.int64).reshape(-1, 2), device=g.E.device)
                     self.observe_s = self._observe(self.trace_table_s)
                 self.march2 = [march_descriptor(g, g.E, g.H, self.E_b, self.H_b, psi_a, self.psi_e_b, self.sources,
