@@ -227,7 +227,7 @@ def observed_cube_line(steps=96, width=48):
         torch.cuda.synchronize()
         ms = (time.perf_counter()-t0)/steps*1e3
         out[name] = dict(path=run.path, ms_per_step=round(ms, 4), value=round(cells/ms/1e6, 3), unit='Gcell-updates/s',
-                         snapshot_cells=int(run.snap_e.shape[0]) if run.two_step and run.snapshot_ranges else None)
+                         snapshot_cells=int(run.snap_e.shape[0]) if run.two_step and run.snapshot_boxes else None)
         del run, graphs
         torch.cuda.empty_cache()
     out['speedup'] = round(out['one_step']['ms_per_step']/out['default']['ms_per_step'], 3)

@@ -73,7 +73,7 @@ def test_two_step_observers_equal_the_one_step_paths(gpu, layout):
         run.advance(80)
         results[path] = state(run)
         if path == 'two_step':
-            boxes = run.snapshot_ranges
+            boxes = run.snapshot_boxes
             total = sum(math.prod(b - a for a, b in zip(lo, hi)) for lo, hi in boxes)
             assert len(boxes) == (1 if layout == 'one box' else 2) and 0 < total <= run.grid.E.numel() // 3 // 8, boxes
             assert run.snap_e.shape[0] == total
@@ -93,7 +93,7 @@ def test_observers_spread_over_many_cells_keep_the_one_step_paths(gpu):
     from torchfdtd_amd.solver import Stepper
     project = observed_project(scattered=True)
     run = Stepper(project, one_pass=True)
-    boxes = run.observed_planes()
+    boxes = run.observed_boxes()
     assert sum(math.prod(b - a for a, b in zip(lo, hi)) for lo, hi in boxes) > run.grid.E.numel() // 3 // 8
     assert run.paths() == ['split', 'one_pass'] and not run.two_step
     with pytest.raises(ValueError, match='two-step'):

@@ -334,7 +334,7 @@ class Stepper:
             boxes.append((tuple(int(c.min()) for c in coords), tuple(int(c.max())+1 for c in coords)))
         return boxes
 
-    def observed_planes(self):
+    def observed_boxes(self):
         """The boxes of cells a step n+1 snapshot covers: every cell the observers read, merged into at
         most SNAPSHOT_BOXES disjoint boxes ([] without observers)."""
         return merge_boxes(self._observer_boxes())
@@ -343,7 +343,7 @@ class Stepper:
         if self.grid.dtype != torch.float32:
             return False
         g = self.grid
-        cells = sum(_cells(b) for b in self.observed_planes())
+        cells = sum(_cells(b) for b in self.observed_boxes())
         return cells <= max(2*g.Ny*g.Nz, g.Nx*g.Ny*g.Nz // SNAPSHOT_MAX_FRACTION)
 
     def _snapshot_index(self, indices, boxes):
@@ -462,7 +462,7 @@ class Stepper:
                 psi_h = g.psi_buffers[True]
                 # Observers sample step n+1 from a snapshot of the cells they read, which the pass
                 # writes box after box; the snapshot's plane and trace tables index into it.
-                self.snapshot_ranges = boxes = self.observed_planes()
+                self.snapshot_boxes = boxes = self.observed_boxes()
                 snapshot = None
                 if boxes:
                     total = sum(_cells(b) for b in boxes)
@@ -572,7 +572,7 @@ class Stepper:
         advance the counter."""
         stream = N.stream_handle(self.grid.device)
         lib = self.lib
-        observed = bool(self.snapshot_ranges)
+        observed = bool(self.snapshot_boxes)
         for k, planes, observe in ((0, self.plane_table_b, self.observe_b), (1, self.plane_table, self.observe)):
             self.march2_step(k, stream)
             for table, obs in (((self.plane_table_s, self.observe_s) if observed else (planes, observe)), (planes, observe)):
