@@ -206,10 +206,12 @@ int tfdtd_yee_step_march2(const tfdtd_march* step, void* stream);
  * (tfdtd_batch_march_bytes(count, TFDTD_F32) bytes) and block_prefix[3][count+1]; the caller copies
  * both to device memory once and launches tfdtd_batch_march2 for every pass with totals[v] =
  * block_prefix[v][count].  The tile parameters are then read from device memory instead of being
- * held in kernel-argument registers. */
+ * held in kernel-argument registers.  `snapshot` is nonzero exactly when a packed descriptor has a
+ * step n+1 snapshot (snap_count > 0; tile_y 12 or 16): the kernels that store it are compiled apart,
+ * and the others write none. */
 int tfdtd_batch_march2_pack(const tfdtd_march* steps, int32_t count, void* host_table, size_t bytes, int32_t* block_prefix);
 int tfdtd_batch_march2(const void* device_table, const int32_t* device_prefix, int32_t count, int32_t tile_y,
-                       int32_t eps_stride, const int32_t* totals, void* stream);
+                       int32_t eps_stride, int32_t snapshot, const int32_t* totals, void* stream);
 
 /* torchfdtd/cuda_kernels.py:349-365 (FusedYeeCUDA.update_E / update_H), with the
  * family's sources of torchfdtd/solver.py:778-783 folded in when nsrc > 0. */

@@ -164,7 +164,7 @@ PROTOTYPES = {
     'tfdtd_yee_step_march': (i32, [ctypes.POINTER(March), vp]),
     'tfdtd_yee_step_march2': (i32, [ctypes.POINTER(March), vp]),
     'tfdtd_batch_march2_pack': (i32, [ctypes.POINTER(March), i32, vp, ctypes.c_size_t, ctypes.POINTER(i32)]),
-    'tfdtd_batch_march2': (i32, [vp, vp, i32, i32, i32, ctypes.POINTER(i32), vp]),
+    'tfdtd_batch_march2': (i32, [vp, vp, i32, i32, i32, i32, ctypes.POINTER(i32), vp]),
     'tfdtd_batch_table_bytes': (ctypes.c_size_t, [i32, i32]),
     'tfdtd_batch_pack': (i32, [ctypes.POINTER(HalfStep), i32, vp, ctypes.c_size_t, ctypes.POINTER(i32)]),
     'tfdtd_batch_half_step': (i32, [vp, vp, i32, i32, i32, i32, vp]),

@@ -796,7 +796,9 @@ constexpr int kTY2 = 9;    // owned y rows of a two-step tile
 // plane ahead; E1(x-1), H1(x-1) and E2(x-2) are exchanged between wavefronts through LDS, and every
 // value a cell needs from its own previous plane (H(n), E1, H1, E2, 1/eps, CPML memory) is carried in
 // registers.  Cells outside the grid are never computed or stored; their addresses are clamped.
-template <typename real, bool EPS3, int CPML, int W, int NR = 12>
+// SNAP: the pass also stores step n+1 in the descriptor's snapshot boxes (compiled apart: the
+// store code, even unused, slows the other kernels' loops).
+template <typename real, bool EPS3, int CPML, int W, int NR = 12, bool SNAP = false>
 __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, const int G) {
     constexpr int RW = NR / W;   // NR widened rows (owned NR - 3), RW per wavefront
     static_assert(RW * W == NR, "the wavefronts split the widened rows evenly");
@@ -1053,7 +1055,7 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                     add_sources(a.se, q.nse, q.ebl, q.ebh, n, q.samples, x, y, z, v.x, v.y, v.z);
                     e1[k] = sel3(lanes, v);
                     // Snapshot of step n+1: the cells this tile owns (those whose E2 it stores).
-                    if (x >= q.snap_lo && x < q.snap_hi && x >= xa && x < xb && r >= 1 && r < NR - 2 && lane >= 1 &&
+                    if (SNAP && x >= q.snap_lo && x < q.snap_hi && x >= xa && x < xb && r >= 1 && r < NR - 2 && lane >= 1 &&
                         lane < 62 && zin && y < ylim) {
                         const int sc = snapshot_cell(a, x, y, z);
                         if (sc >= 0) store3(a.snap_e, 3u * (unsigned)sc, v);
@@ -1078,7 +1080,7 @@ __device__ __forceinline__ void tile2_body(const March<real>& a, const int w, co
                                                               ph1[k]);
                     add_sources(a.sh, q.nsh, q.hbl, q.hbh, n, q.samples, x - 1, y, z, v.x, v.y, v.z);
                     h1[k] = sel3(lanes, v);
-                    if (x - 1 >= q.snap_lo && x - 1 < q.snap_hi && x - 1 >= xa && x - 1 < xb && r >= 1 && r < NR - 2 &&
+                    if (SNAP && x - 1 >= q.snap_lo && x - 1 < q.snap_hi && x - 1 >= xa && x - 1 < xb && r >= 1 && r < NR - 2 &&
                         lane >= 1 && lane < 62 && zin && y < ylim) {
                         const int sc = snapshot_cell(a, x - 1, y, z);
                         if (sc >= 0) store3(a.snap_h, 3u * (unsigned)sc, v);
@@ -1410,14 +1412,14 @@ int launch_tile(const tfdtd_march* d, hipStream_t stream) {
 #define TFDTD_TILE2_WAVES(CPML, W) \
     ((CPML) == 6 || (CPML) == 7 ? TFDTD_TILE2_BOUNDARY_WAVES : (W) == 16 ? 8 : (W) == 12 ? 6 : 1)
 
-template <bool EPS3, int CPML, int W>
+template <bool EPS3, int CPML, int W, bool SNAP = false>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TFDTD_TILE2_WAVES(CPML, W), 8))) void tile2_kernel(const March<float> a) {
-    tile2_body<float, EPS3, CPML, W, W == 16 ? 16 : 12>(a, blockIdx.x, gridDim.x);
+    tile2_body<float, EPS3, CPML, W, W == 16 ? 16 : 12, SNAP>(a, blockIdx.x, gridDim.x);
 }
 
 // Many grids (or one) from a device table, as batch_tile_kernel: the tile parameters are loaded on
 // demand instead of occupying the kernel-argument registers.
-template <bool EPS3, int CPML, int W>
+template <bool EPS3, int CPML, int W, bool SNAP = false>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TFDTD_TILE2_WAVES(CPML, W), 8))) void batch_tile2_kernel(
     const March<float>* __restrict__ table, const int* __restrict__ prefix, int grids) {
     const int b = blockIdx.x;
@@ -1440,7 +1442,7 @@ __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TFDTD_TI
         const bool x_only = ty >= a.box_lo[1] && ty < a.box_hi[1] && tz >= a.zbox_lo && tz < a.zbox_hi;
         if ((CPML == 1) != x_only) return;
     }
-    tile2_body<float, EPS3, CPML, W, W == 16 ? 16 : 12>(a, w, count);
+    tile2_body<float, EPS3, CPML, W, W == 16 ? 16 : 12, SNAP>(a, w, count);
 }
 
 // Tiles [lo, hi) along one axis whose cells of both stages avoid every CPML row of that axis: a tile
@@ -1574,16 +1576,16 @@ int layout_march2(const tfdtd_march* d, March<float>* a) {
 }
 
 // W wavefronts per interior tile; the z-face and boundary tiles (more registers per row) use 12.
-template <int W>
+template <int W, bool SNAP = false>
 int launch_tile2(const March<float>& a, bool eps3, hipStream_t stream) {
     return launch_variants(a.blocks, stream, [&](int v, hipStream_t s) {
         auto go = [&](auto kernel, int waves, const char* what) {
             hipLaunchKernelGGL(kernel, dim3((unsigned)a.blocks[v]), dim3(64 * waves), 0, s, a);
             return tfdtd_check_launch(what);
         };
-        if (v == 0) return eps3 ? go(tile2_kernel<true, 0, W>, W, "march2 interior") : go(tile2_kernel<false, 0, W>, W, "march2 interior");
-        if (v == 1) return eps3 ? go(tile2_kernel<true, 4, 12>, 12, "march2 z faces") : go(tile2_kernel<false, 4, 12>, 12, "march2 z faces");
-        return eps3 ? go(tile2_kernel<true, 7, 12>, 12, "march2 boundary") : go(tile2_kernel<false, 7, 12>, 12, "march2 boundary");
+        if (v == 0) return eps3 ? go(tile2_kernel<true, 0, W, SNAP>, W, "march2 interior") : go(tile2_kernel<false, 0, W, SNAP>, W, "march2 interior");
+        if (v == 1) return eps3 ? go(tile2_kernel<true, 4, 12, SNAP>, 12, "march2 z faces") : go(tile2_kernel<false, 4, 12, SNAP>, 12, "march2 z faces");
+        return eps3 ? go(tile2_kernel<true, 7, 12, SNAP>, 12, "march2 boundary") : go(tile2_kernel<false, 7, 12, SNAP>, 12, "march2 boundary");
     });
 }
 
@@ -1607,7 +1609,7 @@ int pack_marches2(const tfdtd_march* d, int count, void* host_table, size_t byte
     return TFDTD_OK;
 }
 
-template <int W>
+template <int W, bool SNAP = false>
 int launch_batch_tiles2(const void* table, const int32_t* prefix, int count, bool eps3, const int32_t* totals, hipStream_t stream) {
     const March<float>* t = static_cast<const March<float>*>(table);
     const int blocks[3] = {totals[0], totals[1], totals[2]};
@@ -1616,21 +1618,21 @@ int launch_batch_tiles2(const void* table, const int32_t* prefix, int count, boo
             hipLaunchKernelGGL(kernel, dim3((unsigned)blocks[v]), dim3(64 * waves), 0, s, t, prefix + v * (count + 1), count);
             return tfdtd_check_launch(what);
         };
-        if (v == 0) return eps3 ? go(batch_tile2_kernel<true, 0, W>, W, "batch march2 interior")
-                                : go(batch_tile2_kernel<false, 0, W>, W, "batch march2 interior");
-        if (v == 1) return eps3 ? go(batch_tile2_kernel<true, 4, 12>, 12, "batch march2 z faces")
-                                : go(batch_tile2_kernel<false, 4, 12>, 12, "batch march2 z faces");
+        if (v == 0) return eps3 ? go(batch_tile2_kernel<true, 0, W, SNAP>, W, "batch march2 interior")
+                                : go(batch_tile2_kernel<false, 0, W, SNAP>, W, "batch march2 interior");
+        if (v == 1) return eps3 ? go(batch_tile2_kernel<true, 4, 12, SNAP>, 12, "batch march2 z faces")
+                                : go(batch_tile2_kernel<false, 4, 12, SNAP>, 12, "batch march2 z faces");
         // Boundary tiles, three launches over the same workgroup range (each skips the others' tiles), one
         // after the other: the x end chunks' tiles that meet y or z slabs (every slab), the other x end
         // tiles (x slabs only) and the x-middle chunks' tiles (y and z slabs only); the fewer slabs a
         // launch compiles in, the fewer registers it spills.
-        int status = eps3 ? go(batch_tile2_kernel<true, 7, 12>, 12, "batch march2 boundary")
-                          : go(batch_tile2_kernel<false, 7, 12>, 12, "batch march2 boundary");
-        if (!status) status = eps3 ? go(batch_tile2_kernel<true, 1, 12>, 12, "batch march2 x boundary")
-                                   : go(batch_tile2_kernel<false, 1, 12>, 12, "batch march2 x boundary");
+        int status = eps3 ? go(batch_tile2_kernel<true, 7, 12, SNAP>, 12, "batch march2 boundary")
+                          : go(batch_tile2_kernel<false, 7, 12, SNAP>, 12, "batch march2 boundary");
+        if (!status) status = eps3 ? go(batch_tile2_kernel<true, 1, 12, SNAP>, 12, "batch march2 x boundary")
+                                   : go(batch_tile2_kernel<false, 1, 12, SNAP>, 12, "batch march2 x boundary");
         if (status) return status;
-        return eps3 ? go(batch_tile2_kernel<true, 6, 12>, 12, "batch march2 y boundary")
-                    : go(batch_tile2_kernel<false, 6, 12>, 12, "batch march2 y boundary");
+        return eps3 ? go(batch_tile2_kernel<true, 6, 12, SNAP>, 12, "batch march2 y boundary")
+                    : go(batch_tile2_kernel<false, 6, 12, SNAP>, 12, "batch march2 y boundary");
     });
 }
 
@@ -1639,6 +1641,13 @@ int dispatch_tile2(const tfdtd_march* d, hipStream_t stream) {
     int status = layout_march2(d, &a);
     if (status) return status;
     const bool eps3 = d->eps_stride == 3;
+    if (a.snap_count > 0) {
+        switch (d->tile_y) {
+            case 0: case 12: return launch_tile2<12, true>(a, eps3, stream);
+            case 16: return launch_tile2<16, true>(a, eps3, stream);
+            default: return tfdtd_fail(TFDTD_EINVAL, "march2: a step n+1 snapshot takes 12 or 16 wavefronts per interior tile");
+        }
+    }
     switch (d->tile_y) {
         case 4: return launch_tile2<4>(a, eps3, stream);
         case 6: return launch_tile2<6>(a, eps3, stream);
@@ -1724,11 +1733,18 @@ extern "C" int tfdtd_batch_march2_pack(const tfdtd_march* steps, int32_t count, 
 }
 
 extern "C" int tfdtd_batch_march2(const void* device_table, const int32_t* device_prefix, int32_t count, int32_t tile_y,
-                                  int32_t eps_stride, const int32_t* totals, void* stream) {
+                                  int32_t eps_stride, int32_t snapshot, const int32_t* totals, void* stream) {
     if (!device_table || !device_prefix || count < 1 || !totals) return tfdtd_fail(TFDTD_EINVAL, "batch march2: bad launch arguments");
     if (eps_stride != 1 && eps_stride != 3) return tfdtd_fail(TFDTD_EINVAL, "march: eps_stride must be 1 or 3");
     hipStream_t s = static_cast<hipStream_t>(stream);
     const bool eps3 = eps_stride == 3;
+    if (snapshot) {
+        switch (tile_y) {
+            case 0: case 12: return launch_batch_tiles2<12, true>(device_table, device_prefix, count, eps3, totals, s);
+            case 16: return launch_batch_tiles2<16, true>(device_table, device_prefix, count, eps3, totals, s);
+            default: return tfdtd_fail(TFDTD_EINVAL, "march2: a step n+1 snapshot takes 12 or 16 wavefronts per interior tile");
+        }
+    }
     switch (tile_y) {
         case 4: return launch_batch_tiles2<4>(device_table, device_prefix, count, eps3, totals, s);
         case 6: return launch_batch_tiles2<6>(device_table, device_prefix, count, eps3, totals, s);

@@ -562,8 +562,8 @@ class Stepper:
         self.refresh_materials()
         table, prefix, totals = self.march2_tables[k]
         d = self.march2[k]
-        N.check(self.lib.tfdtd_batch_march2(table.data_ptr(), prefix.data_ptr(), 1, d.tile_y, d.eps_stride, totals, stream),
-                'tfdtd_batch_march2')
+        N.check(self.lib.tfdtd_batch_march2(table.data_ptr(), prefix.data_ptr(), 1, d.tile_y, d.eps_stride,
+                                            int(d.snap_count > 0), totals, stream), 'tfdtd_batch_march2')
 
     def quad(self):
         """Four steps with the two-step kernel: grid buffers -> second set -> grid buffers.  After each
