@@ -1409,8 +1409,14 @@ int launch_tile(const tfdtd_march* d, hipStream_t stream) {
 #ifndef TFDTD_TILE2_BOUNDARY_WAVES
 #define TFDTD_TILE2_BOUNDARY_WAVES 6
 #endif
+// Waves per SIMD the 12-wavefront interior, z-face and x-only boundary variants ask for: at 7 the z-face
+// variant compiles without scratch spills (81 VGPRs) and the 512^3 pass takes 1.344 instead of 1.381 ms
+// per step (8: the same as 7).
+#ifndef TFDTD_TILE2_INTERIOR_WAVES
+#define TFDTD_TILE2_INTERIOR_WAVES 7
+#endif
 #define TFDTD_TILE2_WAVES(CPML, W) \
-    ((CPML) == 6 || (CPML) == 7 ? TFDTD_TILE2_BOUNDARY_WAVES : (W) == 16 ? 8 : (W) == 12 ? 6 : 1)
+    ((CPML) == 6 || (CPML) == 7 ? TFDTD_TILE2_BOUNDARY_WAVES : (W) == 16 ? 8 : (W) == 12 ? TFDTD_TILE2_INTERIOR_WAVES : 1)
 
 template <bool EPS3, int CPML, int W, bool SNAP = false>
 __global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(TFDTD_TILE2_WAVES(CPML, W), 8))) void tile2_kernel(const March<float> a) {
