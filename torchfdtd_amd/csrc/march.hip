@@ -1166,9 +1166,13 @@ constexpr bool kConcurrentVariants = false;
 // against 0.139 s per 100 steps on the 64-grid uneven batch.  Loading only the z-slab CPML memory a
 // plane ahead at 6 waves (E after the plane's E, H after its H) spills more and was slower too: 2.29
 // ms and 0.158 s.
+// FP32 z-face and boundary tiles at 7 waves per SIMD: 2.36 against 2.24 ms per step at 512^3.
+#ifndef TFDTD_TILE_EDGE_WAVES
+#define TFDTD_TILE_EDGE_WAVES 6
+#endif
 #define TFDTD_TILE_BOUNDS                                                                          \
     __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(                                   \
-        CPML == 0 ? 8 : sizeof(real) == 4 ? 6 : 3,                                                   \
+        CPML == 0 ? 8 : sizeof(real) == 4 ? TFDTD_TILE_EDGE_WAVES : 3,                               \
         8)))
 
 template <typename real, int W, bool EPS3, int CPML>
