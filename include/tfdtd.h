@@ -179,9 +179,12 @@ typedef struct tfdtd_march {
  * with events and joins them back, so the work is ordered on `stream` as one launch would be
  * and can be captured into a graph.  The side streams (default priority, one pair per device)
  * are created by the first call on a device made outside stream capture; calls captured
- * before that run the three launches one after another on `stream`.  Calls on one device
- * from several host threads at once must be serialised by the caller.  The same holds for
- * tfdtd_batch_march. */
+ * before that run the three launches one after another on `stream`.  Eager calls on one
+ * device from several host threads at once are safe: the library holds a per-device lock while
+ * it enqueues the fork, the launches and the join.  While a thread captures such a call into a
+ * graph, the side streams belong to that capture until it ends: other threads must not call
+ * these entry points on that device meanwhile.  The same holds for tfdtd_batch_march and the
+ * two-step entry points. */
 int tfdtd_yee_step_march(const tfdtd_march* step, void* stream);
 
 /* Two whole Yee steps, n -> n+2, in one pass over HBM (float32): the step closure of

@@ -1322,10 +1322,13 @@ int layout_march(const tfdtd_march* d, March<real>* a) {
 
 // Two side streams per device for the concurrent variant launches, with the events that fork them
 // from and join them back to the caller's stream.  Created on first use outside stream capture and
-// kept for the life of the process; every stream has the default priority.
+// kept for the life of the process; every stream has the default priority.  `enqueue` is held while
+// a call records the fork, launches and records the join, so calls from several host threads keep
+// their fork and join events apart.
 struct Side {
     hipStream_t stream[2];
     hipEvent_t fork, join[2];
+    std::mutex enqueue;
 };
 
 Side* side_streams(hipStream_t caller) {
@@ -1374,6 +1377,7 @@ int launch_variants(const int blocks[3], hipStream_t stream, Launch&& launch) {
         for (int v = 0; v < 3; ++v) run(v, stream);
         return status;
     }
+    std::lock_guard<std::mutex> guard(side->enqueue);
     check(hipEventRecord(side->fork, stream), "fork");
     for (int k = 0; k < 2; ++k) check(hipStreamWaitEvent(side->stream[k], side->fork, 0), "fork wait");
     run(2, side->stream[0]);
