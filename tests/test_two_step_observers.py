@@ -14,13 +14,15 @@ import torch
 pytestmark = pytest.mark.gpu
 
 
-def observed_project(steps=80, trace_x=1.0, reflection=False, wide=False, scattered=False):
+def observed_project(steps=80, trace_x=1.0, reflection=False, wide=False, scattered=False, tall=False):
     """x-normal plane near high x and two traces; `reflection` adds an x-normal plane near low x (two
-    snapshot boxes), `wide` a y-normal plane along x beside it (two boxes), `scattered` a z-normal and a
-    y-normal plane across each other (one box of more cells than a snapshot takes)."""
+    snapshot boxes), `wide` a y-normal plane along x beside it (two boxes), `tall` makes the grid 60 x 150
+    cells in y and z, so that the one box also passes through interior tiles (at 40 x 30 every tile
+    holds CPML cells), `scattered` a z-normal and a y-normal plane across each other (one box of more cells than a
+    snapshot takes)."""
     from torchfdtd_amd import FieldMonitor, Monitor, Project, Region, Source, Structure
     from torchfdtd_amd.models import SpectrumSettings
-    region = Region(dimension='3d', size=(3.2, 1.6, 1.2), mesh=.04, pml_cells=8, steps=steps, backend='cuda')
+    region = Region(dimension='3d', size=(3.2, 2.4, 6.0) if tall else (3.2, 1.6, 1.2), mesh=.04, pml_cells=8, steps=steps, backend='cuda')
     spectrum = SpectrumSettings(sampling='wavelength', frequency_points=8, wavelength_start=1.45, wavelength_stop=1.65,
                                 apodization='none')
     monitors = [FieldMonitor(id='plane', center=(1.0, 0, 0), normal='x', size=(0, .4, .5), dft_precision='float64',
@@ -58,14 +60,15 @@ def assert_same(got, expected):
         assert torch.equal(a, b)
 
 
-@pytest.mark.parametrize('layout', ['one box', 'reflection', 'wide'])
+@pytest.mark.parametrize('layout', ['one box', 'reflection', 'wide', 'tall'])
 def test_two_step_observers_equal_the_one_step_paths(gpu, layout):
     """One snapshot box (a plane and two traces near high x), two (a reflection plane near low x as
-    well, or a y-normal plane along x beside the x-normal one)."""
+    well, or a y-normal plane along x beside the x-normal one); one box on a grid with interior
+    tiles."""
     reflection = layout == 'reflection'
     from torchfdtd_amd.graphs import StepGraphs
     from torchfdtd_amd.solver import Stepper
-    project = observed_project(reflection=reflection, wide=layout == 'wide')
+    project = observed_project(reflection=reflection, wide=layout == 'wide', tall=layout == 'tall')
     results = {}
     for path in ('split', 'one_pass', 'two_step'):
         run = Stepper(project, one_pass=path != 'split', two_step=path == 'two_step')
@@ -75,7 +78,10 @@ def test_two_step_observers_equal_the_one_step_paths(gpu, layout):
         if path == 'two_step':
             boxes = run.snapshot_boxes
             total = sum(math.prod(b - a for a, b in zip(lo, hi)) for lo, hi in boxes)
-            assert len(boxes) == (1 if layout == 'one box' else 2) and 0 < total <= run.grid.E.numel() // 3 // 8, boxes
+            assert len(boxes) == (1 if layout in ('one box', 'tall') else 2), boxes
+            assert 0 < total <= run.grid.E.numel() // 3 // 8, boxes
+            interior = run.march2_tables[0][2][0]   # workgroups of the interior variant
+            assert (interior > 0) == (layout == 'tall'), interior
             assert run.snap_e.shape[0] == total
     reference = results['split']
     assert np.abs(reference['traces'].numpy()).max() > 0 and reference['planes'][0].abs().max() > 0
